@@ -1,0 +1,1262 @@
+/*
+ * CPU oracle for the Monte Carlo path engine -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain C11, FP64 throughout, built with -ffp-contract=off so every expression
+ * rounds exactly as the reference's x86-64 build does (no FMA contraction).
+ * Only the tests, the smoke check and bench.py's cpu_baseline leg load it, and
+ * only as the checker / CPU baseline; the product library never calls it.
+ *
+ * Each function restates the reference algorithm it names (file:line in the
+ * reference tree). Expressions keep the reference's evaluation order so that,
+ * with the same chunk count C, prices reproduce the reference bit for bit.
+ */
+#define _GNU_SOURCE
+#include "mcd_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10: src/core/rng.cpp:9-47                                      */
+/* ------------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; ++round) { /* rng.cpp:41-47 */
+        const uint64_t p0 = (uint64_t)PHILOX_M0 * c0; /* rng.cpp:26-30 */
+        const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += PHILOX_W0; /* bump_key, rng.cpp:32 */
+        k1 += PHILOX_W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* key = {seed_lo, seed_hi} (rng.cpp:49-51); counter = {path_lo, path_hi, draw, stream}
+ * (rng.cpp:53-57); only output word 0 is consumed (rng.cpp:64,73). */
+uint32_t orc_philox_bits(uint64_t seed, uint64_t path, uint32_t draw, uint32_t stream) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)path, (uint32_t)(path >> 32), draw, stream};
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return out[0];
+}
+
+void orc_philox_bits_range(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw,
+                           uint32_t stream, uint32_t* out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = orc_philox_bits(seed, path0 + i, draw, stream);
+}
+
+/* rng.cpp:34-37 */
+double orc_uniform_open01(uint32_t bits) { return ((double)bits + 0.5) / 4294967296.0; }
+
+/* ------------------------------------------------------------------------ */
+/* Normal distribution: src/core/normal.cpp:8-82                             */
+/* ------------------------------------------------------------------------ */
+static const double kInvSqrtPi = 0.564189583547756286948079451560772586;
+static const double kSqrt2 = 1.41421356237309504880168872420969808;
+static const double kPi = 3.14159265358979323846264338327950288;
+
+double orc_normal_pdf(double x) { /* normal.cpp:8-11 */
+    const double inv_sqrt_2pi = kInvSqrtPi / kSqrt2;
+    return inv_sqrt_2pi * exp(-0.5 * x * x);
+}
+
+double orc_normal_cdf(double x) { return 0.5 * erfc(-x / kSqrt2); } /* normal.cpp:13-15 */
+
+/* Acklam coefficients: normal.cpp:22-48 (published constants of Acklam's algorithm). */
+static const double A1 = -3.969683028665376e+01, A2 = 2.209460984245205e+02,
+                    A3 = -2.759285104469687e+02, A4 = 1.383577518672690e+02,
+                    A5 = -3.066479806614716e+01, A6 = 2.506628277459239e+00;
+static const double B1 = -5.447609879822406e+01, B2 = 1.615858368580409e+02,
+                    B3 = -1.556989798598866e+02, B4 = 6.680131188771972e+01,
+                    B5 = -1.328068155288572e+01;
+static const double C1 = -7.784894002430293e-03, C2 = -3.223964580411365e-01,
+                    C3 = -2.400758277161838e+00, C4 = -2.549732539343734e+00,
+                    C5 = 4.374664141464968e+00, C6 = 2.938163982698783e+00;
+static const double D1 = 7.784695709041462e-03, D2 = 3.224671290700398e-01,
+                    D3 = 2.445134137142996e+00, D4 = 3.754408661907416e+00;
+static const double kLowBreak = 0.02425;
+
+static double acklam_tail(double u) { /* normal.cpp:50-57 */
+    const int lower = u < kLowBreak;
+    const double z = sqrt(-2.0 * log(lower ? u : 1.0 - u));
+    const double num = (((((C1 * z + C2) * z + C3) * z + C4) * z + C5) * z + C6);
+    const double den = ((((D1 * z + D2) * z + D3) * z + D4) * z + 1.0);
+    const double r = num / den;
+    return lower ? r : -r;
+}
+
+static double acklam_central(double u) { /* normal.cpp:59-66 */
+    const double z = u - 0.5;
+    const double r = z * z;
+    const double num = (((((A1 * r + A2) * r + A3) * r + A4) * r + A5) * r + A6) * z;
+    const double den = (((((B1 * r + B2) * r + B3) * r + B4) * r + B5) * r + 1.0);
+    return num / den;
+}
+
+double orc_acklam(double u) {
+    const double high_break = 1.0 - kLowBreak;
+    return (u < kLowBreak || high_break < u) ? acklam_tail(u) : acklam_central(u);
+}
+
+double orc_inverse_normal_cdf(double u) { /* normal.cpp:70-82 */
+    double z = orc_acklam(u);
+    const double error = orc_normal_cdf(z) - u;
+    const double sqrt_2pi = sqrt(2.0 * kPi);
+    const double r = error * sqrt_2pi * exp(0.5 * z * z);
+    z -= r / (1.0 + 0.5 * z * r);
+    return z;
+}
+
+double orc_standard_normal_variate(uint64_t seed, uint64_t path, uint32_t draw) {
+    return orc_inverse_normal_cdf(orc_uniform_open01(orc_philox_bits(seed, path, draw, 0)));
+}
+
+double orc_uniform_variate(uint64_t seed, uint64_t path, uint32_t draw) {
+    return orc_uniform_open01(orc_philox_bits(seed, path, draw, 1));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Welford + Chan merge: include/mcd/core/stats.hpp:21-71                    */
+/* ------------------------------------------------------------------------ */
+void orc_welford_add(orc_welford* w, double x) { /* stats.hpp:23-29 */
+    ++w->n;
+    const double delta = x - w->mean;
+    w->mean += delta / (double)w->n;
+    const double delta2 = x - w->mean;
+    w->m2 += delta * delta2;
+}
+
+void orc_welford_merge(orc_welford* w, const orc_welford* o) { /* stats.hpp:47-65 */
+    if (o->n == 0) return;
+    if (w->n == 0) { *w = *o; return; }
+    const uint64_t nc = w->n + o->n;
+    const double delta = o->mean - w->mean;
+    const double ow = (double)o->n / (double)nc;
+    const double mean = w->mean + delta * ow;
+    const double m2 = w->m2 + o->m2 + delta * delta * (double)w->n * (double)o->n / (double)nc;
+    w->n = nc;
+    w->mean = mean;
+    w->m2 = m2;
+}
+
+double orc_welford_variance(const orc_welford* w) { /* stats.hpp:35-37 */
+    return w->n < 2 ? 0.0 : w->m2 / (double)(w->n - 1);
+}
+
+double orc_welford_se(const orc_welford* w) { /* stats.hpp:39-41 */
+    return w->n == 0 ? 0.0 : sqrt(orc_welford_variance(w) / (double)w->n);
+}
+
+/* src/core/thread_pool.cpp:9-18 */
+void orc_chunk_bounds(unsigned chunk, uint64_t total, unsigned chunks, uint64_t* begin,
+                      uint64_t* end) {
+    const uint64_t base = total / chunks;
+    const uint64_t rem = total % chunks;
+    const uint64_t b = (uint64_t)chunk * base + ((uint64_t)chunk < rem ? chunk : rem);
+    *begin = b;
+    *end = b + base + ((uint64_t)chunk < rem ? 1 : 0);
+}
+
+/* std::max / std::min exactly: max(a,b) = (a<b)?b:a, min(a,b) = (b<a)?b:a. */
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+/* ------------------------------------------------------------------------ */
+/* GBM exact step: src/models/gbm.cpp:7-12                                   */
+/* ------------------------------------------------------------------------ */
+double orc_gbm_terminal_spot(double spot, double rate, double carry, double vol, double time,
+                             double z) {
+    const double drift = (rate - carry - 0.5 * vol * vol) * time;
+    const double diffusion = vol * sqrt(time) * z;
+    return spot * exp(drift + diffusion);
+}
+
+/* Kemna-Vorst (src/pricers/analytic.cpp:51-59) is needed by the control variate. */
+double orc_kemna_vorst(double s, double k, double r, double q, double v, double t, int type);
+
+/* One pass of one path with draws negated or not; returns the discounted payoff.
+ * Restates monte_carlo_terminal (monte_carlo.cpp:136-149), monte_carlo_path_dependent
+ * (:155-177), monte_carlo_barrier_brownian_bridge (:184-230), the control-variate lambda
+ * (:276-303) and the payoff structs in include/mcd/payoffs/. */
+static double path_once(const orc_product* p, uint64_t seed, uint64_t path, int negate,
+                        double kv_price) {
+    const double phi = p->type == 0 ? 1.0 : -1.0;
+    const double discount = exp(-p->rate * p->time);
+    if (p->product == ORC_EUROPEAN || p->product == ORC_DIGITAL) {
+        const double z = orc_standard_normal_variate(seed, path, 0);
+        const double st = orc_gbm_terminal_spot(p->spot, p->rate, p->carry_yield, p->vol,
+                                                p->time, negate ? -z : z);
+        double pay;
+        if (p->product == ORC_EUROPEAN) { /* european.hpp:19-22 */
+            pay = smax(phi * (st - p->strike), 0.0);
+        } else { /* digital.hpp:16-23 */
+            const int itm = phi * (st - p->strike) > 0.0;
+            pay = !itm ? 0.0 : (p->digital_style == 0 ? p->cash_amount : st);
+        }
+        return discount * pay;
+    }
+    const int m = p->monitoring_points;
+    const double dt = p->time / (double)m;
+    double s = p->spot;
+    if (p->product == ORC_BARRIER && p->brownian_bridge) { /* monte_carlo.cpp:184-230 */
+        const double log_barrier = log(p->barrier);
+        int breached = 0;
+        for (int step = 0; step < m; ++step) {
+            const double z = orc_standard_normal_variate(seed, path, (uint32_t)step);
+            const double sn = orc_gbm_terminal_spot(s, p->rate, p->carry_yield, p->vol, dt,
+                                                    negate ? -z : z);
+            const int crossed = p->direction == 0 ? sn >= p->barrier : sn <= p->barrier;
+            breached = breached || crossed;
+            if (!breached) {
+                const double ls = log(s);
+                const double lsn = log(sn);
+                const double pc = exp(-2.0 * (log_barrier - ls) * (log_barrier - lsn) /
+                                      (p->vol * p->vol * dt));
+                const double u = orc_uniform_variate(seed, path, (uint32_t)step);
+                breached = u < pc;
+            }
+            s = sn;
+        }
+        const double vanilla = smax(phi * (s - p->strike), 0.0);
+        const int pays = p->knock == 1 ? !breached : breached;
+        return discount * (pays ? vanilla : p->rebate);
+    }
+    /* Path-dependent state, one struct's worth per product (the payoffs headers). */
+    double sum = 0.0, log_sum = 0.0, last = 0.0;
+    unsigned count = 0;
+    int breached = 0;
+    double run_max = p->spot, run_min = p->spot;
+    if (p->product == ORC_LOOKBACK && p->strike_style == 1) last = p->spot; /* lookback.hpp:44-46 */
+    const int cv = p->product == ORC_ASIAN && p->average_style == 0 && p->control_variate &&
+                   p->strike_style == 0;
+    for (int step = 0; step < m; ++step) {
+        const double z = orc_standard_normal_variate(seed, path, (uint32_t)step);
+        s = orc_gbm_terminal_spot(s, p->rate, p->carry_yield, p->vol, dt, negate ? -z : z);
+        switch (p->product) {
+        case ORC_ASIAN:
+            if (p->average_style == 0 || cv) sum += s; /* asian.hpp:23-27 */
+            if (p->average_style == 1 || cv) log_sum += log(s); /* asian.hpp:52-56 */
+            last = s;
+            ++count;
+            break;
+        case ORC_BARRIER: { /* barrier.hpp:25-30 */
+            const int crossed = p->direction == 0 ? s >= p->barrier : s <= p->barrier;
+            breached = breached || crossed;
+            last = s;
+            break;
+        }
+        default: /* lookback.hpp:27-30, 48-52 */
+            run_max = smax(run_max, s);
+            run_min = smin(run_min, s);
+            last = s;
+            break;
+        }
+    }
+    switch (p->product) {
+    case ORC_ASIAN: {
+        if (cv) { /* monte_carlo.cpp:276-303 */
+            const double a_avg = sum / (double)count;
+            const double a_pay = smax(phi * (a_avg - p->strike), 0.0);
+            const double g_avg = exp(log_sum / (double)count);
+            const double g_pay = smax(phi * (g_avg - p->strike), 0.0);
+            const double ap = discount * a_pay;
+            const double gp = discount * g_pay;
+            return ap - (gp - kv_price);
+        }
+        const double avg = p->average_style == 0 ? sum / (double)count
+                                                 : exp(log_sum / (double)count);
+        const double ref = p->strike_style == 0 ? (avg - p->strike) : (last - avg);
+        return discount * smax(phi * ref, 0.0); /* asian.hpp:29-35, 58-64 */
+    }
+    case ORC_BARRIER: { /* barrier.hpp:32-37 */
+        const double vanilla = smax(phi * (last - p->strike), 0.0);
+        const int pays = p->knock == 1 ? !breached : breached;
+        return discount * (pays ? vanilla : p->rebate);
+    }
+    default:
+        if (p->strike_style == 0) { /* lookback.hpp:32-35 */
+            const double r = p->type == 0 ? smax(run_max - p->strike, 0.0)
+                                          : smax(p->strike - run_min, 0.0);
+            return discount * r;
+        }
+        /* lookback.hpp:54-56 */
+        return discount * (p->type == 0 ? (last - run_min) : (run_max - last));
+    }
+}
+
+static double kv_for(const orc_product* p) {
+    if (p->product == ORC_ASIAN && p->average_style == 0 && p->control_variate &&
+        p->strike_style == 0)
+        return orc_kemna_vorst(p->spot, p->strike, p->rate, p->carry_yield, p->vol, p->time,
+                               p->type);
+    return 0.0;
+}
+
+/* accumulate_paths sample_at (monte_carlo.cpp:73-80). */
+static double sample_at(const orc_product* p, uint64_t seed, uint64_t path, double kv) {
+    double s = path_once(p, seed, path, 0, kv);
+    if (p->antithetic) {
+        const double s2 = path_once(p, seed, path, 1, kv);
+        s = 0.5 * (s + s2);
+    }
+    return s;
+}
+
+double orc_path_sample(const orc_product* p, uint64_t seed, uint64_t path) {
+    return sample_at(p, seed, path, kv_for(p));
+}
+
+void orc_path_samples(const orc_product* p, uint64_t seed, uint64_t path0, uint64_t n,
+                      double* out) {
+    const double kv = kv_for(p);
+    for (uint64_t i = 0; i < n; ++i) out[i] = sample_at(p, seed, path0 + i, kv);
+}
+
+/* run_chunked_and_merge (monte_carlo.cpp:42-67) with an explicit chunk count. */
+typedef struct chunk_job {
+    const orc_product* p;
+    uint64_t seed, total;
+    unsigned chunks, next, stride;
+    double kv;
+    orc_welford* per_chunk;
+} chunk_job;
+
+static void* chunk_worker(void* arg) {
+    chunk_job* job = (chunk_job*)arg;
+    for (unsigned c = job->next; c < job->chunks; c += job->stride) {
+        uint64_t b, e;
+        orc_chunk_bounds(c, job->total, job->chunks, &b, &e);
+        orc_welford local = {0, 0.0, 0.0};
+        for (uint64_t i = b; i < e; ++i) orc_welford_add(&local, sample_at(job->p, job->seed, i, job->kv));
+        job->per_chunk[c] = local;
+    }
+    return NULL;
+}
+
+orc_result orc_price(const orc_product* p, uint64_t path_count, uint64_t seed, unsigned chunks,
+                     unsigned threads) {
+    if (chunks == 0) chunks = 1;
+    if (threads == 0) threads = 1;
+    if (threads > chunks) threads = chunks;
+    orc_welford* per = (orc_welford*)calloc(chunks, sizeof(orc_welford));
+    chunk_job* jobs = (chunk_job*)calloc(threads, sizeof(chunk_job));
+    pthread_t* tids = (pthread_t*)calloc(threads, sizeof(pthread_t));
+    const double kv = kv_for(p);
+    for (unsigned t = 0; t < threads; ++t) {
+        jobs[t] = (chunk_job){p, seed, path_count, chunks, t, threads, kv, per};
+        if (t > 0) pthread_create(&tids[t], NULL, chunk_worker, &jobs[t]);
+    }
+    chunk_worker(&jobs[0]);
+    for (unsigned t = 1; t < threads; ++t) pthread_join(tids[t], NULL);
+    orc_welford merged = {0, 0.0, 0.0};
+    for (unsigned c = 0; c < chunks; ++c) orc_welford_merge(&merged, &per[c]);
+    free(per);
+    free(jobs);
+    free(tids);
+    orc_result r = {merged.mean, orc_welford_se(&merged), merged.n};
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sobol: src/core/sobol.cpp:9-97 (dims 0-6), plus the documented extension  */
+/* ------------------------------------------------------------------------ */
+#define SOBOL_REF_DIMS 7
+#define SOBOL_EXT_DIMS 1024
+
+/* sobol.cpp:22-30: degree and interior-coefficient bits (bit j-1 = a_j). */
+static const unsigned kRefDeg[SOBOL_REF_DIMS] = {0, 1, 2, 3, 4, 5, 6};
+static const uint32_t kRefCoef[SOBOL_REF_DIMS] = {0u, 0u, 1u, 1u, 1u, 2u, 1u};
+
+static uint32_t sobol_v[SOBOL_EXT_DIMS][33];
+static pthread_once_t sobol_once = PTHREAD_ONCE_INIT;
+
+/* LFSR maximal-period test of x^s + sum_j a_j x^j + 1 (the method sobol.cpp:20-21 names). */
+static int is_primitive(unsigned s, uint32_t interior) {
+    const uint32_t poly = (1u << s) | (interior << 1) | 1u;
+    uint32_t state = 1;
+    const uint32_t period = (1u << s) - 1;
+    for (uint32_t i = 1; i <= period; ++i) {
+        state <<= 1;
+        if (state & (1u << s)) state ^= poly;
+        if (state == 1) return i == period;
+    }
+    return 0;
+}
+
+/* Bratley-Fox recurrence exactly as sobol.cpp:40-68, with the initial m_1..m_s given. */
+static void direction_numbers(unsigned s, uint32_t interior, const uint32_t* m_init,
+                              uint32_t* v) {
+    uint32_t m[33] = {0};
+    if (s == 0) {
+        for (unsigned k = 1; k <= 32; ++k) m[k] = 1;
+    } else {
+        for (unsigned k = 1; k <= s; ++k) m[k] = m_init[k - 1];
+        for (unsigned k = s + 1; k <= 32; ++k) {
+            uint32_t mk = m[k - s] ^ (m[k - s] << s);
+            for (unsigned j = 1; j < s; ++j)
+                if ((interior >> (j - 1)) & 1u) mk ^= (m[k - j] << j);
+            m[k] = mk;
+        }
+    }
+    v[0] = 0;
+    for (unsigned k = 1; k <= 32; ++k) v[k] = m[k] << (32 - k);
+}
+
+/* Dims >= 7 (no reference definition): primitive polynomials in order of (degree,
+ * interior bits), skipping the six used by dims 1-6; initial m_k odd and < 2^k, drawn from
+ * Philox key 0x50B01 at counter {dim, 0, k, 3}. */
+static void sobol_init(void) {
+    const uint32_t ones[32] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                               1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    for (unsigned d = 0; d < SOBOL_REF_DIMS; ++d)
+        direction_numbers(kRefDeg[d], kRefCoef[d], ones, sobol_v[d]);
+    unsigned d = SOBOL_REF_DIMS;
+    for (unsigned s = 1; s <= 16 && d < SOBOL_EXT_DIMS; ++s) {
+        for (uint32_t c = 0; c < (s > 1 ? (1u << (s - 1)) : 1u) && d < SOBOL_EXT_DIMS; ++c) {
+            if (!is_primitive(s, c)) continue;
+            int used = 0;
+            for (unsigned r = 1; r < SOBOL_REF_DIMS; ++r)
+                if (kRefDeg[r] == s && kRefCoef[r] == c) used = 1;
+            if (used) continue;
+            uint32_t m_init[32];
+            for (unsigned k = 1; k <= s; ++k) {
+                const uint32_t bits = orc_philox_bits(0x50B01ull, d, k, 3);
+                m_init[k - 1] = (k >= 32 ? bits : (bits & ((1u << k) - 1u))) | 1u;
+            }
+            direction_numbers(s, c, m_init, sobol_v[d]);
+            ++d;
+        }
+    }
+}
+
+unsigned orc_sobol_max_dims(void) { return SOBOL_EXT_DIMS; }
+
+int orc_sobol_direction_table(unsigned dim, uint32_t* v32) {
+    if (dim >= SOBOL_EXT_DIMS) return -1;
+    pthread_once(&sobol_once, sobol_init);
+    for (unsigned k = 0; k < 32; ++k) v32[k] = sobol_v[dim][k + 1];
+    return 0;
+}
+
+uint32_t orc_sobol_int(unsigned dim, uint64_t index) { /* sobol.cpp:83-97 */
+    pthread_once(&sobol_once, sobol_init);
+    if (index == 0 || dim >= SOBOL_EXT_DIMS) return 0;
+    const uint64_t gray = index ^ (index >> 1);
+    uint32_t x = 0;
+    for (unsigned b = 0; b < 32; ++b)
+        if ((gray >> b) & 1ull) x ^= sobol_v[dim][b + 1];
+    return x;
+}
+
+double orc_sobol_point(unsigned dim, uint64_t index) {
+    if (index == 0) return 0.0;
+    return (double)orc_sobol_int(dim, index) / 4294967296.0;
+}
+
+/* src/models/brownian_bridge_path.cpp:20-77: BFS bisection, any n. */
+void orc_brownian_bridge_path(double spot, double rate, double carry, double vol, double time,
+                              int n, const double* normals, double* spots) {
+    double* t = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    double* w = (double*)calloc((size_t)n, sizeof(double));
+    int* ql = (int*)malloc(sizeof(int) * (size_t)(2 * n + 1));
+    int* qr = (int*)malloc(sizeof(int) * (size_t)(2 * n + 1));
+    for (int i = 0; i <= n; ++i) t[i] = (double)i * time / (double)n;
+    w[n - 1] = sqrt(t[n]) * normals[0];
+    if (n > 1) {
+        size_t head = 0, tail = 0;
+        ql[tail] = -1; qr[tail] = n - 1; ++tail;
+        int draw = 1;
+        while (head < tail && draw < n) {
+            const int l = ql[head], r = qr[head];
+            ++head;
+            if (r - l <= 1) continue;
+            const int mid = (l + r) / 2;
+            const double tl = l == -1 ? 0.0 : t[l + 1];
+            const double wl = l == -1 ? 0.0 : w[l];
+            const double tr = t[r + 1];
+            const double wr = w[r];
+            const double tm = t[mid + 1];
+            const double mean = wl + (tm - tl) / (tr - tl) * (wr - wl);
+            const double var = (tm - tl) * (tr - tm) / (tr - tl);
+            w[mid] = mean + sqrt(var) * normals[draw];
+            ++draw;
+            ql[tail] = l; qr[tail] = mid; ++tail;
+            ql[tail] = mid; qr[tail] = r; ++tail;
+        }
+    }
+    const double drift_rate = rate - carry - 0.5 * vol * vol;
+    for (int i = 0; i < n; ++i) {
+        const double drift = drift_rate * t[i + 1];
+        spots[i] = spot * exp(drift + vol * w[i]);
+    }
+    free(t); free(w); free(ql); free(qr);
+}
+
+/* src/pricers/qmc.cpp:15-30 */
+double orc_qmc_sobol_european(double spot, double strike, double rate, double carry, double vol,
+                              double time, int type, uint64_t path_count) {
+    const double phi = type == 0 ? 1.0 : -1.0;
+    const double discount = exp(-rate * time);
+    double sum = 0.0;
+    for (uint64_t i = 1; i <= path_count; ++i) {
+        const double z = orc_inverse_normal_cdf(orc_sobol_point(0, i));
+        const double st = orc_gbm_terminal_spot(spot, rate, carry, vol, time, z);
+        sum += discount * smax(phi * (st - strike), 0.0);
+    }
+    return sum / (double)path_count;
+}
+
+/* src/pricers/qmc.cpp:32-57, with the dimension cap lifted to the extended table. */
+double orc_qmc_sobol_asian(double spot, double strike, double rate, double carry, double vol,
+                           double time, int type, int strike_style, int m,
+                           uint64_t path_count) {
+    const double phi = type == 0 ? 1.0 : -1.0;
+    const double discount = exp(-rate * time);
+    double* z = (double*)malloc(sizeof(double) * (size_t)m);
+    double* sp = (double*)malloc(sizeof(double) * (size_t)m);
+    double sum = 0.0;
+    for (uint64_t i = 1; i <= path_count; ++i) {
+        for (int d = 0; d < m; ++d) z[d] = orc_inverse_normal_cdf(orc_sobol_point((unsigned)d, i));
+        orc_brownian_bridge_path(spot, rate, carry, vol, time, m, z, sp);
+        double s = 0.0, last = 0.0;
+        for (int d = 0; d < m; ++d) { s += sp[d]; last = sp[d]; }
+        const double avg = s / (double)m;
+        const double ref = strike_style == 0 ? (avg - strike) : (last - avg);
+        sum += discount * smax(phi * ref, 0.0);
+    }
+    free(z);
+    free(sp);
+    return sum / (double)path_count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Heston: src/pricers/heston.cpp:26-156                                     */
+/* ------------------------------------------------------------------------ */
+double orc_qe_variance_step(double v, double kappa, double theta, double xi, double dt,
+                            double u) { /* heston.cpp:91-116 */
+    const double e = exp(-kappa * dt);
+    const double m = theta + (v - theta) * e;
+    const double s2 = v * xi * xi * e / kappa * (1.0 - e) +
+                      theta * xi * xi / (2.0 * kappa) * (1.0 - e) * (1.0 - e);
+    const double psi = s2 / (m * m);
+    if (psi <= 1.5) {
+        const double inv_psi = 1.0 / psi;
+        const double b2 = 2.0 * inv_psi - 1.0 + sqrt(2.0 * inv_psi) * sqrt(2.0 * inv_psi - 1.0);
+        const double a = m / (1.0 + b2);
+        const double zv = orc_inverse_normal_cdf(u);
+        const double b = sqrt(b2);
+        return a * (b + zv) * (b + zv);
+    }
+    const double p = (psi - 1.0) / (psi + 1.0);
+    const double beta = (1.0 - p) / m;
+    if (u <= p) return 0.0;
+    return log((1.0 - p) / (1.0 - u)) / beta;
+}
+
+typedef struct heston_k {
+    double dt, drift, k0, k1, k2, k3, k4, discount, log_s0;
+} heston_k;
+
+static heston_k heston_consts(const orc_heston* h, int steps) { /* heston.cpp:123-132 */
+    heston_k k;
+    k.dt = h->time / (double)steps;
+    k.drift = (h->rate - h->carry_yield) * k.dt;
+    k.k0 = -h->rho * h->kappa * h->theta * k.dt / h->xi;
+    k.k1 = 0.5 * k.dt * (h->kappa * h->rho / h->xi - 0.5) - h->rho / h->xi;
+    k.k2 = 0.5 * k.dt * (h->kappa * h->rho / h->xi - 0.5) + h->rho / h->xi;
+    k.k3 = 0.5 * k.dt * (1.0 - h->rho * h->rho);
+    k.k4 = k.k3;
+    k.discount = exp(-h->rate * h->time);
+    k.log_s0 = log(h->spot);
+    return k;
+}
+
+static double heston_terminal(const orc_heston* h, const heston_k* k, int steps, uint64_t seed,
+                              uint64_t path) { /* heston.cpp:136-148 */
+    double v = h->v0;
+    double log_s = k->log_s0;
+    for (int step = 0; step < steps; ++step) {
+        const double zs = orc_standard_normal_variate(seed, path, (uint32_t)step);
+        const double u = orc_uniform_variate(seed, path, (uint32_t)step);
+        const double vn = orc_qe_variance_step(v, h->kappa, h->theta, h->xi, k->dt, u);
+        const double dv = smax(k->k3 * v + k->k4 * vn, 0.0);
+        log_s += k->drift + k->k0 + k->k1 * v + k->k2 * vn + sqrt(dv) * zs;
+        v = vn;
+    }
+    return exp(log_s);
+}
+
+double orc_heston_path_sample(const orc_heston* h, double strike, int type, int steps,
+                              uint64_t seed, uint64_t path) {
+    const heston_k k = heston_consts(h, steps);
+    const double phi = type == 0 ? 1.0 : -1.0;
+    const double st = heston_terminal(h, &k, steps, seed, path);
+    return k.discount * smax(phi * (st - strike), 0.0);
+}
+
+orc_result orc_heston_qe_european(const orc_heston* h, double strike, int type, int steps,
+                                  uint64_t path_count, uint64_t seed) { /* heston.cpp:120-156 */
+    const heston_k k = heston_consts(h, steps);
+    const double phi = type == 0 ? 1.0 : -1.0;
+    orc_welford acc = {0, 0.0, 0.0};
+    for (uint64_t i = 0; i < path_count; ++i) {
+        const double st = heston_terminal(h, &k, steps, seed, i);
+        orc_welford_add(&acc, k.discount * smax(phi * (st - strike), 0.0));
+    }
+    orc_result r = {acc.mean, orc_welford_se(&acc), acc.n};
+    return r;
+}
+
+/* Many strikes from one path pass, split across threads by contiguous path chunks, each
+ * chunk's Welford merged in chunk order: the CPU-baseline form of the same computation. */
+typedef struct heston_job {
+    const orc_heston* h;
+    const heston_k* k;
+    const double* strikes;
+    int n_strikes, type, steps;
+    uint64_t seed, begin, end;
+    orc_welford* acc;
+} heston_job;
+
+static void* heston_worker(void* arg) {
+    heston_job* j = (heston_job*)arg;
+    const double phi = j->type == 0 ? 1.0 : -1.0;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        const double st = heston_terminal(j->h, j->k, j->steps, j->seed, i);
+        for (int s = 0; s < j->n_strikes; ++s)
+            orc_welford_add(&j->acc[s], j->k->discount * smax(phi * (st - j->strikes[s]), 0.0));
+    }
+    return NULL;
+}
+
+void orc_heston_qe_multi(const orc_heston* h, const double* strikes, int n_strikes, int type,
+                         int steps, uint64_t path_count, uint64_t seed, unsigned threads,
+                         orc_result* out) {
+    if (threads == 0) threads = 1;
+    const heston_k k = heston_consts(h, steps);
+    heston_job* jobs = (heston_job*)calloc(threads, sizeof(heston_job));
+    pthread_t* tids = (pthread_t*)calloc(threads, sizeof(pthread_t));
+    orc_welford* acc = (orc_welford*)calloc((size_t)threads * (size_t)n_strikes, sizeof(orc_welford));
+    for (unsigned t = 0; t < threads; ++t) {
+        uint64_t b, e;
+        orc_chunk_bounds(t, path_count, threads, &b, &e);
+        jobs[t] = (heston_job){h, &k, strikes, n_strikes, type, steps, seed, b, e,
+                               acc + (size_t)t * (size_t)n_strikes};
+        if (t > 0) pthread_create(&tids[t], NULL, heston_worker, &jobs[t]);
+    }
+    heston_worker(&jobs[0]);
+    for (unsigned t = 1; t < threads; ++t) pthread_join(tids[t], NULL);
+    for (int s = 0; s < n_strikes; ++s) {
+        orc_welford m = {0, 0.0, 0.0};
+        for (unsigned t = 0; t < threads; ++t) orc_welford_merge(&m, &acc[(size_t)t * n_strikes + s]);
+        out[s].price = m.mean;
+        out[s].standard_error = orc_welford_se(&m);
+        out[s].path_count = m.n;
+    }
+    free(jobs); free(tids); free(acc);
+}
+
+/* src/core/gauss_legendre.cpp:17-69 */
+static void legendre(int n, double x, double* pn, double* pn1) {
+    double prev = 1.0, curr = x;
+    if (n == 0) { *pn = 1.0; *pn1 = 0.0; return; }
+    for (int k = 2; k <= n; ++k) {
+        const double next = ((2.0 * (double)k - 1.0) * x * curr - ((double)k - 1.0) * prev) / (double)k;
+        prev = curr;
+        curr = next;
+    }
+    *pn = curr;
+    *pn1 = prev;
+}
+
+void orc_gauss_legendre_rule(int order, double* nodes, double* weights) {
+    const int half = (order + 1) / 2;
+    for (int i = 0; i < half; ++i) {
+        double x = cos(kPi * ((double)i + 0.75) / ((double)order + 0.5));
+        double pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double pn, pn1;
+            legendre(order, x, &pn, &pn1);
+            pp = (double)order * (x * pn - pn1) / (x * x - 1.0);
+            const double dx = pn / pp;
+            x -= dx;
+            if (fabs(dx) < 1e-15) break;
+        }
+        const double w = 2.0 / ((1.0 - x * x) * pp * pp);
+        nodes[i] = -x;
+        nodes[order - 1 - i] = x;
+        weights[i] = w;
+        weights[order - 1 - i] = w;
+    }
+}
+
+/* Little-Trap characteristic function: heston.cpp:26-40 */
+static double complex heston_cf(const orc_heston* p, double complex u) {
+    const double complex I1 = I;
+    const double complex rxiu = p->rho * p->xi * I1 * u;
+    const double complex bmr = p->kappa - rxiu;
+    const double complex d = csqrt(bmr * bmr + p->xi * p->xi * (I1 * u + u * u));
+    const double complex c = (bmr - d) / (bmr + d);
+    const double complex e = cexp(-d * p->time);
+    const double complex bc = I1 * u * (p->rate - p->carry_yield) * p->time +
+                              (p->kappa * p->theta / (p->xi * p->xi)) *
+                                  ((bmr - d) * p->time - 2.0 * clog((1.0 - c * e) / (1.0 - c)));
+    const double complex bd = (bmr - d) / (p->xi * p->xi) * (1.0 - e) / (1.0 - c * e);
+    return cexp(bc + bd * p->v0);
+}
+
+double orc_heston_semi_analytic(const orc_heston* h, double strike, int type) { /* :48-82 */
+    const double lm = log(strike / h->spot);
+    const double complex phi_ni = heston_cf(h, CMPLX(0.0, -1.0));
+    double nodes[128], weights[128];
+    orc_gauss_legendre_rule(128, nodes, weights);
+    const double a = 0.0, b = 200.0;
+    const double hw = 0.5 * (b - a), mid = 0.5 * (a + b);
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < 128; ++i) {
+        const double u = mid + hw * nodes[i];
+        const double complex v1 =
+            cexp(-I * u * lm) * (heston_cf(h, CMPLX(u, -1.0)) / phi_ni) / (I * u);
+        s1 += weights[i] * creal(v1);
+    }
+    for (int i = 0; i < 128; ++i) {
+        const double u = mid + hw * nodes[i];
+        const double complex v2 = cexp(-I * u * lm) * heston_cf(h, CMPLX(u, 0.0)) / (I * u);
+        s2 += weights[i] * creal(v2);
+    }
+    const double p1 = 0.5 + hw * s1 / kPi;
+    const double p2 = 0.5 + hw * s2 / kPi;
+    const double dq = exp(-h->carry_yield * h->time);
+    const double dr = exp(-h->rate * h->time);
+    const double call = h->spot * dq * p1 - strike * dr * p2;
+    if (type == 0) return call;
+    return call - h->spot * dq + strike * dr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Householder least squares: src/core/linalg.cpp:7-71                       */
+/* ------------------------------------------------------------------------ */
+void orc_householder_ls(double* a, int rows, int cols, double* y, double* beta) {
+    double* v = (double*)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+    for (int k = 0; k < cols; ++k) {
+        double nsq = 0.0;
+        for (int i = k; i < rows; ++i) { const double t = a[i * cols + k]; nsq += t * t; }
+        const double norm = sqrt(nsq);
+        if (norm == 0.0) continue;
+        const double akk = a[k * cols + k];
+        const double alpha = akk >= 0.0 ? -norm : norm;
+        v[0] = akk - alpha;
+        for (int i = k + 1; i < rows; ++i) v[i - k] = a[i * cols + k];
+        double vn = 0.0;
+        for (int i = 0; i < rows - k; ++i) vn += v[i] * v[i];
+        if (vn == 0.0) continue;
+        for (int j = k; j < cols; ++j) {
+            double dot = 0.0;
+            for (int i = k; i < rows; ++i) dot += v[i - k] * a[i * cols + j];
+            const double f = 2.0 * dot / vn;
+            for (int i = k; i < rows; ++i) a[i * cols + j] -= f * v[i - k];
+        }
+        double dy = 0.0;
+        for (int i = k; i < rows; ++i) dy += v[i - k] * y[i];
+        const double fy = 2.0 * dy / vn;
+        for (int i = k; i < rows; ++i) y[i] -= fy * v[i - k];
+    }
+    for (int i = cols - 1; i >= 0; --i) {
+        double sum = y[i];
+        for (int j = i + 1; j < cols; ++j) sum -= a[i * cols + j] * beta[j];
+        const double diag = a[i * cols + i];
+        beta[i] = diag != 0.0 ? sum / diag : 0.0;
+    }
+    free(v);
+}
+
+/* ------------------------------------------------------------------------ */
+/* LSM: src/pricers/lsm.cpp:16-213                                           */
+/* ------------------------------------------------------------------------ */
+static void laguerre(double x, double out[4]) { /* lsm.cpp:21-25 */
+    const double e = exp(-x / 2.0);
+    out[0] = e;
+    out[1] = e * (1.0 - x);
+    out[2] = e * (1.0 - 2.0 * x + 0.5 * x * x);
+    out[3] = e * (1.0 - 3.0 * x + 1.5 * x * x - x * x * x / 6.0);
+}
+
+static double intrinsic(double s, double k, int type) { /* lsm.cpp:27-30 */
+    const double phi = type == 0 ? 1.0 : -1.0;
+    return smax(phi * (s - k), 0.0);
+}
+
+static double* lsm_paths(double spot, double rate, double carry, double vol, double time,
+                         int steps, uint64_t n, uint64_t seed) { /* lsm.cpp:35-58 */
+    const double dt = time / (double)steps;
+    const size_t stride = (size_t)steps + 1;
+    double* paths = (double*)malloc(sizeof(double) * (size_t)n * stride);
+    for (uint64_t i = 0; i < n; ++i) {
+        double s = spot;
+        const size_t row = (size_t)i * stride;
+        paths[row] = s;
+        for (int k = 0; k < steps; ++k) {
+            const double z = orc_standard_normal_variate(seed, i, (uint32_t)k);
+            s = orc_gbm_terminal_spot(s, rate, carry, vol, dt, z);
+            paths[row + (size_t)k + 1] = s;
+        }
+    }
+    return paths;
+}
+
+orc_result orc_lsm_american(double spot, double strike, double rate, double carry, double vol,
+                            double time, int type, int steps, uint64_t n, uint64_t seed,
+                            double* coeffs, uint8_t* has_coeffs,
+                            uint8_t* exercise_at_inception) { /* lsm.cpp:62-149 */
+    const double dt = time / (double)steps;
+    const double disc = exp(-rate * dt);
+    const size_t stride = (size_t)steps + 1;
+    double* paths = lsm_paths(spot, rate, carry, vol, time, steps, n, seed);
+    double* cf = (double*)malloc(sizeof(double) * (size_t)n);
+    double* design = (double*)malloc(sizeof(double) * (size_t)n * 4);
+    double* targets = (double*)malloc(sizeof(double) * (size_t)n);
+    uint64_t* itm = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
+    for (size_t s = 0; s < stride; ++s) has_coeffs[s] = 0;
+    for (uint64_t i = 0; i < n; ++i) cf[i] = intrinsic(paths[i * stride + (size_t)steps], strike, type);
+    for (int step = steps - 1; step >= 1; --step) {
+        for (uint64_t i = 0; i < n; ++i) cf[i] *= disc;
+        uint64_t cnt = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const double s = paths[i * stride + (size_t)step];
+            if (intrinsic(s, strike, type) > 0.0) {
+                laguerre(s / strike, &design[cnt * 4]);
+                targets[cnt] = cf[i];
+                itm[cnt] = i;
+                ++cnt;
+            }
+        }
+        if (cnt < 4) continue;
+        double beta[4] = {0, 0, 0, 0};
+        orc_householder_ls(design, (int)cnt, 4, targets, beta);
+        for (int b = 0; b < 4; ++b) coeffs[(size_t)step * 4 + (size_t)b] = beta[b];
+        has_coeffs[step] = 1;
+        for (uint64_t j = 0; j < cnt; ++j) {
+            const uint64_t i = itm[j];
+            const double s = paths[i * stride + (size_t)step];
+            double basis[4];
+            laguerre(s / strike, basis);
+            double cont = 0.0;
+            for (int b = 0; b < 4; ++b) cont += beta[b] * basis[b];
+            const double ex = intrinsic(s, strike, type);
+            if (ex > cont) cf[i] = ex;
+        }
+    }
+    for (uint64_t i = 0; i < n; ++i) cf[i] *= disc;
+    orc_welford acc = {0, 0.0, 0.0};
+    for (uint64_t i = 0; i < n; ++i) orc_welford_add(&acc, cf[i]);
+    free(paths); free(cf); free(design); free(targets); free(itm);
+    const double inc = intrinsic(spot, strike, type);
+    orc_result r;
+    r.path_count = n;
+    if (inc > acc.mean) {
+        *exercise_at_inception = 1;
+        r.price = inc;
+        r.standard_error = 0.0;
+        return r;
+    }
+    *exercise_at_inception = 0;
+    r.price = acc.mean;
+    r.standard_error = orc_welford_se(&acc);
+    return r;
+}
+
+double orc_lsm_reprice_frozen(double spot, double strike, double rate, double carry, double vol,
+                              double time, int type, int steps, uint64_t n, uint64_t seed,
+                              const double* coeffs, const uint8_t* has_coeffs,
+                              uint8_t exercise_at_inception) { /* lsm.cpp:151-213 */
+    const double dt = time / (double)steps;
+    const double disc = exp(-rate * dt);
+    const size_t stride = (size_t)steps + 1;
+    double* paths = lsm_paths(spot, rate, carry, vol, time, steps, n, seed);
+    double* cf = (double*)malloc(sizeof(double) * (size_t)n);
+    for (uint64_t i = 0; i < n; ++i) cf[i] = intrinsic(paths[i * stride + (size_t)steps], strike, type);
+    for (int step = steps - 1; step >= 1; --step) {
+        for (uint64_t i = 0; i < n; ++i) cf[i] *= disc;
+        if (!has_coeffs[step]) continue;
+        const double* beta = &coeffs[(size_t)step * 4];
+        for (uint64_t i = 0; i < n; ++i) {
+            const double s = paths[i * stride + (size_t)step];
+            const double ex = intrinsic(s, strike, type);
+            if (ex <= 0.0) continue;
+            double basis[4];
+            laguerre(s / strike, basis);
+            double cont = 0.0;
+            for (int b = 0; b < 4; ++b) cont += beta[b] * basis[b];
+            if (ex > cont) cf[i] = ex;
+        }
+    }
+    for (uint64_t i = 0; i < n; ++i) cf[i] *= disc;
+    orc_welford acc = {0, 0.0, 0.0};
+    for (uint64_t i = 0; i < n; ++i) orc_welford_add(&acc, cf[i]);
+    free(paths); free(cf);
+    if (exercise_at_inception) return intrinsic(spot, strike, type);
+    return acc.mean;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Analytic: src/pricers/analytic.cpp:12-235                                 */
+/* ------------------------------------------------------------------------ */
+static double sgn(int type) { return type == 0 ? 1.0 : -1.0; }
+
+static void d1d2(double s, double k, double r, double q, double v, double t, double* d1,
+                 double* d2) { /* analytic.cpp:19-26 */
+    const double b = r - q;
+    const double st = sqrt(t);
+    *d1 = (log(s / k) + (b + 0.5 * v * v) * t) / (v * st);
+    *d2 = *d1 - v * st;
+}
+
+double orc_bsm(double s, double k, double r, double q, double v, double t, int type) { /* :30-38 */
+    double d1, d2;
+    d1d2(s, k, r, q, v, t, &d1, &d2);
+    const double phi = sgn(type);
+    const double dq = exp(-q * t);
+    const double dr = exp(-r * t);
+    return phi * s * dq * orc_normal_cdf(phi * d1) - phi * k * dr * orc_normal_cdf(phi * d2);
+}
+
+double orc_black76(double f, double k, double r, double v, double t, int type) { /* :40-49 */
+    const double st = sqrt(t);
+    const double d1 = (log(f / k) + 0.5 * v * v * t) / (v * st);
+    const double d2 = d1 - v * st;
+    const double phi = sgn(type);
+    const double df = exp(-r * t);
+    return phi * df * (f * orc_normal_cdf(phi * d1) - k * orc_normal_cdf(phi * d2));
+}
+
+double orc_kemna_vorst(double s, double k, double r, double q, double v, double t, int type) {
+    const double sigma_g = v / 1.73205080756887729352744634150587237; /* :51-59 */
+    const double b_g = 0.5 * (r - q - v * v / 6.0);
+    const double q_eff = r - b_g;
+    return orc_bsm(s, k, r, q_eff, sigma_g, t, type);
+}
+
+double orc_reiner_rubinstein(double s, double k, double h, double r, double q, double v,
+                             double t, int type, int direction, int knock, double rebate) {
+    /* analytic.cpp:61-132 */
+    const double b = r - q;
+    const double st = sqrt(t);
+    const double mu = (b - 0.5 * v * v) / (v * v);
+    const double lambda = sqrt(mu * mu + 2.0 * r / (v * v));
+    const double phi = sgn(type);
+    const double eta = direction == 1 ? 1.0 : -1.0;
+    const double x1 = log(s / k) / (v * st) + (1.0 + mu) * v * st;
+    const double x2 = log(s / h) / (v * st) + (1.0 + mu) * v * st;
+    const double y1 = log(h * h / (s * k)) / (v * st) + (1.0 + mu) * v * st;
+    const double y2 = log(h / s) / (v * st) + (1.0 + mu) * v * st;
+    const double z = log(h / s) / (v * st) + lambda * v * st;
+    const double dq = exp((b - r) * t);
+    const double dr = exp(-r * t);
+    const double hs1 = pow(h / s, 2.0 * (mu + 1.0));
+    const double hs0 = pow(h / s, 2.0 * mu);
+    const double A = phi * s * dq * orc_normal_cdf(phi * x1) -
+                     phi * k * dr * orc_normal_cdf(phi * x1 - phi * v * st);
+    const double B = phi * s * dq * orc_normal_cdf(phi * x2) -
+                     phi * k * dr * orc_normal_cdf(phi * x2 - phi * v * st);
+    const double C = phi * s * dq * hs1 * orc_normal_cdf(eta * y1) -
+                     phi * k * dr * hs0 * orc_normal_cdf(eta * y1 - eta * v * st);
+    const double D = phi * s * dq * hs1 * orc_normal_cdf(eta * y2) -
+                     phi * k * dr * hs0 * orc_normal_cdf(eta * y2 - eta * v * st);
+    const double E = rebate * dr *
+                     (orc_normal_cdf(eta * x2 - eta * v * st) -
+                      hs0 * orc_normal_cdf(eta * y2 - eta * v * st));
+    const double F = rebate * (pow(h / s, mu + lambda) * orc_normal_cdf(eta * z) +
+                               pow(h / s, mu - lambda) *
+                                   orc_normal_cdf(eta * z - 2.0 * eta * lambda * v * st));
+    const int kab = k > h;
+    if (knock == 0) {
+        if (direction == 1) {
+            if (type == 0) return kab ? (C + E) : (A - B + D + E);
+            return kab ? (B - C + D + E) : (A + E);
+        }
+        if (type == 0) return kab ? (A + E) : (B - C + D + E);
+        return kab ? (A - B + D + E) : (C + E);
+    }
+    if (direction == 1) {
+        if (type == 0) return kab ? (A - C + F) : (B - D + F);
+        return kab ? (A - B + C - D + F) : F;
+    }
+    if (type == 0) return kab ? F : (A - B + C - D + F);
+    return kab ? (B - D + F) : (A - C + F);
+}
+
+static void lb_core(double s, double ref, double r, double q, double v, double t, double* lambda,
+                    double* d1, double* sd) { /* analytic.cpp:146-158 */
+    *lambda = 2.0 * (r - q) / (v * v);
+    *sd = v * sqrt(t);
+    *d1 = log(s / ref) / *sd + 0.5 * (*lambda + 1.0) * *sd;
+}
+
+static double lb_fixed_ab(double s, double ref, double r, double q, double v, double t,
+                          double eta) { /* analytic.cpp:163-175 */
+    double lambda, d1, sd;
+    lb_core(s, ref, r, q, v, t, &lambda, &d1, &sd);
+    const double n1 = orc_normal_cdf(eta * d1);
+    const double n2 = orc_normal_cdf(eta * (d1 - sd));
+    const double n3 = orc_normal_cdf(eta * (d1 - lambda * sd));
+    const double n4 = orc_normal_cdf(eta * d1);
+    const double pr = pow(s / ref, -lambda);
+    const double dq = exp(-q * t);
+    const double dr = exp(-r * t);
+    return eta * (s * dq * n1 - ref * dr * n2 - s * dr * (pr * n3 - dq * n4 / dr) / lambda);
+}
+
+double orc_lookback_fixed(double s, double k, double r, double q, double v, double t, int type) {
+    const double ext = s; /* analytic.cpp:184-195 */
+    const double eta = type == 0 ? 1.0 : -1.0;
+    const int guaranteed = type == 0 ? k <= ext : k >= ext;
+    if (guaranteed)
+        return lb_fixed_ab(s, ext, r, q, v, t, eta) + eta * exp(-r * t) * (ext - k);
+    return lb_fixed_ab(s, k, r, q, v, t, eta);
+}
+
+double orc_lookback_floating(double s, double r, double q, double v, double t, int type) {
+    const double ext = s; /* analytic.cpp:197-212 */
+    const double eta = type == 0 ? 1.0 : -1.0;
+    double lambda, d1, sd;
+    lb_core(s, ext, r, q, v, t, &lambda, &d1, &sd);
+    const double n1 = orc_normal_cdf(eta * d1);
+    const double n2 = orc_normal_cdf(eta * (d1 - sd));
+    const double n3 = orc_normal_cdf(eta * (-d1 + lambda * sd));
+    const double n4 = orc_normal_cdf(eta * -d1);
+    const double pr = pow(s / ext, -lambda);
+    const double dq = exp(-q * t);
+    const double dr = exp(-r * t);
+    return eta * ((s * dq * n1 - ext * dr * n2) + s * dr * (pr * n3 - dq * n4 / dr) / lambda);
+}
+
+double orc_digital(double s, double k, double r, double q, double v, double t, int type,
+                   int style, double cash) { /* analytic.cpp:214-222 */
+    double d1, d2;
+    d1d2(s, k, r, q, v, t, &d1, &d2);
+    const double phi = sgn(type);
+    if (style == 0) return cash * exp(-r * t) * orc_normal_cdf(phi * d2);
+    return s * exp(-q * t) * orc_normal_cdf(phi * d1);
+}
+
+/* src/pricers/binomial.cpp:9-37 */
+double orc_crr(double s, double k, double r, double q, double v, double t, int steps, int type,
+               double* pi_out, double* up, double* down) {
+    const double dt = t / (double)steps;
+    const double u = exp(v * sqrt(dt));
+    const double d = 1.0 / u;
+    const double g = exp((r - q) * dt);
+    const double pi = (g - d) / (u - d);
+    const double disc = exp(-r * dt);
+    const double phi = sgn(type);
+    double* val = (double*)malloc(sizeof(double) * (size_t)(steps + 1));
+    for (int i = 0; i <= steps; ++i) {
+        const double ts = s * pow(u, (double)i) * pow(d, (double)steps - (double)i);
+        val[i] = smax(phi * (ts - k), 0.0);
+    }
+    for (int level = steps - 1; level >= 0; --level)
+        for (int i = 0; i <= level; ++i) val[i] = disc * (pi * val[i + 1] + (1.0 - pi) * val[i]);
+    const double price = val[0];
+    free(val);
+    if (pi_out) *pi_out = pi;
+    if (up) *up = u;
+    if (down) *down = d;
+    return price;
+}
+
+/* src/pricers/binomial.cpp:39-72 */
+double orc_crr_american(double s, double k, double r, double q, double v, double t, int steps,
+                        int type) {
+    const double dt = t / (double)steps;
+    const double u = exp(v * sqrt(dt));
+    const double d = 1.0 / u;
+    const double g = exp((r - q) * dt);
+    const double pi = (g - d) / (u - d);
+    const double disc = exp(-r * dt);
+    const double phi = sgn(type);
+    double* val = (double*)malloc(sizeof(double) * (size_t)(steps + 1));
+    for (int i = 0; i <= steps; ++i) {
+        const double sp = s * pow(u, (double)i) * pow(d, (double)steps - (double)i);
+        val[i] = smax(phi * (sp - k), 0.0);
+    }
+    for (int level = steps - 1; level >= 0; --level) {
+        for (int i = 0; i <= level; ++i) {
+            const double cont = disc * (pi * val[i + 1] + (1.0 - pi) * val[i]);
+            const double ns = s * pow(u, (double)i) * pow(d, (double)level - (double)i);
+            const double ex = smax(phi * (ns - k), 0.0);
+            val[i] = smax(cont, ex);
+        }
+    }
+    const double price = val[0];
+    free(val);
+    return price;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Greeks: src/greeks/pathwise.cpp, src/greeks/likelihood_ratio.cpp          */
+/* ------------------------------------------------------------------------ */
+static void finish_greek(const orc_welford* w, orc_greek* g) {
+    g->value = w->mean;
+    g->standard_error = orc_welford_se(w);
+}
+
+void orc_pathwise_european(double s, double k, double r, double q, double v, double t, int type,
+                           uint64_t n, uint64_t seed, orc_greek out[3]) { /* pathwise.cpp:28-62 */
+    const double disc = exp(-r * t);
+    const double st_ = sqrt(t);
+    const double phi = sgn(type);
+    orc_welford a[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (uint64_t i = 0; i < n; ++i) {
+        const double z = orc_standard_normal_variate(seed, i, 0);
+        const double sT = orc_gbm_terminal_spot(s, r, q, v, t, z);
+        const int itm = phi * (sT - k) > 0.0;
+        const double slope = itm ? phi : 0.0;
+        const double h = disc * smax(phi * (sT - k), 0.0);
+        const double dsp = sT / s;
+        const double dsv = sT * (st_ * z - v * t);
+        const double dsr = sT * t;
+        orc_welford_add(&a[0], disc * slope * dsp);
+        orc_welford_add(&a[1], disc * slope * dsv);
+        orc_welford_add(&a[2], disc * slope * dsr - t * h);
+    }
+    for (int j = 0; j < 3; ++j) finish_greek(&a[j], &out[j]);
+}
+
+void orc_pathwise_asian(double s0, double k, double r, double q, double v, double t, int type,
+                        int strike_style, int average_style, int m, uint64_t n, uint64_t seed,
+                        orc_greek out[3]) { /* pathwise.cpp:64-147 */
+    const double dt = t / (double)m;
+    const double sdt = sqrt(dt);
+    const double disc = exp(-r * t);
+    const double phi = sgn(type);
+    const double nn = (double)m;
+    orc_welford a[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (uint64_t p = 0; p < n; ++p) {
+        double s = s0, cz = 0.0, ss = 0.0, ls = 0.0, ssv = 0.0, sdv = 0.0, ssr = 0.0, sdr = 0.0;
+        double lsv = 0.0, lsr = 0.0;
+        for (int step = 0; step < m; ++step) {
+            const double z = orc_standard_normal_variate(seed, p, (uint32_t)step);
+            cz += z;
+            s = orc_gbm_terminal_spot(s, r, q, v, dt, z);
+            const double i = (double)(step + 1);
+            const double dlv = -v * i * dt + sdt * cz;
+            const double dlr = i * dt;
+            const double dv = s * dlv;
+            const double dr = s * dlr;
+            ss += s;
+            ls += log(s);
+            ssv += dv;
+            sdv += dlv;
+            ssr += dr;
+            sdr += dlr;
+            lsv = dv;
+            lsr = dr;
+        }
+        const double avg = average_style == 0 ? ss / nn : exp(ls / nn);
+        const double dad_s = avg / s0;
+        const double dad_v = average_style == 0 ? ssv / nn : avg * (sdv / nn);
+        const double dad_r = average_style == 0 ? ssr / nn : avg * (sdr / nn);
+        const double dl_s = s / s0;
+        const double ref = strike_style == 0 ? (avg - k) : (s - avg);
+        const int itm = phi * ref > 0.0;
+        const double slope = itm ? phi : 0.0;
+        const double h = disc * smax(phi * ref, 0.0);
+        double dd, vv, rr;
+        if (strike_style == 0) {
+            dd = slope * dad_s;
+            vv = slope * dad_v;
+            rr = slope * dad_r;
+        } else {
+            dd = slope * (dl_s - dad_s);
+            vv = slope * (lsv - dad_v);
+            rr = slope * (lsr - dad_r);
+        }
+        orc_welford_add(&a[0], disc * dd);
+        orc_welford_add(&a[1], disc * vv);
+        orc_welford_add(&a[2], disc * rr - t * h);
+    }
+    for (int j = 0; j < 3; ++j) finish_greek(&a[j], &out[j]);
+}
+
+void orc_lr_terminal(double s, double k, double r, double q, double v, double t, int type,
+                     int digital, int digital_style, double cash, uint64_t n, uint64_t seed,
+                     orc_greek out[5]) { /* likelihood_ratio.cpp:34-99 */
+    const double disc = exp(-r * t);
+    const double st_ = sqrt(t);
+    const double mu = r - q - 0.5 * v * v;
+    const double phi = sgn(type);
+    orc_welford a[5];
+    memset(a, 0, sizeof(a));
+    for (uint64_t i = 0; i < n; ++i) {
+        const double z = orc_standard_normal_variate(seed, i, 0);
+        const double sT = orc_gbm_terminal_spot(s, r, q, v, t, z);
+        double pay;
+        if (!digital) {
+            pay = smax(phi * (sT - k), 0.0);
+        } else {
+            const int itm = phi * (sT - k) > 0.0;
+            pay = !itm ? 0.0 : (digital_style == 0 ? cash : sT);
+        }
+        const double h = disc * pay;
+        const double ds = z / (s * v * st_);
+        const double gs = (z * z - v * st_ * z - 1.0) / (s * s * v * v * t);
+        const double vs = (z * z - 1.0) / v - st_ * z;
+        const double rs = st_ * z / v;
+        const double ts = (z * z - 1.0) / (2.0 * t) + mu * z / (v * st_);
+        orc_welford_add(&a[0], h * ds);
+        orc_welford_add(&a[1], h * gs);
+        orc_welford_add(&a[2], h * vs);
+        orc_welford_add(&a[3], h * (rs - t));
+        orc_welford_add(&a[4], h * (r - ts));
+    }
+    for (int j = 0; j < 5; ++j) finish_greek(&a[j], &out[j]);
+}
+
+void orc_lr_barrier(double s0, double k, double hb, double r, double q, double v, double t,
+                    int type, int direction, int knock, double rebate, int m, uint64_t n,
+                    uint64_t seed, orc_greek out[4]) { /* likelihood_ratio.cpp:105-154 */
+    const double dt = t / (double)m;
+    const double sdt = sqrt(dt);
+    const double disc = exp(-r * t);
+    const double phi = sgn(type);
+    orc_welford a[4];
+    memset(a, 0, sizeof(a));
+    for (uint64_t p = 0; p < n; ++p) {
+        double s = s0, z1 = 0.0, vsum = 0.0, rsum = 0.0, last = 0.0;
+        int breached = 0;
+        for (int step = 0; step < m; ++step) {
+            const double z = orc_standard_normal_variate(seed, p, (uint32_t)step);
+            if (step == 0) z1 = z;
+            vsum += (z * z - 1.0) / v - sdt * z;
+            rsum += sdt * z / v;
+            s = orc_gbm_terminal_spot(s, r, q, v, dt, z);
+            const int crossed = direction == 0 ? s >= hb : s <= hb;
+            breached = breached || crossed;
+            last = s;
+        }
+        const double vanilla = smax(phi * (last - k), 0.0);
+        const int pays = knock == 1 ? !breached : breached;
+        const double h = disc * (pays ? vanilla : rebate);
+        const double ds = z1 / (s0 * v * sdt);
+        const double gs = (z1 * z1 - v * sdt * z1 - 1.0) / (s0 * s0 * v * v * dt);
+        orc_welford_add(&a[0], h * ds);
+        orc_welford_add(&a[1], h * gs);
+        orc_welford_add(&a[2], h * vsum);
+        orc_welford_add(&a[3], h * (rsum - t));
+    }
+    for (int j = 0; j < 4; ++j) finish_greek(&a[j], &out[j]);
+}
