@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Headline benchmark: path-dependent GBM exotics on one MI355X.
+
+Workload (one "step"): price three contracts, each 2^24 Philox paths x 252 monitoring
+dates, S=100 r=0.05 q=0 sigma=0.2 T=1, seed 42, FP32 path generation + FP64 accumulation:
+  * arithmetic-average Asian call, K=100 (fixed strike)
+  * up-and-out barrier call, H=120, K=100, rebate 0 (discrete monitoring)
+  * floating-strike lookback call
+Each contract is one pricing call of the engine (path kernel + fixed-order merge),
+enqueued through the mcdx C ABI on torch's current stream; inputs are scalars, so nothing
+crosses PCIe inside the timed region. `value` = contract-paths priced per second.
+
+Also reported (untimed): FP64-mode prices of the same paths (bias and SE), the roofline of
+the dominant path kernel from HIP events around each launch, and the reference's own
+multithreaded CPU implementation (oracle/_ref) timed on this host on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "derivative-pricing_amd", "python"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+N_PATHS = 1 << 24
+STEPS = 252
+SEED = 42
+
+# Algorithmic operation count per path-step in FP32 mode (FMA = 2 flops, every other
+# lane operation = 1): Philox4x32-10 word 0 = 10 x (2 mul_hi + 2 mul_lo + 4 xor) = 80;
+# inverse normal (central branch) = 2 (u map) + 11 FMA (22) + 1 mul + 1 rcp + 1 mul = 27;
+# log-space GBM step = 1 FMA + 1 add = 3; payoff observation: arithmetic average
+# exp2 + cvt + f64 add = 3, barrier compare + or = 2, running max + min = 2.
+OPS_DRAW = 80 + 27
+OPS_GBM = 3
+OPS_OBSERVE = {"asian": 3, "barrier_uo": 2, "lookback_float": 2}
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FMA)
+PEAK_ISSUE_TOPS = 39.3     # MI355X non-packed VALU lane-op issue rate (256 CU x 64 x 2.4 GHz)
+
+
+def contracts(mcdx, n):
+    common = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.2, time=1.0,
+                  path_count=n, seed=SEED, monitoring_points=STEPS)
+    return {
+        "asian": mcdx.contract(mcdx.ASIAN, average_style=0, strike_style=0, **common),
+        "barrier_uo": mcdx.contract(mcdx.BARRIER, barrier=120.0, direction=0, knock=1,
+                                    rebate=0.0, **common),
+        "lookback_float": mcdx.contract(mcdx.LOOKBACK, strike_style=1, **common),
+    }
+
+
+def cpu_baseline(budget_s=12.0):
+    """Time the reference's own multithreaded implementation (oracle/_ref) on this host, or
+    the C oracle where the reference build is absent, on a bounded sample of the workload."""
+    import oracle_lib as O
+    cores = len(os.sched_getaffinity(0))
+    R = O.ref()
+    res = O.Result()
+
+    def run(n):
+        t0 = time.perf_counter()
+        if R is not None:
+            R.ref_mc_asian(100, 100, .05, 0, .2, 1, 0, 0, 0, STEPS, n, SEED, 0, 0, cores, C.byref(res))
+            R.ref_mc_barrier(100, 100, 120, .05, 0, .2, 1, 0, 0, 1, 0.0, STEPS, n, SEED, 0, 0,
+                             cores, C.byref(res))
+            R.ref_mc_lookback(100, 100, .05, 0, .2, 1, 0, 1, STEPS, n, SEED, 0, cores, C.byref(res))
+        else:
+            for prod in (O.product(O.ASIAN, monitoring_points=STEPS),
+                         O.product(O.BARRIER, barrier=120.0, monitoring_points=STEPS),
+                         O.product(O.LOOKBACK, strike_style=1, monitoring_points=STEPS)):
+                O.price(prod, n, SEED, chunks=cores, threads=cores)
+        return time.perf_counter() - t0
+
+    n = 1 << 14
+    dt = run(n)
+    n2 = int(min(N_PATHS, max(n, n * budget_s / max(dt, 1e-3) / 1.2)))
+    dt2 = run(n2)
+    return {"value": 3 * n2 / dt2, "unit": "paths/s", "cores": cores,
+            "kind": "reference" if R is not None else "port",
+            "sample": f"3 contracts x {n2} paths x {STEPS} steps (same contracts, seed {SEED}), "
+                      f"{dt2:.1f} s, num_threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--paths", type=int, default=N_PATHS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
+              "single-GPU build (multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    import torch
+    import mcdx
+
+    torch.cuda.init()
+    L = mcdx.lib()
+    cs = contracts(mcdx, args.paths)
+    names = list(cs)
+    # A dedicated stream (default priority): its handle is non-null, so the engine enqueues
+    # on it rather than on its own per-thread stream, and the events below see the kernels.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    handle = C.c_void_p(stream.cuda_stream)
+    assert handle.value, "need a non-default stream handle"
+    out = torch.zeros((len(names), 3), dtype=torch.float64, device="cuda")
+
+    def step(events=None):
+        for i, k in enumerate(names):
+            if events is not None:
+                events[i].record(stream)
+            mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(cs[k]), mcdx.PREC_FP32, handle,
+                                                C.c_void_p(out[i].data_ptr())),
+                       "mcdx_price_gbm_enqueue")
+        if events is not None:
+            events[len(names)].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Timed region: K steps between two events, per-launch events inside.
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+          for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for s in range(args.steps):
+        step(ev[s])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = t_start.elapsed_time(t_end) / args.steps
+    per_kernel_ms = {k: sum(ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)) /
+                     args.steps for i, k in enumerate(names)}
+
+    # Each row holds the Welford triple (uint64 n, double mean, double m2).
+    host = out.cpu().numpy()
+    counts = host.view("u8")
+    prices = {k: mcdx.welford_finish(int(counts[i, 0]), float(host[i, 1]), float(host[i, 2]))
+              for i, k in enumerate(names)}
+
+    parity = {}
+    for k in names:
+        p, se, n = prices[k]
+        parity[k] = {"price_fp32": p, "se": se, "paths": n}
+    if not args.no_fp64:
+        for k in names:
+            p64, se64, _ = mcdx.price(cs[k], mcdx.PREC_FP64)
+            parity[k].update(price_fp64=p64, bias_fp32_vs_fp64=parity[k]["price_fp32"] - p64,
+                             bias_in_se=(parity[k]["price_fp32"] - p64) / se64)
+
+    dom = max(per_kernel_ms, key=per_kernel_ms.get)
+    ops = (OPS_DRAW + OPS_GBM + OPS_OBSERVE[dom]) * args.paths * STEPS
+    t_dom = per_kernel_ms[dom] / 1e3
+    achieved_tops = ops / t_dom / 1e12
+    roofline = {
+        "bound": "valu", "kernel": f"gbm_tile_kernel[{dom}, fp32]",
+        "achieved": round(achieved_tops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved_tops / PEAK_FP32_TFLOPS, 4),
+        "frac_of_issue_rate": round(achieved_tops / PEAK_ISSUE_TOPS, 4),
+        "ops_per_path_step": OPS_DRAW + OPS_GBM + OPS_OBSERVE[dom],
+        "launch_ms": round(per_kernel_ms[dom], 4),
+        "per_launch_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
+        "traffic": None,
+    }
+
+    value = len(names) * args.paths / (ms_per_step / 1e3)
+    line = {
+        "metric": "paths/sec + % VALU roofline at <=3 SE vs FP64 oracle; batched vs per-contract",
+        "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32-gen/f64-acc",
+        "data": "synthetic (Philox counter-based draws, seed 42)",
+        "config": {"workload": "GBM exotics: arithmetic Asian call K=100, up-and-out call "
+                               "H=120 K=100, floating lookback call; S=100 r=0.05 q=0 "
+                               "sigma=0.2 T=1", "paths_per_contract": args.paths,
+                   "monitoring_points": STEPS, "contracts": len(names), "rng": "philox4x32-10",
+                   "parallelism": "single GPU"},
+        "path_steps_per_s": value * STEPS,
+        "roofline": roofline,
+        "parity": parity,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

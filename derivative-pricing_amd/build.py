@@ -1,0 +1,96 @@
+"""Build recipe for the MI355X engine (run by __graft_entry__.build()).
+
+Produces, in-tree:
+  derivative-pricing_amd/lib/libmcd_hip.so   HIP kernels + the mcdx C ABI (include/mcdx.h)
+  derivative-pricing_amd/python/mcd*.so       pybind11 `mcd` module (drop-in Python API)
+and asks oracle/Makefile for the CPU oracle (test infrastructure) and, where the reference
+tree exists, oracle/_ref.
+"""
+import glob
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIBDIR = os.path.join(PKG, "lib")
+PYDIR = os.path.join(PKG, "python")
+LIB = os.path.join(LIBDIR, "libmcd_hip.so")
+ARCH = os.environ.get("MCDX_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++20",
+    # FP64 mode must round exactly as the reference's scalar code: no contraction unless a
+    # function opts in (the FP32 generation code uses explicit fmaf).
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, **kw)
+
+
+def build_hip(jobs=8):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objs = []
+    procs = []
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s) + ".o")
+        objs.append(o)
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", s, "-o", o]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd)))
+        if len(procs) >= jobs:
+            s0, p0 = procs.pop(0)
+            if p0.wait() != 0:
+                raise RuntimeError(f"hipcc failed on {s0}")
+    for s0, p0 in procs:
+        if p0.wait() != 0:
+            raise RuntimeError(f"hipcc failed on {s0}")
+    tmp = LIB + ".tmp"
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+        ["-Wl,-soname,libmcd_hip.so"])
+    os.replace(tmp, LIB)
+
+
+def build_pybind():
+    hosts = sorted(glob.glob(os.path.join(HOST, "*.cpp")))
+    if not hosts:
+        return
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(PYDIR, "mcd" + suffix)
+    cmd = ["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
+           "-I", os.path.join(ROOT, "include"), "-I", pybind11.get_include(),
+           "-I", sysconfig.get_paths()["include"]] + hosts + [
+           "-L", LIBDIR, "-lmcd_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", out + ".tmp"]
+    run(cmd)
+    os.replace(out + ".tmp", out)
+
+
+def build_oracle():
+    run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    ref_root = os.environ.get("MCDX_REFERENCE_ROOT", "/root/reference")
+    if os.path.isdir(os.path.join(ref_root, "src")):
+        run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF_ROOT={ref_root}"])
+
+
+def main():
+    build_hip()
+    build_pybind()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,48 @@
+// Host-side runtime shared by the kernel families: per-calling-thread HIP stream and
+// scratch, status codes, the fixed-order tile merge, and result finishing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+#include "mcdx.h"
+
+namespace mcdx {
+
+// Per-thread engine state: its own non-blocking stream (default priority) and growable
+// device/pinned scratch, so concurrent callers on different threads never share buffers
+// or serialize on a lock while kernels run.
+struct ThreadCtx {
+    hipStream_t stream = nullptr;
+    void* dev = nullptr;
+    size_t dev_bytes = 0;
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+// Returns MCDX_OK or an error status; on success *ctx is ready on the current device.
+int thread_ctx(ThreadCtx** ctx);
+// Ensure the device scratch holds at least `bytes`; contents are not preserved.
+int ensure_dev(ThreadCtx* ctx, size_t bytes);
+int ensure_pinned(ThreadCtx* ctx, size_t bytes);
+int status_of(hipError_t e);
+
+// Stream to launch on: the caller's stream if given, else the thread's own.
+inline hipStream_t pick_stream(ThreadCtx* ctx, void* stream) {
+    return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+// Launch the fixed-order merge of `n_tiles` tile partials (one group per contract when
+// batched) into `out[contract]`. `tile_offsets` may be null for a single contract.
+hipError_t launch_final_merge(const Welford* partials, const uint64_t* tile_offsets,
+                              uint64_t n_tiles, uint32_t n_contracts, Welford* out,
+                              hipStream_t stream);
+
+// price = mean, SE = sqrt(var/n) with var = m2/(n-1) (stats.hpp:35-41).
+mcdx_result finish(const Welford& w);
+
+inline uint64_t tiles_for(uint64_t paths) { return (paths + kTilePaths - 1) / kTilePaths; }
+
+} // namespace mcdx

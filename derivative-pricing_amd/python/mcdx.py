@@ -1,0 +1,138 @@
+"""ctypes view of the mcdx C ABI (include/mcdx.h) in libmcd_hip.so.
+
+This is the thin Python handle that tests, smoke() and bench.py use to call the HIP
+engine through its C boundary. It loads only the in-tree build and raises if it is
+missing; there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libmcd_hip.so")
+
+OK, EINVAL, EDEVICE, ENOMEM, ENODEV = range(5)
+PREC_FP32, PREC_FP64 = 0, 1
+EUROPEAN, DIGITAL, ASIAN, BARRIER, LOOKBACK = range(5)
+
+u8, u32, u64, i32, dbl = C.c_uint8, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+P = C.POINTER
+
+
+class Result(C.Structure):
+    _fields_ = [("price", dbl), ("standard_error", dbl), ("path_count", u64)]
+
+
+class Options(C.Structure):
+    _fields_ = [("antithetic", u8), ("control_variate", u8), ("brownian_bridge", u8),
+                ("reserved", u8), ("num_threads", u32)]
+
+
+class GbmContract(C.Structure):
+    _fields_ = [(n, dbl) for n in ("spot", "strike", "barrier", "rebate", "rate", "carry_yield",
+                                   "vol", "time", "cash_amount")] + [
+        ("path_count", u64), ("seed", u64), ("monitoring_points", i32)] + [
+        (n, u8) for n in ("product", "type", "direction", "knock", "strike_style",
+                          "average_style", "digital_style", "antithetic", "control_variate",
+                          "brownian_bridge")] + [("reserved", u8 * 2)]
+
+
+def contract(product, spot=100.0, strike=100.0, barrier=0.0, rebate=0.0, rate=0.05,
+             carry_yield=0.0, vol=0.2, time=1.0, cash_amount=1.0, path_count=1 << 16, seed=42,
+             monitoring_points=1, type=0, direction=0, knock=1, strike_style=0,
+             average_style=0, digital_style=0, antithetic=0, control_variate=0,
+             brownian_bridge=0):
+    c = GbmContract()
+    for k, v in dict(spot=spot, strike=strike, barrier=barrier, rebate=rebate, rate=rate,
+                     carry_yield=carry_yield, vol=vol, time=time, cash_amount=cash_amount,
+                     path_count=path_count, seed=seed, monitoring_points=monitoring_points,
+                     product=product, type=type, direction=direction, knock=knock,
+                     strike_style=strike_style, average_style=average_style,
+                     digital_style=digital_style, antithetic=antithetic,
+                     control_variate=control_variate, brownian_bridge=brownian_bridge).items():
+        setattr(c, k, v)
+    return c
+
+
+# name -> (restype, argtypes); the exported-symbol test checks this against include/mcdx.h.
+SIGNATURES = {
+    "mcdx_device_count": (C.c_int, [P(C.c_int)]),
+    "mcdx_set_precision": (C.c_int, [C.c_int]),
+    "mcdx_get_precision": (C.c_int, []),
+    "mcdx_version": (C.c_char_p, []),
+    "mcdx_philox_bits": (C.c_int, [u64, u64, u64, u32, u32, P(u32)]),
+    "mcdx_normal_draws": (C.c_int, [u64, u64, u64, u32, C.c_int, P(dbl)]),
+    "mcdx_monte_carlo_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Options), P(Result)]),
+    "mcdx_monte_carlo_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Options),
+                                                       P(Result)]),
+    "mcdx_monte_carlo_asian": (C.c_int, [dbl] * 6 + [u8, u8, u8, i32, u64, u64, P(Options),
+                                                     P(Result)]),
+    "mcdx_monte_carlo_barrier": (C.c_int, [dbl] * 7 + [u8, u8, u8, dbl, i32, u64, u64,
+                                                       P(Options), P(Result)]),
+    "mcdx_monte_carlo_lookback": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, u64, P(Options),
+                                                        P(Result)]),
+    "mcdx_price_gbm": (C.c_int, [P(GbmContract), C.c_int, P(Result)]),
+    "mcdx_price_gbm_enqueue": (C.c_int, [P(GbmContract), C.c_int, C.c_void_p, C.c_void_p]),
+    "mcdx_gbm_path_samples": (C.c_int, [P(GbmContract), C.c_int, u64, u64, P(dbl)]),
+}
+
+_lib = None
+
+
+class McdxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmcd_hip.so (built in-tree); raise if it is not there."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise McdxError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what=""):
+    if status != OK:
+        raise McdxError(f"{what} failed with mcdx status {status}")
+
+
+def philox_bits(seed, path0, n, draw, stream):
+    out = np.empty(n, dtype=np.uint32)
+    check(lib().mcdx_philox_bits(seed, path0, n, draw, stream, out.ctypes.data_as(P(u32))),
+          "mcdx_philox_bits")
+    return out
+
+
+def normal_draws(seed, path0, n, draw, precision):
+    out = np.empty(n, dtype=np.float64)
+    check(lib().mcdx_normal_draws(seed, path0, n, draw, precision, out.ctypes.data_as(P(dbl))),
+          "mcdx_normal_draws")
+    return out
+
+
+def price(c, precision=PREC_FP32):
+    r = Result()
+    check(lib().mcdx_price_gbm(C.byref(c), precision, C.byref(r)), "mcdx_price_gbm")
+    return r.price, r.standard_error, r.path_count
+
+
+def path_samples(c, precision, path0, n):
+    out = np.empty(n, dtype=np.float64)
+    check(lib().mcdx_gbm_path_samples(C.byref(c), precision, path0, n,
+                                      out.ctypes.data_as(P(dbl))), "mcdx_gbm_path_samples")
+    return out
+
+
+def welford_finish(n, mean, m2):
+    """price, SE, count from a device Welford triple (stats.hpp:35-41)."""
+    var = 0.0 if n < 2 else m2 / (n - 1)
+    se = 0.0 if n == 0 else (var / n) ** 0.5
+    return mean, se, n

@@ -1,0 +1,131 @@
+/*
+ * mcdx -- C ABI of the MI355X (gfx950) Monte Carlo path engine, libmcd_hip.so.
+ *
+ * Plain C types only: scalars, POD structs and caller-owned host pointers. Every entry
+ * point returns an mcdx_status and never throws or aborts. Results are written to
+ * caller-allocated out-structs. Each function names the reference `mcd` interface it
+ * replaces (file:line in FredTheMay/derivative-pricing).
+ *
+ * Enum arguments use the reference's uint8_t values (include/mcd/core/types.hpp:12-17):
+ *   type 0 Call / 1 Put; direction 0 Up / 1 Down; knock 0 In / 1 Out;
+ *   strike_style 0 Fixed / 1 Floating; digital_style 0 CashOrNothing / 1 AssetOrNothing;
+ *   average_style 0 Arithmetic / 1 Geometric.
+ *
+ * Precision: MCDX_PREC_FP32 generates paths in FP32 (Philox bits exact, FP32 inverse
+ * normal and log-space GBM stepping) and accumulates every payoff in FP64;
+ * MCDX_PREC_FP64 repeats the reference's FP64 arithmetic operation for operation. The
+ * reference-signature entry points use the process default (mcdx_set_precision).
+ *
+ * Threading: every entry point is reentrant. Each calling thread gets its own HIP stream
+ * and scratch; no lock is held while kernels run. The reference's `num_threads` option is
+ * accepted and ignored (the GPU schedule does not depend on it).
+ */
+#ifndef MCDX_H
+#define MCDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mcdx_status {
+    MCDX_OK = 0,
+    MCDX_EINVAL = 1,  /* an argument is outside what the entry point accepts */
+    MCDX_EDEVICE = 2, /* a HIP call failed */
+    MCDX_ENOMEM = 3,  /* device or pinned allocation failed */
+    MCDX_ENODEV = 4   /* no HIP device is visible */
+} mcdx_status;
+
+enum { MCDX_PREC_FP32 = 0, MCDX_PREC_FP64 = 1 };
+
+/* McResult, include/mcd/pricers/monte_carlo.hpp:9-13. */
+typedef struct mcdx_result {
+    double price;
+    double standard_error;
+    uint64_t path_count;
+} mcdx_result;
+
+/* McOptions, include/mcd/pricers/monte_carlo.hpp:18-33 (num_threads ignored on GPU). */
+typedef struct mcdx_options {
+    uint8_t antithetic;
+    uint8_t control_variate;
+    uint8_t brownian_bridge;
+    uint8_t reserved;
+    uint32_t num_threads;
+} mcdx_options;
+
+/* One GBM contract: the union of every GBM pricer's arguments
+ * (include/mcd/pricers/monte_carlo.hpp:38-80) plus its path count and seed. */
+enum {
+    MCDX_EUROPEAN = 0,
+    MCDX_DIGITAL = 1,
+    MCDX_ASIAN = 2,
+    MCDX_BARRIER = 3,
+    MCDX_LOOKBACK = 4
+};
+typedef struct mcdx_gbm_contract {
+    double spot, strike, barrier, rebate, rate, carry_yield, vol, time, cash_amount;
+    uint64_t path_count;
+    uint64_t seed;
+    int32_t monitoring_points; /* ignored for European/digital */
+    uint8_t product;           /* MCDX_EUROPEAN .. MCDX_LOOKBACK */
+    uint8_t type, direction, knock, strike_style, average_style, digital_style;
+    uint8_t antithetic, control_variate, brownian_bridge;
+    uint8_t reserved[2];
+} mcdx_gbm_contract;
+
+/* ---- engine ---- */
+int mcdx_device_count(int* count);
+int mcdx_set_precision(int precision); /* process default for reference-signature calls */
+int mcdx_get_precision(void);
+const char* mcdx_version(void); /* mcd::version(), src/core/types.cpp:5 */
+
+/* ---- RNG streams (src/core/rng.cpp:41-74) ---- */
+/* out[i] = word 0 of Philox4x32-10(counter {path0+i, draw, stream}, key seed), i < n. */
+int mcdx_philox_bits(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw,
+                     uint32_t stream, uint32_t* out);
+/* out[i] = standard_normal_variate(seed, path0+i, draw) at `precision` (rng.cpp:59-66). */
+int mcdx_normal_draws(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw, int precision,
+                      double* out);
+
+/* ---- GBM Monte Carlo pricers (src/pricers/monte_carlo.cpp:234-338) ---- */
+int mcdx_monte_carlo_european(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, uint64_t path_count,
+                              uint64_t seed, const mcdx_options* options, mcdx_result* out);
+int mcdx_monte_carlo_digital(double spot, double strike, double rate, double carry_yield,
+                             double vol, double time, uint8_t type, uint8_t style,
+                             double cash_amount, uint64_t path_count, uint64_t seed,
+                             const mcdx_options* options, mcdx_result* out);
+int mcdx_monte_carlo_asian(double spot, double strike, double rate, double carry_yield,
+                           double vol, double time, uint8_t type, uint8_t strike_style,
+                           uint8_t average_style, int32_t monitoring_points,
+                           uint64_t path_count, uint64_t seed, const mcdx_options* options,
+                           mcdx_result* out);
+int mcdx_monte_carlo_barrier(double spot, double strike, double barrier, double rate,
+                             double carry_yield, double vol, double time, uint8_t type,
+                             uint8_t direction, uint8_t knock, double rebate,
+                             int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                             const mcdx_options* options, mcdx_result* out);
+int mcdx_monte_carlo_lookback(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, uint8_t style,
+                              int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                              const mcdx_options* options, mcdx_result* out);
+
+/* Any GBM contract at an explicit precision (the pricers above route through this). */
+int mcdx_price_gbm(const mcdx_gbm_contract* contract, int precision, mcdx_result* out);
+/* Stream-ordered form for callers that own a HIP stream (hipStream_t passed as void*, NULL
+ * = the calling thread's stream): enqueues the pricing and writes {n, mean, m2} as
+ * (uint64_t, double, double) to the 24-byte device buffer `dev_welford`; returns at once. */
+int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* contract, int precision, void* stream,
+                           void* dev_welford);
+/* Discounted per-path samples exactly as the pricer accumulates them (antithetic pairs
+ * averaged) for paths [path0, path0+n): the per-path parity hook. */
+int mcdx_gbm_path_samples(const mcdx_gbm_contract* contract, int precision, uint64_t path0,
+                          uint64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCDX_H */

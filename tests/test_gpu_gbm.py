@@ -1,0 +1,223 @@
+"""GPU parity of the GBM path engine against the CPU oracle, through the mcdx C ABI.
+
+Bars (stated per test):
+  * Philox bits: bit-exact, every (seed, path, draw, stream) tested.
+  * FP64 mode: per-path samples within 1e-12 relative of the oracle (device libm vs glibc
+    differ by at most an ulp per call); prices within 1e-12 relative of the oracle's
+    chunked FP64 price (the merge trees differ).
+  * FP32 mode: per-path samples agree with the oracle to FP32 generation accuracy; prices
+    lie within 3 SE of the closed-form oracle where one exists and within 0.25 SE of the
+    FP64 price on the same paths.
+  * Two runs of either mode are bitwise identical.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mcdx
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+FP32, FP64 = mcdx.PREC_FP32, mcdx.PREC_FP64
+
+
+def both(kind, **kw):
+    """The same contract as an mcdx contract and an oracle product."""
+    n = kw.pop("path_count", 4096)
+    seed = kw.pop("seed", 42)
+    return mcdx.contract(kind, path_count=n, seed=seed, **kw), O.product(kind, **kw), n, seed
+
+
+# --------------------------------------------------------------------------- RNG
+@pytest.mark.parametrize("seed,path0,draw,stream", [
+    (42, 0, 0, 0), (42, 0, 251, 0), (42, 12345, 7, 1), (0xDEADBEEFCAFEF00D, 0, 3, 0),
+    (7, (1 << 32) - 300, 0, 0), (2024, (1 << 40) + 5, (1 << 31) + 9, 1)])
+def test_philox_bits_bit_exact(seed, path0, draw, stream):
+    n = 4099
+    got = mcdx.philox_bits(seed, path0, n, draw, stream)
+    want = O.philox_bits(seed, path0, n, draw, stream)
+    assert np.array_equal(got, want)
+
+
+def test_philox_known_answers_through_stream_layout():
+    # Random123 KATs (reference tests/rng_test.cpp:17-39) mapped onto the
+    # {path_lo, path_hi, draw, stream} / {seed_lo, seed_hi} layout (rng.cpp:49-57).
+    assert mcdx.philox_bits(0, 0, 1, 0, 0)[0] == 0x6627E8D5
+    assert mcdx.philox_bits((1 << 64) - 1, (1 << 64) - 1, 1, 0xFFFFFFFF, 0xFFFFFFFF)[0] == 0x408F276D
+    assert mcdx.philox_bits(0x299F31D0A4093822, 0x85A308D3243F6A88, 1, 0x13198A2E,
+                            0x03707344)[0] == 0xD16CFE09
+
+
+def test_normal_draws_fp64_match_oracle():
+    n = 1 << 16
+    got = mcdx.normal_draws(42, 0, n, 3, FP64)
+    want = np.array([O.lib().orc_standard_normal_variate(42, i, 3) for i in range(n)])
+    assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) <= 1e-14
+
+
+def test_normal_draws_fp32_accuracy_and_tails():
+    n = 1 << 18
+    got = mcdx.normal_draws(7, 0, n, 0, FP32)
+    want = np.array([O.lib().orc_standard_normal_variate(7, i, 0) for i in range(n)])
+    err = np.abs(got - want) / (1.0 + np.abs(want))
+    assert np.all(np.isfinite(got))
+    assert err.max() <= 2e-6, err.max()
+    # same branch (central vs tail) as the FP64 comparison: signs agree everywhere
+    assert np.array_equal(np.sign(got), np.sign(want))
+
+
+# --------------------------------------------------------------------------- per-path parity
+CASES = [
+    dict(kind=O.EUROPEAN),
+    dict(kind=O.EUROPEAN, type=1, antithetic=1),
+    dict(kind=O.DIGITAL, digital_style=0, cash_amount=1.5),
+    dict(kind=O.DIGITAL, digital_style=1, type=1),
+    dict(kind=O.ASIAN, monitoring_points=12),
+    dict(kind=O.ASIAN, monitoring_points=12, average_style=1, strike_style=1, type=1),
+    dict(kind=O.ASIAN, monitoring_points=52, control_variate=1, antithetic=1),
+    dict(kind=O.BARRIER, barrier=120.0, monitoring_points=20),
+    dict(kind=O.BARRIER, barrier=90.0, direction=1, knock=0, rebate=2.0, monitoring_points=20,
+         type=1),
+    dict(kind=O.BARRIER, barrier=120.0, monitoring_points=50, brownian_bridge=1),
+    dict(kind=O.BARRIER, barrier=85.0, direction=1, monitoring_points=50, brownian_bridge=1,
+         antithetic=1),
+    dict(kind=O.LOOKBACK, strike_style=1, monitoring_points=20),
+    dict(kind=O.LOOKBACK, strike_style=0, strike=105.0, monitoring_points=20, type=1),
+]
+
+
+def _ids(c):
+    return "-".join(f"{k}={v}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_path_samples_fp64_match_oracle(case):
+    case = dict(case)
+    kind = case.pop("kind")
+    c, p, n, seed = both(kind, path_count=2048, **case)
+    got = mcdx.path_samples(c, FP64, 1000, 2048)
+    want = O.path_samples(p, seed, 1000, 2048)
+    assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) <= 1e-12
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_path_samples_fp32_close_to_oracle(case):
+    case = dict(case)
+    kind = case.pop("kind")
+    c, p, n, seed = both(kind, path_count=4096, **case)
+    got = mcdx.path_samples(c, FP32, 0, 4096)
+    want = O.path_samples(p, seed, 0, 4096)
+    err = np.abs(got - want) / (1.0 + np.abs(want))
+    # Discrete decisions (barrier touch, ITM) may flip on paths within FP32 rounding of the
+    # boundary; everything else agrees to FP32 generation accuracy.
+    assert np.mean(err > 1e-4) <= 0.002, np.sort(err)[-10:]
+    assert np.median(err) <= 2e-6
+
+
+# --------------------------------------------------------------------------- prices
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_price_fp64_matches_oracle_price(case):
+    case = dict(case)
+    kind = case.pop("kind")
+    c, p, n, seed = both(kind, path_count=50_003, seed=424242, **case)
+    gp, gse, gn = mcdx.price(c, FP64)
+    op, ose, on = O.price(p, n, seed)
+    assert gn == on == n
+    assert abs(gp - op) <= 1e-12 * max(1.0, abs(op))
+    assert abs(gse - ose) <= 1e-9 * max(1e-12, ose)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_price_fp32_bias_small_against_fp64(case):
+    case = dict(case)
+    kind = case.pop("kind")
+    c, _, n, _ = both(kind, path_count=200_000, seed=11, **case)
+    p32, se32, n32 = mcdx.price(c, FP32)
+    p64, se64, n64 = mcdx.price(c, FP64)
+    assert n32 == n64 == 200_000
+    assert abs(p32 - p64) <= 0.25 * se64 + 1e-12, (p32, p64, se64)
+
+
+def test_european_c1_within_3se_of_bsm():
+    # C1: S=K=100 r=0.05 q=0 sigma=0.2 T=1, 1e6 paths, seed 42; BSM 10.450583572185565.
+    bsm = O.lib().orc_bsm(100, 100, 0.05, 0.0, 0.2, 1.0, 0)
+    assert abs(bsm - 10.450583572185565) < 1e-12
+    for prec in (FP32, FP64):
+        c = mcdx.contract(mcdx.EUROPEAN, path_count=1_000_000, seed=42)
+        p, se, n = mcdx.price(c, prec)
+        assert n == 1_000_000
+        assert abs(p - bsm) < 3 * se
+
+
+def test_digital_and_geometric_asian_within_3se_of_closed_forms():
+    c = mcdx.contract(mcdx.DIGITAL, path_count=1 << 20, seed=5)
+    p, se, _ = mcdx.price(c, FP32)
+    ana = O.lib().orc_digital(100, 100, 0.05, 0, 0.2, 1, 0, 0, 1.0)
+    assert abs(p - ana) < 3 * se
+    # geometric Asian at fine monitoring vs continuous Kemna-Vorst (reference
+    # tests/exotics_mc_test.cpp:24-40 uses 400 steps)
+    c = mcdx.contract(mcdx.ASIAN, average_style=1, monitoring_points=400, path_count=1 << 18,
+                      seed=9)
+    p, se, _ = mcdx.price(c, FP32)
+    kv = O.lib().orc_kemna_vorst(100, 100, 0.05, 0, 0.2, 1, 0)
+    assert abs(p - kv) < 3 * se + 0.01
+
+
+def test_in_out_parity_on_gpu():
+    # knock-in + knock-out = vanilla path by path (reference exotics_mc_test.cpp:110-137)
+    kw = dict(barrier=110.0, monitoring_points=30, path_count=1 << 16, seed=3)
+    pin = mcdx.path_samples(mcdx.contract(mcdx.BARRIER, knock=0, **kw), FP32, 0, 1 << 16)
+    pout = mcdx.path_samples(mcdx.contract(mcdx.BARRIER, knock=1, **kw), FP32, 0, 1 << 16)
+    van = mcdx.path_samples(mcdx.contract(mcdx.BARRIER, knock=1, barrier=1e12,
+                                          monitoring_points=30, path_count=1 << 16, seed=3),
+                            FP32, 0, 1 << 16)
+    assert np.allclose(pin + pout, van, rtol=0, atol=1e-12)
+
+
+def test_runs_bitwise_reproducible():
+    for prec in (FP32, FP64):
+        c = mcdx.contract(mcdx.ASIAN, monitoring_points=24, path_count=300_001, seed=77)
+        a = mcdx.price(c, prec)
+        b = mcdx.price(c, prec)
+        assert a == b
+
+
+@pytest.mark.parametrize("n", [0, 1, 255, 1023, 1024, 1025, 1_000_003])
+def test_ragged_path_counts(n):
+    c, p, _, seed = both(O.EUROPEAN, path_count=n, seed=1)
+    gp, gse, gn = mcdx.price(c, FP64)
+    assert gn == n
+    if n == 0:
+        assert (gp, gse) == (0.0, 0.0)
+        return
+    if n <= 2000:
+        op, ose, on = O.price(p, n, seed)
+        assert abs(gp - op) <= 1e-12 * max(1.0, abs(op))
+
+
+def test_invalid_arguments_rejected():
+    L = mcdx.lib()
+    r = mcdx.Result()
+    c = mcdx.contract(mcdx.ASIAN, monitoring_points=0)
+    assert L.mcdx_price_gbm(c, FP32, r) == mcdx.EINVAL
+    c = mcdx.contract(7)
+    assert L.mcdx_price_gbm(c, FP32, r) == mcdx.EINVAL
+    c = mcdx.contract(mcdx.EUROPEAN)
+    assert L.mcdx_price_gbm(c, 5, r) == mcdx.EINVAL
+
+
+def test_reference_signature_entry_points():
+    L = mcdx.lib()
+    r = mcdx.Result()
+    opts = mcdx.Options(0, 0, 0, 0, 8)
+    assert L.mcdx_set_precision(FP64) == 0
+    try:
+        assert L.mcdx_monte_carlo_lookback(100, 100, 0.05, 0, 0.2, 1, 0, 1, 20, 50_003, 424242,
+                                           opts, r) == 0
+        op, _, _ = O.price(O.product(O.LOOKBACK, strike_style=1, monitoring_points=20), 50_003,
+                           424242)
+        assert abs(r.price - op) <= 1e-12 * op
+    finally:
+        L.mcdx_set_precision(FP32)
