@@ -145,9 +145,8 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
         const bool need_geo = a.average_style == 1 || a.cv;
         StateF32 s = {0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
         StateF32 t = s;
-        for (int step = 0; step < a.m; ++step) {
-            const uint32_t draw = (uint32_t)step;
-            const float z = inv_normal_f32(philox_w0(plo, phh, draw, 0u, k0, k1));
+        // One monitoring step given its draw (draw index = step, monte_carlo.cpp:166-167).
+        auto apply = [&](uint32_t draw, float z) {
             if constexpr (KIND == kBarrierBB) {
                 // monte_carlo.cpp:200-223 in log2 space: the crossing probability
                 // exp(-2 ln(H/S) ln(H/S') / (vol^2 dt)) = exp2(bb_c (xb - x)(xb - x')).
@@ -155,9 +154,8 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
                 const float xt = ANTI ? t.x + fmaf(-a.sig2, z, a.mu2) : 0.f;
                 const bool cs = a.direction == 0 ? xn >= a.xb : xn <= a.xb;
                 s.breached = s.breached || cs;
-                bool ct = false;
                 if (ANTI) {
-                    ct = a.direction == 0 ? xt >= a.xb : xt <= a.xb;
+                    const bool ct = a.direction == 0 ? xt >= a.xb : xt <= a.xb;
                     t.breached = t.breached || ct;
                 }
                 if (!s.breached || (ANTI && !t.breached)) {
@@ -177,7 +175,17 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
                     observe_f32<KIND>(a, t, need_arith, need_geo);
                 }
             }
+        };
+        constexpr int B = 8;
+        int step = 0;
+        for (; step + B <= a.m; step += B) {
+            float z[B];
+            normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
+#pragma unroll
+            for (int j = 0; j < B; ++j) apply((uint32_t)(step + j), z[j]);
         }
+        for (; step < a.m; ++step)
+            apply((uint32_t)step, inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
         double r = settle_f32<KIND>(a, s);
         if (ANTI) r = 0.5 * (r + settle_f32<KIND>(a, t));
         return r;
