@@ -1,0 +1,403 @@
+"""CPU tests that pin the oracle (oracle/mcd_oracle.c) before it is trusted as the checker.
+
+Each function of the oracle is pinned by at least one of:
+  * the reference's golden vectors and known answers (Random123 Philox KATs, Lewis Heston
+    prices, textbook least-squares systems), held as data;
+  * fixtures written by the reference's own implementation (tests/golden/*.json, made by
+    tests/golden/make_golden.py from oracle/_ref), compared bit for bit;
+  * the live reference build (oracle/_ref) where present;
+  * mathematics: closed forms, invariants, brute force on tiny inputs.
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_small.json")))
+L = O.lib()
+fh = float.fromhex
+
+
+# ----------------------------------------------------------------------------- Philox
+@pytest.mark.parametrize("ctr,key,expected", [
+    # Random123 known-answer vectors (reference tests/rng_test.cpp:17-39)
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox_known_answers(ctr, key, expected):
+    out = (C.c_uint32 * 4)()
+    L.orc_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+    assert tuple(out) == expected
+
+
+def test_philox_stream_layout_and_uniform_map():
+    # key {seed_lo, seed_hi}, counter {path_lo, path_hi, draw, stream} (rng.cpp:49-57)
+    out = (C.c_uint32 * 4)()
+    L.orc_philox4x32_10((C.c_uint32 * 4)(0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344),
+                        (C.c_uint32 * 2)(0xA4093822, 0x299F31D0), out)
+    assert L.orc_philox_bits(0x299F31D0A4093822, 0x85A308D3243F6A88, 0x13198A2E,
+                             0x03707344) == out[0]
+    # u = (bits + 0.5) / 2^32 lies strictly inside (0, 1) at both ends (rng.cpp:34-37)
+    assert L.orc_uniform_open01(0) == 0.5 / 2 ** 32
+    assert L.orc_uniform_open01(0xFFFFFFFF) == (2 ** 32 - 0.5) / 2 ** 32 < 1.0
+
+
+def test_draws_match_reference_fixture_bits():
+    for seed, path, draw, z, u in GOLD["draws"]:
+        assert L.orc_standard_normal_variate(seed, path, draw) == fh(z)
+        assert L.orc_uniform_variate(seed, path, draw) == fh(u)
+
+
+# ----------------------------------------------------------------------------- normal
+def test_inverse_normal_matches_reference_fixture_bits():
+    for u, z in GOLD["inverse_normal"]:
+        assert L.orc_inverse_normal_cdf(fh(u)) == fh(z)
+
+
+def _newton_inverse(u):
+    # Independent method (reference tests/rng_test.cpp:60-102): Newton on Phi(x) = u.
+    x = 0.0
+    for _ in range(100):
+        dx = (L.orc_normal_cdf(x) - u) / L.orc_normal_pdf(x)
+        x -= dx
+        if abs(dx) < 1e-15:
+            break
+    return x
+
+
+def test_inverse_normal_against_newton_oracle():
+    us = np.geomspace(1e-9, 0.5, 600)
+    err = max(abs(L.orc_inverse_normal_cdf(u) - _newton_inverse(u)) for u in us)
+    assert err < 1e-9
+    # Upper half by symmetry at points where 1-u is exact (u = 1 - 2^-k). There the Halley
+    # step compares 0.5*erfc(-z/sqrt2) ~ 1 with u, which carries absolute error ~ulp(1), so
+    # z is pinned only to ~ulp(1)/phi(z) (normal.cpp:76-80).
+    for k in range(2, 30):
+        a = 2.0 ** -k
+        z = _newton_inverse(a)
+        tol = 1e-9 + 4.0 * 2.0 ** -53 / L.orc_normal_pdf(z)
+        assert abs(L.orc_inverse_normal_cdf(1 - a) + z) < tol
+
+
+def test_acklam_without_halley_is_within_published_bound():
+    # Acklam's rational alone: relative error below 1.15e-9 (normal.hpp:9-11); this is the
+    # FP32 path's approximation, so its bound is what the FP32 draws inherit.
+    us = np.linspace(1e-6, 1 - 1e-6, 20001)
+    rel = max(abs(L.orc_acklam(u) - L.orc_inverse_normal_cdf(u)) /
+              max(abs(L.orc_inverse_normal_cdf(u)), 1e-300) for u in us if abs(u - 0.5) > 1e-9)
+    assert rel < 1.2e-9
+
+
+def test_tail_thresholds_used_by_the_kernels():
+    # The FP32 kernel decides central vs tail on integer bits: bits in [104152957,
+    # 4190814338] are exactly the u in [0.02425, 1-0.02425] of the FP64 comparison.
+    lo, hi = 104152957, 4190814338
+    assert L.orc_uniform_open01(lo) >= 0.02425 > L.orc_uniform_open01(lo - 1)
+    assert L.orc_uniform_open01(hi) <= 1 - 0.02425 < L.orc_uniform_open01(hi + 1)
+
+
+# ----------------------------------------------------------------------------- Welford
+def test_welford_matches_two_pass_and_merge_identities():
+    rng = np.random.default_rng(0)
+    x = rng.normal(3.0, 2.0, 10007)
+    w = O.lib()
+    acc = (O.Result)()  # noqa: F841 (type check only)
+
+    class W(C.Structure):
+        _fields_ = [("n", C.c_uint64), ("mean", C.c_double), ("m2", C.c_double)]
+    L.orc_welford_add.argtypes = [C.POINTER(W), C.c_double]
+    L.orc_welford_merge.argtypes = [C.POINTER(W), C.POINTER(W)]
+    L.orc_welford_se.argtypes = [C.POINTER(W)]
+    L.orc_welford_se.restype = C.c_double
+    a = W(0, 0.0, 0.0)
+    for v in x:
+        L.orc_welford_add(C.byref(a), v)
+    assert abs(a.mean - x.mean()) < 1e-12
+    assert abs(a.m2 / (a.n - 1) - x.var(ddof=1)) < 1e-9
+    assert abs(L.orc_welford_se(C.byref(a)) - math.sqrt(x.var(ddof=1) / len(x))) < 1e-12
+    # merge of two halves == whole; empty is an exact identity (stats.hpp:47-55)
+    b, c = W(0, 0.0, 0.0), W(0, 0.0, 0.0)
+    for v in x[:4000]:
+        L.orc_welford_add(C.byref(b), v)
+    for v in x[4000:]:
+        L.orc_welford_add(C.byref(c), v)
+    L.orc_welford_merge(C.byref(b), C.byref(c))
+    assert b.n == a.n and abs(b.mean - a.mean) < 1e-12 and abs(b.m2 - a.m2) < 1e-8 * a.m2
+    e = W(0, 0.0, 0.0)
+    keep = (b.n, b.mean, b.m2)
+    L.orc_welford_merge(C.byref(b), C.byref(e))
+    assert (b.n, b.mean, b.m2) == keep
+    L.orc_welford_merge(C.byref(e), C.byref(b))
+    assert (e.n, e.mean, e.m2) == keep
+
+
+def test_chunk_bounds_partition():
+    b, e = C.c_uint64(), C.c_uint64()
+    for total, chunks in [(0, 8), (5, 8), (50_003, 8), (1_000_000, 11)]:
+        covered = 0
+        prev_end = 0
+        for c in range(chunks):
+            L.orc_chunk_bounds(c, total, chunks, C.byref(b), C.byref(e))
+            assert b.value == prev_end and e.value >= b.value
+            assert e.value - b.value in (total // chunks, total // chunks + 1)
+            covered += e.value - b.value
+            prev_end = e.value
+        assert covered == total
+
+
+# ----------------------------------------------------------------------------- MC prices
+def _product(kind, a):
+    common = dict(spot=a["s"], strike=a["k"], rate=a["r"], carry_yield=a["q"], vol=a["v"],
+                  time=a["t"], type=a["type"], antithetic=a.get("anti", 0))
+    if kind == "eur":
+        return O.product(O.EUROPEAN, **common)
+    if kind == "dig":
+        return O.product(O.DIGITAL, digital_style=a["style"], cash_amount=a["cash"], **common)
+    if kind == "asian":
+        return O.product(O.ASIAN, strike_style=a["ss"], average_style=a["avs"],
+                         monitoring_points=a["m"], control_variate=a["cv"], **common)
+    if kind == "barrier":
+        return O.product(O.BARRIER, barrier=a["h"], direction=a["dir"], knock=a["knock"],
+                         rebate=a["rebate"], monitoring_points=a["m"], brownian_bridge=a["bb"],
+                         **common)
+    return O.product(O.LOOKBACK, strike_style=a["style"], monitoring_points=a["m"], **common)
+
+
+@pytest.mark.parametrize("case", GOLD["mc"], ids=lambda c: c["name"])
+def test_mc_prices_bit_equal_to_reference_fixture(case):
+    p = _product(case["kind"], case["args"])
+    price, se, n = O.price(p, case["paths"], case["seed"], chunks=GOLD["chunk_count"], threads=4)
+    assert n == case["paths"]
+    assert price == fh(case["price"]) and se == fh(case["se"])
+
+
+def test_mc_price_independent_of_thread_count():
+    p = O.product(O.ASIAN, monitoring_points=12)
+    a = O.price(p, 20_011, 9, chunks=8, threads=1)
+    b = O.price(p, 20_011, 9, chunks=8, threads=8)
+    assert a == b
+
+
+def test_european_within_3se_of_bsm_and_parities():
+    bsm = L.orc_bsm(100, 100, 0.05, 0, 0.2, 1, 0)
+    assert abs(bsm - 10.450583572185565) < 1e-12       # C1 closed form
+    assert bsm == fh(GOLD["bsm_c1"])
+    p, se, _ = O.price(O.product(O.EUROPEAN), 200_000, 123)
+    assert abs(p - bsm) < 3 * se
+    # put-call parity of the closed form (reference tests/analytic_test.cpp:20-42)
+    c, pt = L.orc_bsm(100, 95, .05, .02, .3, .7, 0), L.orc_bsm(100, 95, .05, .02, .3, .7, 1)
+    assert abs((c - pt) - (100 * math.exp(-.02 * .7) - 95 * math.exp(-.05 * .7))) < 1e-12
+
+
+def test_in_plus_out_equals_vanilla_path_by_path():
+    kw = dict(barrier=110.0, monitoring_points=16)
+    n = 3000
+    pin = O.path_samples(O.product(O.BARRIER, knock=0, **kw), 5, 0, n)
+    pout = O.path_samples(O.product(O.BARRIER, knock=1, **kw), 5, 0, n)
+    van = O.path_samples(O.product(O.BARRIER, knock=1, barrier=1e300, monitoring_points=16), 5, 0, n)
+    assert np.array_equal(pin + pout, van)
+
+
+def test_analytic_values_match_reference_fixture():
+    assert L.orc_kemna_vorst(100, 100, .05, 0, .2, 1, 0) == fh(GOLD["kemna_vorst"])
+    assert L.orc_lookback_fixed(100, 90, .05, .01, .2, 1, 0) == fh(GOLD["lookback_fixed"])
+    assert L.orc_lookback_floating(100, .05, .01, .2, 1, 1) == fh(GOLD["lookback_floating"])
+    for h, t, d, k, v in GOLD["reiner_rubinstein"]:
+        assert L.orc_reiner_rubinstein(100, 100, h, .05, .01, .25, 1, t, d, k, 1.5) == fh(v)
+
+
+def test_reiner_rubinstein_in_out_parity():
+    for t in (0, 1):
+        for d, h in ((0, 120.0), (1, 80.0)):
+            tot = (L.orc_reiner_rubinstein(100, 100, h, .05, .01, .25, 1, t, d, 0, 0.0) +
+                   L.orc_reiner_rubinstein(100, 100, h, .05, .01, .25, 1, t, d, 1, 0.0))
+            assert abs(tot - L.orc_bsm(100, 100, .05, .01, .25, 1, t)) < 1e-10
+
+
+# ----------------------------------------------------------------------------- Sobol / bridge
+def test_sobol_dims_0_to_6_match_reference_fixture():
+    for d, pts in GOLD["sobol"].items():
+        for i, x in enumerate(pts):
+            assert L.orc_sobol_point(int(d), i) == fh(x)
+
+
+def test_sobol_every_dimension_is_stratified():
+    # Each dimension's first 2^m points hit every dyadic interval of width 2^-m once (true
+    # for any valid direction numbers; reference tests/sobol_test.cpp:61-81).
+    for d in list(range(7)) + [7, 30, 100, 251, 300]:
+        for m in (4, 8, 10):
+            pts = np.array([L.orc_sobol_int(d, i) for i in range(1 << m)], dtype=np.uint64)
+            cells = pts >> np.uint64(32 - m)
+            assert len(set(cells.tolist())) == 1 << m, (d, m)
+
+
+def test_sobol_direction_numbers_valid():
+    # v_k = m_k << (32-k) with m_k odd and m_k < 2^k (Bratley-Fox requirement)
+    for d in range(1, 260):
+        v = O.sobol_table(d)
+        for k in range(1, 33):
+            m = int(v[k - 1]) >> (32 - k)
+            assert m % 2 == 1 and m < (1 << k), (d, k)
+    # dimensions are pairwise distinct sequences
+    firsts = {tuple(int(L.orc_sobol_int(d, i)) for i in range(1, 256)) for d in range(1024)}
+    assert len(firsts) == 1024
+
+
+def test_brownian_bridge_is_the_exact_linear_map():
+    # W(T) = sqrt(T) z0 and Cov(W_i, W_j) = min(t_i, t_j): the bridge is a linear map A
+    # with A A^T = min(t_i, t_j); build A column by column from unit normals.
+    n, T = 7, 1.3
+    A = np.zeros((n, n))
+    sp = np.zeros(n)
+    for j in range(n):
+        z = np.zeros(n)
+        z[j] = 1.0
+        # r=q=0, vol=1, spot=1: spots = exp(-0.5 t + W) -> W = log(spots) + 0.5 t
+        L.orc_brownian_bridge_path(1.0, 0.0, 0.0, 1.0, T, n, z.ctypes.data_as(C.POINTER(C.c_double)),
+                                   sp.ctypes.data_as(C.POINTER(C.c_double)))
+        t = np.arange(1, n + 1) * T / n
+        A[:, j] = np.log(sp) + 0.5 * t
+    t = np.arange(1, n + 1) * T / n
+    assert np.allclose(A @ A.T, np.minimum.outer(t, t), atol=1e-12)
+    assert abs(A[n - 1, 0] - math.sqrt(T)) < 1e-14 and np.allclose(A[n - 1, 1:], 0)
+
+
+def test_qmc_prices_match_reference_fixture():
+    q = GOLD["qmc"]
+    assert L.orc_qmc_sobol_european(100, 100, .05, .02, .2, 1, 0, 20000) == fh(q["european"])
+    assert L.orc_qmc_sobol_asian(100, 100, .05, .02, .25, 1, 0, 0, 7, 20000) == fh(q["asian7"])
+    assert L.orc_qmc_sobol_asian(100, 95, .05, 0, .3, 1, 1, 1, 5, 5000) == fh(q["asian5_float_put"])
+
+
+# ----------------------------------------------------------------------------- Heston
+LEWIS_CALLS = {80.0: 26.774758743998854221, 90.0: 20.933349000596710388,
+               100.0: 16.070154917028834278, 110.0: 12.132211516709844867,
+               120.0: 9.024913483457835636}   # Lewis reference values (heston_test.cpp:45-62)
+
+
+def test_heston_semi_analytic_lewis_values():
+    h = O.heston(spot=100, rate=0.01, carry_yield=0.02, v0=0.04, kappa=4, theta=0.25, xi=1,
+                 rho=-0.5, time=1)
+    for k, v in LEWIS_CALLS.items():
+        got = L.orc_heston_semi_analytic(C.byref(h), k, 0)
+        assert abs(got - v) < 1e-9
+        assert abs(got - fh(GOLD["heston_semi_analytic"][str(k)])) < 1e-12
+
+
+def test_heston_qe_matches_reference_fixture_bits():
+    h = O.heston(spot=100, rate=0.01, carry_yield=0.02, v0=0.04, kappa=4, theta=0.25, xi=1,
+                 rho=-0.5, time=1)
+    g = GOLD["heston_qe"]
+    r = L.orc_heston_qe_european(C.byref(h), g["strike"], 0, g["steps"], g["paths"], g["seed"])
+    assert r.price == fh(g["price"]) and r.standard_error == fh(g["se"])
+
+
+def test_qe_variance_step_moments():
+    # QE reproduces the CIR conditional mean and variance (Andersen 2008), both branches.
+    kappa, theta, xi, dt = 2.0, 0.04, 0.5, 1 / 252
+    for v in (0.04, 0.0001):
+        us = (np.arange(200000) + 0.5) / 200000
+        vn = np.array([L.orc_qe_variance_step(v, kappa, theta, xi, dt, u) for u in us])
+        e = math.exp(-kappa * dt)
+        m = theta + (v - theta) * e
+        s2 = v * xi * xi * e / kappa * (1 - e) + theta * xi * xi / (2 * kappa) * (1 - e) ** 2
+        assert abs(vn.mean() - m) < 2e-3 * m
+        assert abs(vn.var() - s2) < 2e-2 * s2
+        assert vn.min() >= 0.0
+
+
+def test_gauss_legendre_exact_for_polynomials():
+    n = 16
+    x, w = np.zeros(n), np.zeros(n)
+    L.orc_gauss_legendre_rule(n, x.ctypes.data_as(C.POINTER(C.c_double)),
+                              w.ctypes.data_as(C.POINTER(C.c_double)))
+    for k in range(2 * n):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(np.dot(w, x ** k) - exact) < 1e-13
+
+
+# ----------------------------------------------------------------------------- LSM / QR / CRR
+def test_householder_textbook_systems():
+    # exact square system and an overdetermined line fit (linalg_test.cpp:39-55 style)
+    A = np.array([[2.0, 1.0], [1.0, 3.0]])
+    y = np.array([3.0, 5.0])
+    beta = np.zeros(2)
+    a = A.copy().ravel()
+    L.orc_householder_ls(a.ctypes.data_as(C.POINTER(C.c_double)), 2, 2,
+                         y.copy().ctypes.data_as(C.POINTER(C.c_double)),
+                         beta.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.allclose(beta, [0.8, 1.4], atol=1e-14)
+    rng = np.random.default_rng(1)
+    X = np.column_stack([np.ones(50), rng.normal(size=50), rng.normal(size=50) ** 2])
+    t = X @ [1.0, -2.0, 0.5] + 0.01 * rng.normal(size=50)
+    beta = np.zeros(3)
+    L.orc_householder_ls(X.copy().ravel().ctypes.data_as(C.POINTER(C.c_double)), 50, 3,
+                         t.copy().ctypes.data_as(C.POINTER(C.c_double)),
+                         beta.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.allclose(beta, np.linalg.lstsq(X, t, rcond=None)[0], atol=1e-12)
+
+
+def test_lsm_matches_reference_fixture_bits():
+    g = GOLD["lsm"]
+    p, se, co, has, inc = O.lsm(*g["args"])
+    assert p == fh(g["price"]) and se == fh(g["se"]) and inc == g["inception"]
+    assert has.tolist() == g["has"]
+    assert all(x == fh(y) for x, y in zip(co.ravel(), g["coeffs"]))
+
+
+def test_crr_american_fixture_and_european_limit():
+    for n, v in GOLD["crr_american"].items():
+        assert L.orc_crr_american(100, 100, .06, 0, .2, 1, int(n), 1) == fh(v)
+    # CRR European converges to BSM; American call on no-dividend stock = European
+    bsm = L.orc_bsm(100, 100, .05, 0, .2, 1, 0)
+    assert abs(L.orc_crr(100, 100, .05, 0, .2, 1, 2000, 0, None, None, None) - bsm) < 5e-3
+    assert abs(L.orc_crr_american(100, 100, .05, 0, .2, 1, 500, 0) -
+               L.orc_crr(100, 100, .05, 0, .2, 1, 500, 0, None, None, None)) < 1e-12
+
+
+# ----------------------------------------------------------------------------- Greeks
+def test_pathwise_and_lr_match_reference_fixture_bits():
+    g = (O.Greek * 3)()
+    L.orc_pathwise_european(100, 100, .05, 0, .2, 1, 0, 20_000, 42, g)
+    want = [fh(x) for x in GOLD["pathwise_european"]]
+    assert [v for gg in g for v in (gg.value, gg.standard_error)] == want
+    g5 = (O.Greek * 5)()
+    L.orc_lr_terminal(100, 100, .05, 0, .2, 1, 0, 0, 0, 1.0, 20_000, 42, g5)
+    want = [fh(x) for x in GOLD["lr_european"]]
+    assert [v for gg in g5 for v in (gg.value, gg.standard_error)] == want
+
+
+def test_pathwise_delta_near_bsm_delta():
+    g = (O.Greek * 3)()
+    L.orc_pathwise_european(100, 100, .05, 0, .2, 1, 0, 200_000, 7, g)
+    d1 = (math.log(1) + (0.05 + 0.02) * 1) / 0.2
+    delta = 0.5 * math.erfc(-d1 / math.sqrt(2))
+    assert abs(g[0].value - delta) < 3 * g[0].standard_error
+
+
+# ----------------------------------------------------------------------------- live reference
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference tree here)")
+def test_oracle_equals_live_reference_on_random_cases():
+    R = O.ref()
+    rng = np.random.default_rng(2)
+    res = O.Result()
+    for _ in range(6):
+        s, k = rng.uniform(80, 120), rng.uniform(80, 120)
+        v, t = rng.uniform(.1, .5), rng.uniform(.2, 2)
+        m = int(rng.integers(1, 30))
+        R.ref_mc_asian(s, k, .03, .01, v, t, 0, 0, 0, m, 3001, 17, 0, 0, 1, C.byref(res))
+        p = O.product(O.ASIAN, spot=s, strike=k, rate=.03, carry_yield=.01, vol=v, time=t,
+                      monitoring_points=m)
+        assert O.price(p, 3001, 17, chunks=R.ref_chunk_count(), threads=2)[:2] == (res.price, res.standard_error)
+        for draw in (0, 5):
+            for path in rng.integers(0, 1 << 40, 50):
+                assert L.orc_standard_normal_variate(3, int(path), draw) == \
+                    R.ref_standard_normal_variate(3, int(path), draw)
