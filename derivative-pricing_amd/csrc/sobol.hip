@@ -1,0 +1,597 @@
+// Sobol sequences, Brownian-bridge path construction and the QMC pricers.
+//
+// Reference behaviour restated here:
+//   sobol_point            src/core/sobol.cpp:83-97 (Gray code, x = XOR of v_{b+1} over set
+//                          bits b of i^(i>>1), u = x / 2^32, index 0 -> 0)
+//   direction numbers      src/core/sobol.cpp:22-68 (dims 0-6: the reference's polynomials
+//                          with m_k = 1, Bratley-Fox recurrence)
+//   bridge                 src/models/brownian_bridge_path.cpp:20-77 (BFS bisection: draw 0
+//                          drives W(T), draw 1 the midpoint, ...)
+//   qmc_sobol_european     src/pricers/qmc.cpp:15-30
+//   qmc_sobol_asian        src/pricers/qmc.cpp:32-57 (capped at 7 dimensions there)
+//
+// Dimensions 7 and up have no reference definition. This engine extends the table to 1024
+// dimensions: primitive polynomials (LFSR maximal-period test, the reference's own method)
+// in order of degree and coefficient bits, skipping the six the reference uses, with
+// initial direction integers m_1..m_s odd and < 2^k drawn from Philox key 0x50B01 at
+// counter {dim, 0, k, 3 + attempt}; an attempt is kept only when the dimension's first
+// eight direction integers differ from every earlier dimension's.
+//
+// Kernel layout: one lane per Sobol index. A wave covers 64 consecutive indices, so the
+// Gray code's bits >= 6 are wave-uniform: x = Xhi(dim) ^ Xlo(dim, lane), where Xhi folds
+// up to 26 direction words on the scalar unit and Xlo at most six per lane. The bridge is
+// replayed from a host-built depth-first program over a few LDS slots per lane (depth of
+// the bisection tree + 2 values live at once), and each bridge point's spot feeds the
+// average as soon as it exists, so no lane ever holds the 252-point path.
+#include <math.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace mcdx {
+
+constexpr unsigned kSobolRefDims = 7;
+constexpr unsigned kSobolDims = 1024;
+constexpr int kMaxBridgeSlots = 24;
+
+// ---------------------------------------------------------------------------
+// Direction tables (host)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Poly {
+    unsigned degree;
+    uint32_t interior; // bit j-1 = a_j (sobol.cpp:14-17)
+};
+const Poly kRefPolys[kSobolRefDims] = {{0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 1}, {5, 2}, {6, 1}};
+
+bool primitive(unsigned s, uint32_t interior) {
+    const uint32_t poly = (1u << s) | (interior << 1) | 1u;
+    const uint32_t period = (1u << s) - 1;
+    uint32_t state = 1;
+    for (uint32_t i = 1; i <= period; ++i) {
+        state <<= 1;
+        if (state & (1u << s)) state ^= poly;
+        if (state == 1) return i == period;
+    }
+    return false;
+}
+
+void fill_directions(const Poly& p, const uint32_t* m_init, uint32_t* v /* [32], v[k-1] */) {
+    uint32_t m[33] = {0};
+    const unsigned s = p.degree;
+    if (s == 0) {
+        for (unsigned k = 1; k <= 32; ++k) m[k] = 1;
+    } else {
+        for (unsigned k = 1; k <= s; ++k) m[k] = m_init[k - 1];
+        for (unsigned k = s + 1; k <= 32; ++k) { // sobol.cpp:52-59
+            uint32_t mk = m[k - s] ^ (m[k - s] << s);
+            for (unsigned j = 1; j < s; ++j)
+                if ((p.interior >> (j - 1)) & 1u) mk ^= (m[k - j] << j);
+            m[k] = mk;
+        }
+    }
+    for (unsigned k = 1; k <= 32; ++k) v[k - 1] = m[k] << (32 - k);
+}
+
+std::vector<uint32_t> build_tables() {
+    std::vector<uint32_t> v(kSobolDims * 32);
+    uint32_t ones[32];
+    for (auto& o : ones) o = 1;
+    for (unsigned d = 0; d < kSobolRefDims; ++d) fill_directions(kRefPolys[d], ones, &v[d * 32]);
+    unsigned d = kSobolRefDims;
+    for (unsigned s = 1; s <= 16 && d < kSobolDims; ++s) {
+        const uint32_t ncoef = s > 1 ? (1u << (s - 1)) : 1u;
+        for (uint32_t c = 0; c < ncoef && d < kSobolDims; ++c) {
+            if (!primitive(s, c)) continue;
+            bool used = false;
+            for (unsigned r = 1; r < kSobolRefDims; ++r)
+                used = used || (kRefPolys[r].degree == s && kRefPolys[r].interior == c);
+            if (used) continue;
+            for (uint32_t attempt = 0;; ++attempt) {
+                uint32_t m_init[32];
+                for (unsigned k = 1; k <= s; ++k) {
+                    const uint32_t bits = philox_bits(0x50B01ull, d, k, 3 + attempt);
+                    m_init[k - 1] = (k >= 32 ? bits : (bits & ((1u << k) - 1u))) | 1u;
+                }
+                fill_directions(Poly{s, c}, m_init, &v[d * 32]);
+                bool clash = false;
+                for (unsigned e = 0; e < d && !clash; ++e)
+                    clash = memcmp(&v[e * 32], &v[d * 32], 8 * sizeof(uint32_t)) == 0;
+                if (!clash) break;
+            }
+            ++d;
+        }
+    }
+    return v;
+}
+
+std::mutex g_table_mu;
+uint32_t* g_dev_tables[64] = {nullptr}; // per device, created once, kept for the process
+
+int device_tables(const uint32_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return status_of(e);
+    if (dev < 0 || dev >= 64) return MCDX_EDEVICE;
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    if (!g_dev_tables[dev]) {
+        const std::vector<uint32_t> host = build_tables();
+        uint32_t* d = nullptr;
+        e = hipMalloc(&d, host.size() * sizeof(uint32_t));
+        if (e != hipSuccess) return status_of(e);
+        e = hipMemcpy(d, host.data(), host.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return status_of(e);
+        }
+        g_dev_tables[dev] = d;
+    }
+    *out = g_dev_tables[dev];
+    return MCDX_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Bridge program (host): depth-first replay of the reference's BFS bisection
+// ---------------------------------------------------------------------------
+struct BridgeOp {
+    double frac;     // (t_mid - t_left) / (t_right - t_left)
+    double sd;       // sqrt((t_mid - t_left)(t_right - t_mid) / (t_right - t_left))
+    double drift_t;  // (r - q - vol^2/2) * t_{point+1}
+    float frac_f, sd_f, drift2_f, pad;
+    uint16_t dim;    // Sobol dimension = the reference's draw index for this split
+    uint8_t sl, sr, so; // LDS slots: left endpoint, right endpoint, output
+    uint8_t pad2[3];
+};
+
+namespace {
+
+struct Interval {
+    int l, r;
+};
+
+// Returns the number of slots used (slot 0 holds W(0) = 0), or -1 if more than allowed.
+int build_bridge(int n, double spot_unused, double rate, double carry, double vol, double time,
+                 std::vector<BridgeOp>& ops) {
+    (void)spot_unused;
+    std::vector<double> t(n + 1);
+    for (int i = 0; i <= n; ++i) t[i] = (double)i * time / (double)n; // bridge.cpp:23-26
+    // BFS draw assignment exactly as bridge.cpp:33-66.
+    std::vector<int> draw_of_mid(n, -1);
+    {
+        std::vector<Interval> q;
+        q.reserve(2 * n + 1);
+        size_t head = 0;
+        q.push_back({-1, n - 1});
+        int draw = 1;
+        while (head < q.size() && draw < n) {
+            const Interval iv = q[head++];
+            if (iv.r - iv.l <= 1) continue;
+            const int mid = (iv.l + iv.r) / 2;
+            draw_of_mid[mid] = draw++;
+            q.push_back({iv.l, mid});
+            q.push_back({mid, iv.r});
+        }
+    }
+    const double drift_rate = rate - carry - 0.5 * vol * vol;
+    const double log2e = 1.44269504088896340735992468100189214;
+    auto t_of = [&](int idx) { return idx == -1 ? 0.0 : t[idx + 1]; };
+    // Slot allocator: slot 0 = origin; a node's slot lives until both children are done.
+    std::vector<int> free_slots;
+    for (int s = kMaxBridgeSlots - 1; s >= 1; --s) free_slots.push_back(s);
+    int used = 1;
+    bool overflow = false;
+    auto alloc = [&]() {
+        if (free_slots.empty()) {
+            overflow = true;
+            return 1;
+        }
+        const int s = free_slots.back();
+        free_slots.pop_back();
+        used = s + 1 > used ? s + 1 : used;
+        return s;
+    };
+    auto emit = [&](int dim, int sl, int sr, int so, double frac, double sd, int point) {
+        BridgeOp op{};
+        op.frac = frac;
+        op.sd = sd;
+        op.drift_t = drift_rate * t[point + 1];
+        op.frac_f = (float)frac;
+        op.sd_f = (float)sd;
+        op.drift2_f = (float)(op.drift_t * log2e);
+        op.dim = (uint16_t)dim;
+        op.sl = (uint8_t)sl;
+        op.sr = (uint8_t)sr;
+        op.so = (uint8_t)so;
+        ops.push_back(op);
+    };
+    // Terminal point: w[n-1] = sqrt(t[n]) * z_0 (bridge.cpp:30).
+    const int s_term = alloc();
+    emit(0, 0, 0, s_term, 0.0, sqrt(t[n]), n - 1);
+    // Depth-first over the same intervals; slot_of[idx+1] tracks live endpoint slots.
+    std::vector<int> slot_of(n + 1, -1);
+    slot_of[0] = 0;
+    slot_of[n] = s_term;
+    struct Frame {
+        int l, r, stage;
+    };
+    std::vector<Frame> st;
+    st.push_back({-1, n - 1, 0});
+    while (!st.empty()) {
+        Frame& f = st.back();
+        if (f.r - f.l <= 1) {
+            st.pop_back();
+            continue;
+        }
+        const int mid = (f.l + f.r) / 2;
+        if (f.stage == 0) {
+            const double tl = t_of(f.l), tr = t_of(f.r), tm = t_of(mid);
+            const double frac = (tm - tl) / (tr - tl);
+            const double var = (tm - tl) * (tr - tm) / (tr - tl);
+            const int so = alloc();
+            emit(draw_of_mid[mid], slot_of[f.l + 1], slot_of[f.r + 1], so, frac, sqrt(var), mid);
+            slot_of[mid + 1] = so;
+            f.stage = 1;
+            st.push_back({f.l, mid, 0});
+        } else if (f.stage == 1) {
+            f.stage = 2;
+            st.push_back({mid, f.r, 0});
+        } else {
+            free_slots.push_back(slot_of[mid + 1]);
+            slot_of[mid + 1] = -1;
+            st.pop_back();
+        }
+    }
+    return overflow ? -1 : used;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Device side
+// ---------------------------------------------------------------------------
+// XOR of the direction words of `dim` over the set bits of `g`.
+__device__ __forceinline__ uint32_t sobol_fold(const uint32_t* __restrict__ v, uint32_t g) {
+    uint32_t x = 0;
+    while (g) {
+        const int b = __builtin_ctz(g);
+        x ^= v[b];
+        g &= g - 1;
+    }
+    return x;
+}
+
+// Sobol integer for (dim, i) given the wave-uniform high Gray bits and this lane's low bits.
+__device__ __forceinline__ uint32_t sobol_x(const uint32_t* __restrict__ v, uint32_t g_hi,
+                                            uint32_t g_lo) {
+    // g_hi is wave-uniform: read it as such so the fold runs on the scalar unit.
+    const uint32_t gh = __builtin_amdgcn_readfirstlane(g_hi);
+    uint32_t xh = 0;
+    for (uint32_t g = gh; g; g &= g - 1) xh ^= v[__builtin_ctz(g)];
+    uint32_t xl = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint32_t mask = 0u - ((g_lo >> b) & 1u);
+        xl ^= mask & v[b];
+    }
+    return xh ^ xl;
+}
+
+// u = x / 2^32 (sobol.cpp:96), inverse normal at the requested precision. In FP32 mode the
+// branch uses integer thresholds equal to the FP64 comparison: x/2^32 >= 0.02425 <=>
+// x >= 104152957, x/2^32 <= 0.97575 <=> x <= 4190814339.
+__device__ __forceinline__ float sobol_normal_f32(uint32_t x) {
+    if (x >= 104152957u && x <= 4190814339u) {
+        MCDX_ACKLAM_CENTRAL(double, MCDX_F64)
+        const double q = (double)(int32_t)(x ^ 0x80000000u) * 0x1p-32; // u - 0.5, exact
+        const double r = q * q;
+        double num = __builtin_fma(A1, r, A2);
+        num = __builtin_fma(num, r, A3);
+        num = __builtin_fma(num, r, A4);
+        num = __builtin_fma(num, r, A5);
+        num = __builtin_fma(num, r, A6);
+        double den = __builtin_fma(B1, r, B2);
+        den = __builtin_fma(den, r, B3);
+        den = __builtin_fma(den, r, B4);
+        den = __builtin_fma(den, r, B5);
+        den = __builtin_fma(den, r, 1.0);
+        return (float)(num * q * fast_rcp_f64(den));
+    }
+    MCDX_ACKLAM_TAIL(float, MCDX_F32)
+    const bool lower = x < 104152957u;
+    // lower: u = x / 2^32; upper: 1 - u = (2^32 - x) / 2^32 = (~x + 1) / 2^32
+    const float ut = lower ? (float)x * 0x1p-32f : ((float)(~x) + 1.0f) * 0x1p-32f;
+    const float z = fast_sqrt(-1.38629436111989061883f * fast_log2(ut));
+    float num = __builtin_fmaf(C1, z, C2);
+    num = __builtin_fmaf(num, z, C3);
+    num = __builtin_fmaf(num, z, C4);
+    num = __builtin_fmaf(num, z, C5);
+    num = __builtin_fmaf(num, z, C6);
+    float den = __builtin_fmaf(D1, z, D2);
+    den = __builtin_fmaf(den, z, D3);
+    den = __builtin_fmaf(den, z, D4);
+    den = __builtin_fmaf(den, z, 1.0f);
+    const float res = num * fast_rcp(den);
+    return lower ? res : -res;
+}
+
+struct QmcArgs {
+    double spot, strike, discount, drift, vsd, vol;
+    float vol2, pad;       // FP32: vol * log2(e)
+    float mu2, sig2;       // FP32 terminal: drift*log2e, vsd*log2e
+    uint64_t n_points;     // Sobol indices 1..n_points
+    const uint32_t* v;     // direction table [dims][32]
+    const BridgeOp* ops;   // bridge program (Asian)
+    int32_t n_ops, m, slots;
+    uint8_t type, strike_style, european;
+    Welford* partials;
+};
+
+__device__ __forceinline__ double smax_d(double a, double b) { return (a < b) ? b : a; }
+
+// One point of the QMC pricer (index i >= 1). FP64 mode mirrors qmc.cpp / bridge.cpp.
+template <bool FP64>
+__device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_t g_lo,
+                            float* lds_f, double* lds_d, int tid, int nthreads) {
+    const double phi = a.type == 0 ? 1.0 : -1.0;
+    if (a.european) { // qmc.cpp:24-28, dimension 0
+        const uint32_t x = sobol_x(a.v, g_hi, g_lo);
+        double st;
+        if constexpr (FP64) {
+            const double z = inv_normal_f64((double)x / 4294967296.0);
+            st = a.spot * exp(a.drift + a.vsd * z);
+        } else {
+            const float z = sobol_normal_f32(x);
+            st = a.spot * (double)fast_exp2(__builtin_fmaf(a.sig2, z, a.mu2));
+        }
+        return a.discount * smax_d(phi * (st - a.strike), 0.0);
+    }
+    // Asian over the bridge (qmc.cpp:44-54): replay the depth-first program.
+    double sum = 0.0, last = 0.0;
+    if constexpr (FP64) lds_d[tid] = 0.0;
+    else lds_f[tid] = 0.f;
+    for (int k = 0; k < a.n_ops; ++k) {
+        const BridgeOp& op = a.ops[k];
+        const uint32_t x = sobol_x(a.v + 32 * op.dim, g_hi, g_lo);
+        if constexpr (FP64) {
+            const double z = inv_normal_f64((double)x / 4294967296.0);
+            const double wl = lds_d[op.sl * nthreads + tid];
+            const double wr = lds_d[op.sr * nthreads + tid];
+            const double mean = wl + op.frac * (wr - wl);
+            const double w = mean + op.sd * z;
+            lds_d[op.so * nthreads + tid] = w;
+            const double s = a.spot * exp(op.drift_t + a.vol * w);
+            sum += s;
+            if (k == 0) last = s;
+        } else {
+            const float z = sobol_normal_f32(x);
+            const float wl = lds_f[op.sl * nthreads + tid];
+            const float wr = lds_f[op.sr * nthreads + tid];
+            const float w = __builtin_fmaf(op.sd_f, z, __builtin_fmaf(op.frac_f, wr - wl, wl));
+            lds_f[op.so * nthreads + tid] = w;
+            const float su = fast_exp2(__builtin_fmaf(a.vol2, w, op.drift2_f));
+            sum += (double)su;
+            if (k == 0) last = (double)su;
+        }
+    }
+    if constexpr (!FP64) {
+        sum *= a.spot;
+        last *= a.spot;
+    }
+    const double avg = sum / (double)a.m;
+    const double ref = a.strike_style == 0 ? (avg - a.strike) : (last - avg);
+    return a.discount * smax_d(phi * ref, 0.0);
+}
+
+// Tile of 1024 Sobol indices per block (256 lanes x 4); index i = tile*1024 + k*256 + lane
+// and the points priced are indices 1..n_points (index 0 is skipped, qmc.cpp:24). A wave's
+// 64 indices are 64-aligned, so Gray-code bits >= 6 are wave-uniform. Welford over points as
+// every MC kernel.
+template <bool FP64>
+__global__ __launch_bounds__(kTileLanes) void qmc_tile_kernel(QmcArgs a) {
+    extern __shared__ unsigned char smem[];
+    __shared__ Welford lds4[4];
+    float* lds_f = (float*)smem;
+    double* lds_d = (double*)smem;
+    const uint64_t tile = blockIdx.x;
+    Welford w = {0, 0.0, 0.0};
+#pragma unroll 1
+    for (int k = 0; k < kPathsPerLane; ++k) {
+        const uint64_t wave_base = tile * kTilePaths + (uint64_t)(k * kTileLanes) +
+                                   (threadIdx.x & ~63u);
+        if (wave_base > a.n_points) continue; // wave-uniform
+        const uint64_t i_raw = wave_base + (threadIdx.x & 63u);
+        const bool valid = i_raw >= 1 && i_raw <= a.n_points;
+        const uint64_t i = valid ? i_raw : (wave_base | 1u); // finite stand-in, discarded
+        const uint64_t gray = i ^ (i >> 1);
+        const uint32_t g_lo = (uint32_t)(gray & 63u);
+        const uint32_t g_hi = (uint32_t)(((wave_base | 1u) ^ ((wave_base | 1u) >> 1)) >> 6) << 6;
+        const double x = qmc_point<FP64>(a, i, g_hi, g_lo, lds_f, lds_d, threadIdx.x, kTileLanes);
+        if (valid) {
+            w.n += 1;
+            const double d = x - w.mean;
+            w.mean += d / (double)w.n;
+            w.m2 += d * (x - w.mean);
+        }
+    }
+    const Welford t = block_merge_256(w, lds4);
+    if (threadIdx.x == 0) a.partials[tile] = t;
+}
+
+__global__ __launch_bounds__(256) void sobol_ints_kernel(const uint32_t* __restrict__ v,
+                                                         uint64_t index0, uint64_t n,
+                                                         uint32_t* out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t i = index0 + k;
+    out[k] = i == 0 ? 0u : sobol_fold(v, (uint32_t)(i ^ (i >> 1)));
+}
+
+} // namespace mcdx
+
+namespace mcdx {
+namespace {
+
+int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slots,
+            mcdx_result* out) {
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    if ((st = device_tables(&a.v))) return st;
+    // Sobol indices 0 .. n_points live in tiles of 1024 (index 0 skipped).
+    const uint64_t tiles = (a.n_points + 1 + kTilePaths - 1) / kTilePaths;
+    const size_t ops_off = (tiles + 1) * sizeof(Welford);
+    const size_t ops_bytes = ops.size() * sizeof(BridgeOp);
+    if ((st = ensure_dev(ctx, ops_off + ops_bytes + 256))) return st;
+    if ((st = ensure_pinned(ctx, sizeof(Welford)))) return st;
+    Welford* partials = (Welford*)ctx->dev;
+    BridgeOp* dops = (BridgeOp*)((char*)ctx->dev + ops_off);
+    hipError_t e = hipSuccess;
+    if (ops_bytes)
+        e = hipMemcpyAsync(dops, ops.data(), ops_bytes, hipMemcpyHostToDevice, ctx->stream);
+    a.ops = dops;
+    a.n_ops = (int32_t)ops.size();
+    a.slots = slots;
+    a.partials = partials;
+    const size_t shmem = (size_t)(slots > 0 ? slots : 1) * kTileLanes *
+                         (precision == MCDX_PREC_FP64 ? sizeof(double) : sizeof(float));
+    if (e == hipSuccess) {
+        if (precision == MCDX_PREC_FP64)
+            hipLaunchKernelGGL(qmc_tile_kernel<true>, dim3((unsigned)tiles), dim3(kTileLanes), shmem,
+                               ctx->stream, a);
+        else
+            hipLaunchKernelGGL(qmc_tile_kernel<false>, dim3((unsigned)tiles), dim3(kTileLanes), shmem,
+                               ctx->stream, a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = launch_final_merge(partials, nullptr, tiles, 1, partials + tiles, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ctx->pinned, partials + tiles, sizeof(Welford), hipMemcpyDeviceToHost,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return status_of(e);
+    *out = finish(*(const Welford*)ctx->pinned);
+    return MCDX_OK;
+}
+
+QmcArgs qmc_args(double spot, double strike, double rate, double carry, double vol, double time,
+                 uint8_t type, uint64_t n) {
+    QmcArgs a{};
+    const double log2e = 1.44269504088896340735992468100189214;
+    a.spot = spot;
+    a.strike = strike;
+    a.discount = exp(-rate * time);
+    a.drift = (rate - carry - 0.5 * vol * vol) * time; // gbm.cpp:8-9 with T
+    a.vsd = vol * sqrt(time);
+    a.vol = vol;
+    a.vol2 = (float)(vol * log2e);
+    a.mu2 = (float)(a.drift * log2e);
+    a.sig2 = (float)(a.vsd * log2e);
+    a.n_points = n;
+    a.type = type;
+    return a;
+}
+
+} // namespace
+
+int qmc_asian(double spot, double strike, double rate, double carry, double vol, double time,
+              uint8_t type, uint8_t strike_style, int32_t m, uint64_t n, int precision,
+              mcdx_result* out) {
+    if (!out || type > 1 || strike_style > 1 || m < 1 || m > (int32_t)kSobolDims) return MCDX_EINVAL;
+    if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
+    if (n > (1ull << 32) - 2) return MCDX_EINVAL;
+    if (n == 0) { // reference: sum / 0 (qmc.cpp:56)
+        *out = mcdx_result{NAN, 0.0, 0};
+        return MCDX_OK;
+    }
+    std::vector<BridgeOp> ops;
+    const int slots = build_bridge(m, spot, rate, carry, vol, time, ops);
+    if (slots < 0) return MCDX_EINVAL;
+    QmcArgs a = qmc_args(spot, strike, rate, carry, vol, time, type, n);
+    a.m = m;
+    a.strike_style = strike_style;
+    a.european = 0;
+    return qmc_run(a, precision, ops, slots, out);
+}
+
+} // namespace mcdx
+
+using namespace mcdx;
+
+extern "C" {
+
+uint32_t mcdx_sobol_max_dimensions(void) { return kSobolRefDims; }
+uint32_t mcdx_sobol_extended_dimensions(void) { return kSobolDims; }
+
+int mcdx_sobol_ints(uint32_t dim, uint64_t index0, uint64_t n, uint32_t* out) {
+    if (dim >= kSobolDims || (!out && n)) return MCDX_EINVAL;
+    if (n == 0) return MCDX_OK;
+    if (n > (1ull << 31) || index0 + n > (1ull << 32)) return MCDX_EINVAL;
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    const uint32_t* v;
+    if ((st = device_tables(&v))) return st;
+    if ((st = ensure_dev(ctx, n * 4))) return st;
+    hipLaunchKernelGGL(sobol_ints_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, v + 32 * dim, index0, n, (uint32_t*)ctx->dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->dev, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return status_of(e);
+}
+
+int mcdx_qmc_sobol_european_p(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, uint64_t path_count,
+                              int precision, mcdx_result* out) {
+    if (!out || type > 1) return MCDX_EINVAL;
+    if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
+    if (path_count > (1ull << 32) - 2) return MCDX_EINVAL;
+    if (path_count == 0) {
+        *out = mcdx_result{NAN, 0.0, 0};
+        return MCDX_OK;
+    }
+    QmcArgs a = qmc_args(spot, strike, rate, carry_yield, vol, time, type, path_count);
+    a.european = 1;
+    a.m = 1;
+    std::vector<BridgeOp> none;
+    return qmc_run(a, precision, none, 0, out);
+}
+
+int mcdx_qmc_sobol_european(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, uint64_t path_count,
+                            double* price) {
+    if (!price) return MCDX_EINVAL;
+    mcdx_result r;
+    const int st = mcdx_qmc_sobol_european_p(spot, strike, rate, carry_yield, vol, time, type,
+                                             path_count, mcdx_get_precision(), &r);
+    if (st == MCDX_OK) *price = r.price;
+    return st;
+}
+
+int mcdx_qmc_sobol_asian(double spot, double strike, double rate, double carry_yield, double vol,
+                         double time, uint8_t type, uint8_t strike_style,
+                         int32_t monitoring_points, uint64_t path_count, double* price) {
+    if (!price || monitoring_points < 1 || monitoring_points > (int32_t)kSobolRefDims)
+        return MCDX_EINVAL;
+    mcdx_result r;
+    const int st = qmc_asian(spot, strike, rate, carry_yield, vol, time, type, strike_style,
+                             monitoring_points, path_count, mcdx_get_precision(), &r);
+    if (st == MCDX_OK) *price = r.price;
+    return st;
+}
+
+int mcdx_qmc_sobol_asian_extended(double spot, double strike, double rate, double carry_yield,
+                                  double vol, double time, uint8_t type, uint8_t strike_style,
+                                  int32_t monitoring_points, uint64_t path_count, int precision,
+                                  mcdx_result* out) {
+    return qmc_asian(spot, strike, rate, carry_yield, vol, time, type, strike_style,
+                     monitoring_points, path_count, precision, out);
+}
+
+} // extern "C"

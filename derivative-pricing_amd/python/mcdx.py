@@ -75,6 +75,14 @@ SIGNATURES = {
     "mcdx_price_gbm": (C.c_int, [P(GbmContract), C.c_int, P(Result)]),
     "mcdx_price_gbm_enqueue": (C.c_int, [P(GbmContract), C.c_int, C.c_void_p, C.c_void_p]),
     "mcdx_gbm_path_samples": (C.c_int, [P(GbmContract), C.c_int, u64, u64, P(dbl)]),
+    "mcdx_sobol_max_dimensions": (u32, []),
+    "mcdx_sobol_extended_dimensions": (u32, []),
+    "mcdx_sobol_ints": (C.c_int, [u32, u64, u64, P(u32)]),
+    "mcdx_qmc_sobol_european": (C.c_int, [dbl] * 6 + [u8, u64, P(dbl)]),
+    "mcdx_qmc_sobol_european_p": (C.c_int, [dbl] * 6 + [u8, u64, C.c_int, P(Result)]),
+    "mcdx_qmc_sobol_asian": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, P(dbl)]),
+    "mcdx_qmc_sobol_asian_extended": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, C.c_int,
+                                                            P(Result)]),
 }
 
 _lib = None
@@ -129,6 +137,27 @@ def path_samples(c, precision, path0, n):
     check(lib().mcdx_gbm_path_samples(C.byref(c), precision, path0, n,
                                       out.ctypes.data_as(P(dbl))), "mcdx_gbm_path_samples")
     return out
+
+
+def sobol_ints(dim, index0, n):
+    out = np.empty(n, dtype=np.uint32)
+    check(lib().mcdx_sobol_ints(dim, index0, n, out.ctypes.data_as(P(u32))), "mcdx_sobol_ints")
+    return out
+
+
+def qmc_asian(spot, strike, rate, carry, vol, time, type, strike_style, m, n, precision):
+    r = Result()
+    check(lib().mcdx_qmc_sobol_asian_extended(spot, strike, rate, carry, vol, time, type,
+                                              strike_style, m, n, precision, C.byref(r)),
+          "mcdx_qmc_sobol_asian_extended")
+    return r.price, r.standard_error, r.path_count
+
+
+def qmc_european(spot, strike, rate, carry, vol, time, type, n, precision):
+    r = Result()
+    check(lib().mcdx_qmc_sobol_european_p(spot, strike, rate, carry, vol, time, type, n,
+                                          precision, C.byref(r)), "mcdx_qmc_sobol_european_p")
+    return r.price
 
 
 def welford_finish(n, mean, m2):

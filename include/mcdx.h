@@ -125,6 +125,31 @@ int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* contract, int precision, voi
 int mcdx_gbm_path_samples(const mcdx_gbm_contract* contract, int precision, uint64_t path0,
                           uint64_t n, double* out);
 
+/* ---- Sobol + Brownian bridge QMC (src/core/sobol.cpp, src/pricers/qmc.cpp) ---- */
+/* kSobolMaxDimensions (include/mcd/core/sobol.hpp:14): 7, the drop-in cap. */
+uint32_t mcdx_sobol_max_dimensions(void);
+/* Dimensions of this engine's extended table (dims 0-6 equal the reference's). */
+uint32_t mcdx_sobol_extended_dimensions(void);
+/* out[k] = Sobol integer x of (dim, index0 + k); point = x / 2^32 (sobol.cpp:83-97). */
+int mcdx_sobol_ints(uint32_t dim, uint64_t index0, uint64_t n, uint32_t* out);
+/* qmc_sobol_european (src/pricers/qmc.cpp:15-30): mean over indices 1..path_count. */
+int mcdx_qmc_sobol_european(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, uint64_t path_count,
+                            double* price);
+int mcdx_qmc_sobol_european_p(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, uint64_t path_count,
+                              int precision, mcdx_result* out);
+/* qmc_sobol_asian (src/pricers/qmc.cpp:32-57); EINVAL unless 1 <= monitoring_points <= 7,
+ * as the reference binding raises (bindings/python/src/module.cpp:268-272). */
+int mcdx_qmc_sobol_asian(double spot, double strike, double rate, double carry_yield, double vol,
+                         double time, uint8_t type, uint8_t strike_style,
+                         int32_t monitoring_points, uint64_t path_count, double* price);
+/* The same pricer over up to mcdx_sobol_extended_dimensions() monitoring points. */
+int mcdx_qmc_sobol_asian_extended(double spot, double strike, double rate, double carry_yield,
+                                  double vol, double time, uint8_t type, uint8_t strike_style,
+                                  int32_t monitoring_points, uint64_t path_count, int precision,
+                                  mcdx_result* out);
+
 #ifdef __cplusplus
 }
 #endif
