@@ -55,6 +55,16 @@ def contract(product, spot=100.0, strike=100.0, barrier=0.0, rebate=0.0, rate=0.
     return c
 
 
+class HestonParams(C.Structure):
+    _fields_ = [(n, dbl) for n in ("spot", "rate", "carry_yield", "v0", "kappa", "theta", "xi",
+                                   "rho", "time")]
+
+
+def heston(spot=100.0, rate=0.03, carry_yield=0.0, v0=0.04, kappa=2.0, theta=0.04, xi=0.5,
+           rho=-0.7, time=1.0):
+    return HestonParams(spot, rate, carry_yield, v0, kappa, theta, xi, rho, time)
+
+
 # name -> (restype, argtypes); the exported-symbol test checks this against include/mcdx.h.
 SIGNATURES = {
     "mcdx_device_count": (C.c_int, [P(C.c_int)]),
@@ -83,6 +93,11 @@ SIGNATURES = {
     "mcdx_qmc_sobol_asian": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, P(dbl)]),
     "mcdx_qmc_sobol_asian_extended": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, C.c_int,
                                                             P(Result)]),
+    "mcdx_heston_qe_european": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, u64, P(Result)]),
+    "mcdx_heston_qe_strikes": (C.c_int, [P(HestonParams), P(dbl), i32, u8, i32, u64, u64,
+                                         C.c_int, P(Result)]),
+    "mcdx_heston_path_samples": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, C.c_int, u64, u64,
+                                           P(dbl)]),
 }
 
 _lib = None
@@ -158,6 +173,22 @@ def qmc_european(spot, strike, rate, carry, vol, time, type, n, precision):
     check(lib().mcdx_qmc_sobol_european_p(spot, strike, rate, carry, vol, time, type, n,
                                           precision, C.byref(r)), "mcdx_qmc_sobol_european_p")
     return r.price
+
+
+def heston_strikes(params, strikes, type, steps, n, seed, precision):
+    ks = np.ascontiguousarray(strikes, dtype=np.float64)
+    out = (Result * len(ks))()
+    check(lib().mcdx_heston_qe_strikes(C.byref(params), ks.ctypes.data_as(P(dbl)), len(ks), type,
+                                       steps, n, seed, precision, out), "mcdx_heston_qe_strikes")
+    return [(r.price, r.standard_error, r.path_count) for r in out]
+
+
+def heston_samples(params, strike, type, steps, seed, precision, path0, n):
+    out = np.empty(n, dtype=np.float64)
+    check(lib().mcdx_heston_path_samples(C.byref(params), strike, type, steps, seed, precision,
+                                         path0, n, out.ctypes.data_as(P(dbl))),
+          "mcdx_heston_path_samples")
+    return out
 
 
 def welford_finish(n, mean, m2):

@@ -150,6 +150,25 @@ int mcdx_qmc_sobol_asian_extended(double spot, double strike, double rate, doubl
                                   int32_t monitoring_points, uint64_t path_count, int precision,
                                   mcdx_result* out);
 
+/* ---- Heston QE (src/pricers/heston.cpp) ---- */
+/* HestonParams, include/mcd/models/heston.hpp:8-18 (same field order). */
+typedef struct mcdx_heston_params {
+    double spot, rate, carry_yield, v0, kappa, theta, xi, rho, time;
+} mcdx_heston_params;
+/* heston_qe_european (src/pricers/heston.cpp:120-156); precision = process default. */
+int mcdx_heston_qe_european(const mcdx_heston_params* params, double strike, uint8_t type,
+                            int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                            mcdx_result* out);
+/* Up to 32 strikes priced from one pass over the same paths; out[s] equals the
+ * single-strike result for strikes[s] bit for bit. */
+int mcdx_heston_qe_strikes(const mcdx_heston_params* params, const double* strikes,
+                           int32_t n_strikes, uint8_t type, int32_t monitoring_points,
+                           uint64_t path_count, uint64_t seed, int precision, mcdx_result* out);
+/* Discounted per-path payoffs at `strike` for paths [path0, path0+n): the parity hook. */
+int mcdx_heston_path_samples(const mcdx_heston_params* params, double strike, uint8_t type,
+                             int32_t monitoring_points, uint64_t seed, int precision,
+                             uint64_t path0, uint64_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif
