@@ -98,6 +98,14 @@ SIGNATURES = {
                                          C.c_int, P(Result)]),
     "mcdx_heston_path_samples": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, C.c_int, u64, u64,
                                            P(dbl)]),
+    "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
+    "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
+    "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
+    "mcdx_book_get_schedule": (C.c_int, [C.c_void_p, P(u32), P(u32), P(u32)]),
+    "mcdx_book_enqueue": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mcdx_book_results": (C.c_int, [C.c_void_p, P(Result)]),
+    "mcdx_book_price": (C.c_int, [C.c_void_p, P(Result)]),
+    "mcdx_book_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -189,6 +197,104 @@ def heston_samples(params, strike, type, steps, seed, precision, path0, n):
                                          path0, n, out.ctypes.data_as(P(dbl))),
           "mcdx_heston_path_samples")
     return out
+
+
+class Book:
+    """Owning handle over an mcdx_book (batched contracts, one persistent launch)."""
+
+    def __init__(self, contracts, precision=PREC_FP32):
+        arr = (GbmContract * len(contracts))(*contracts)
+        self.n = len(contracts)
+        self.h = C.c_void_p()
+        check(lib().mcdx_book_create(arr, self.n, precision, C.byref(self.h)), "mcdx_book_create")
+
+    def price(self):
+        out = (Result * self.n)()
+        check(lib().mcdx_book_price(self.h, out), "mcdx_book_price")
+        return [(r.price, r.standard_error, r.path_count) for r in out]
+
+    def enqueue(self, stream=None):
+        check(lib().mcdx_book_enqueue(self.h, C.c_void_p(stream) if stream else None),
+              "mcdx_book_enqueue")
+
+    def results(self):
+        out = (Result * self.n)()
+        check(lib().mcdx_book_results(self.h, out), "mcdx_book_results")
+        return [(r.price, r.standard_error, r.path_count) for r in out]
+
+    def autotune(self):
+        ms = C.c_float()
+        check(lib().mcdx_book_autotune(self.h, C.byref(ms)), "mcdx_book_autotune")
+        return ms.value
+
+    def set_schedule(self, blocks_per_cu, order):
+        check(lib().mcdx_book_set_schedule(self.h, blocks_per_cu, order), "mcdx_book_set_schedule")
+
+    def schedule(self):
+        b, o, g = u32(), u32(), u32()
+        check(lib().mcdx_book_get_schedule(self.h, C.byref(b), C.byref(o), C.byref(g)),
+              "mcdx_book_get_schedule")
+        return b.value, o.value, g.value
+
+    def close(self):
+        if self.h:
+            lib().mcdx_book_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def c4_book(n_contracts=4096, seed=2026, log2_min=14, log2_max=20, scale_paths=1.0):
+    """The batched-book workload: K ~ U[70,130], T in {0.25,0.5,1,2}, sigma ~ U[0.1,0.5],
+    S=100 r=0.05 q=0; type uniform over European, cash digital, arithmetic Asian and the 8
+    barriers (H = K*(1 +/- 0.2) by direction, rebate 0); N = floor(2^U(log2_min, log2_max));
+    M = 252*T for path-dependent products. Parameters come from Philox (key `seed`, stream
+    2) so the book is reproducible without any library RNG."""
+    import math
+    out = []
+    for i in range(n_contracts):
+        u = [(int(b) + 0.5) / 2 ** 32 for b in philox_bits_host(seed, i, 6)]
+        k = 70.0 + 60.0 * u[0]
+        t = (0.25, 0.5, 1.0, 2.0)[min(3, int(u[1] * 4))]
+        vol = 0.1 + 0.4 * u[2]
+        typ = min(10, int(u[3] * 11))
+        n = int(math.floor(2.0 ** (log2_min + (log2_max - log2_min) * u[4]) * scale_paths))
+        m = int(round(252 * t))
+        common = dict(spot=100.0, strike=k, rate=0.05, carry_yield=0.0, vol=vol, time=t,
+                      path_count=max(n, 1), seed=42 + i)
+        if typ == 0:
+            out.append(contract(EUROPEAN, **common))
+        elif typ == 1:
+            out.append(contract(DIGITAL, digital_style=0, cash_amount=1.0, **common))
+        elif typ == 2:
+            out.append(contract(ASIAN, monitoring_points=m, **common))
+        else:
+            b = typ - 3  # 8 barriers: type x direction x knock
+            opt, direction, knock = b & 1, (b >> 1) & 1, (b >> 2) & 1
+            h = k * (1.2 if direction == 0 else 0.8)
+            out.append(contract(BARRIER, barrier=h, direction=direction, knock=knock, type=opt,
+                                rebate=0.0, monitoring_points=m, **common))
+    return out
+
+
+def philox_bits_host(seed, index, n):
+    """Word 0 of Philox4x32-10 at counter {index, 0, j, 2}, key seed, j < n (host side)."""
+    M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+    res = []
+    for j in range(n):
+        c = [index & 0xFFFFFFFF, index >> 32, j, 2]
+        k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+        for _ in range(10):
+            p0, p1 = M0 * c[0], M1 * c[2]
+            c = [((p1 >> 32) ^ c[1] ^ k0) & 0xFFFFFFFF, p1 & 0xFFFFFFFF,
+                 ((p0 >> 32) ^ c[3] ^ k1) & 0xFFFFFFFF, p0 & 0xFFFFFFFF]
+            k0, k1 = (k0 + W0) & 0xFFFFFFFF, (k1 + W1) & 0xFFFFFFFF
+        res.append(c[0])
+    return res
 
 
 def welford_finish(n, mean, m2):

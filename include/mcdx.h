@@ -169,6 +169,27 @@ int mcdx_heston_path_samples(const mcdx_heston_params* params, double strike, ui
                              int32_t monitoring_points, uint64_t seed, int precision,
                              uint64_t path0, uint64_t n, double* out);
 
+/* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
+ * No reference counterpart (the reference prices one contract per call,
+ * src/pricers/monte_carlo.cpp:234-338); each contract's result equals mcdx_price_gbm on
+ * that contract bit for bit, whatever the schedule. */
+typedef struct mcdx_book mcdx_book;
+int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
+                     mcdx_book** out);
+/* Time every schedule (item order x blocks per CU), keep the fastest, remember it for
+ * books of the same shape; best_ms may be NULL. */
+int mcdx_book_autotune(mcdx_book* book, float* best_ms);
+/* order 0 = contract order, 1 = longest tiles first; blocks_per_cu 0 = occupancy limit. */
+int mcdx_book_set_schedule(mcdx_book* book, uint32_t blocks_per_cu, uint32_t order);
+int mcdx_book_get_schedule(const mcdx_book* book, uint32_t* blocks_per_cu, uint32_t* order,
+                           uint32_t* grid);
+/* Replay the captured graph on `stream` (NULL = the book's own stream); returns at once. */
+int mcdx_book_enqueue(mcdx_book* book, void* stream);
+/* Wait for the last enqueue and copy one result per contract into out[0..n). */
+int mcdx_book_results(mcdx_book* book, mcdx_result* out);
+int mcdx_book_price(mcdx_book* book, mcdx_result* out);
+void mcdx_book_destroy(mcdx_book* book);
+
 #ifdef __cplusplus
 }
 #endif
