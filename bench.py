@@ -86,6 +86,115 @@ def cpu_baseline(budget_s=12.0):
                       f"{dt2:.1f} s, num_threads={cores}"}
 
 
+def load_reference_fixture():
+    path = os.path.join(ROOT, "tests", "golden", "reference_baseline.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def timed_events(torch, stream, fn, reps=1):
+    """Average ms of fn() over reps, between HIP events recorded on `stream`."""
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def extras(torch, mcdx, L, stream, handle, ref):
+    """Secondary configurations: C1 European, C2 bridge-corrected barrier and Sobol QMC,
+    C3 Heston 9 strikes, C4 batched book vs per-contract launches."""
+    import oracle_lib as O
+    res = {}
+    fh = float.fromhex
+    dev = torch.zeros(3, dtype=torch.float64, device="cuda")
+
+    def enq(c):
+        mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
+                                            C.c_void_p(dev.data_ptr())), "enqueue")
+
+    # C1: European call, 1e6 paths, seed 42 (launch-latency scale) and 2^24 steady state.
+    bsm = O.lib().orc_bsm(100, 100, 0.05, 0.0, 0.2, 1.0, 0)
+    for n in (1_000_000, 1 << 24):
+        c = mcdx.contract(mcdx.EUROPEAN, path_count=n, seed=42)
+        enq(c)
+        ms = timed_events(torch, stream, lambda: enq(c), reps=10)
+        p, se, _ = mcdx.price(c, mcdx.PREC_FP32)
+        p64, se64, _ = mcdx.price(c, mcdx.PREC_FP64)
+        entry = {"paths_per_s": n / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se,
+                 "price_fp64": p64, "bsm": bsm, "dev_in_se": (p - bsm) / se}
+        if ref and n == 1_000_000:
+            rp = fh(ref["c1_european"]["price"])
+            entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
+        res[f"c1_european_{n}"] = entry
+    # C2: barrier with the Brownian-bridge correction, 2^24 x 252.
+    c = mcdx.contract(mcdx.BARRIER, barrier=120.0, monitoring_points=STEPS, path_count=N_PATHS,
+                      seed=SEED, brownian_bridge=1)
+    enq(c)
+    ms = timed_events(torch, stream, lambda: enq(c), reps=2)
+    p, se, _ = mcdx.price(c, mcdx.PREC_FP32)
+    entry = {"paths_per_s": N_PATHS / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se}
+    if ref:
+        rp = fh(ref["c2_barrier_uo_call_bb"]["price"])
+        entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
+    res["c2_barrier_uo_bb"] = entry
+    # C2 QMC: Sobol 2^20 points x 252 dims + Brownian bridge (synchronous call, wall clock).
+    mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 16, mcdx.PREC_FP32)
+    t0 = time.perf_counter()
+    q32 = mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP32)[0]
+    wall = time.perf_counter() - t0
+    q64 = mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP64)[0]
+    res["c2_qmc_sobol_bb_asian"] = {"paths_per_s_wall": (1 << 20) / wall, "ms_wall": wall * 1e3,
+                                    "price_fp32": q32, "price_fp64": q64}
+    # C3: Heston QE, 2^22 x 252, 9 strikes 80..120 from one pass (wall clock).
+    par = mcdx.heston()
+    ks = [80.0 + 5 * i for i in range(9)]
+    mcdx.heston_strikes(par, ks, 0, STEPS, 1 << 16, SEED, mcdx.PREC_FP32)
+    t0 = time.perf_counter()
+    r = mcdx.heston_strikes(par, ks, 0, STEPS, 1 << 22, SEED, mcdx.PREC_FP32)
+    wall = time.perf_counter() - t0
+    h = O.heston()
+    sa = [O.lib().orc_heston_semi_analytic(C.byref(h), k, 0) for k in ks]
+    res["c3_heston_qe_9_strikes"] = {
+        "paths_per_s_wall": (1 << 22) / wall, "path_steps_per_s_wall": (1 << 22) * STEPS / wall,
+        "ms_wall": wall * 1e3,
+        "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
+        "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)]}
+    # C4: 4096-contract book, one persistent graph launch vs 4096 per-contract launches.
+    cs = mcdx.c4_book()
+    book = mcdx.Book(cs, mcdx.PREC_FP32)
+    tuned_ms = book.autotune()
+    book.enqueue(handle.value)
+    batched_ms = timed_events(torch, stream, lambda: book.enqueue(handle.value), reps=2)
+    outs = torch.zeros((len(cs), 3), dtype=torch.float64, device="cuda")
+
+    def per_contract():
+        for i, c in enumerate(cs):
+            mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
+                                                C.c_void_p(outs[i].data_ptr())), "enqueue")
+    per_contract()
+    per_ms = timed_events(torch, stream, per_contract, reps=1)
+    bres = book.results()
+    host = outs.cpu().numpy()
+    counts = host.view("u8")
+    same = all(bres[i] == mcdx.welford_finish(int(counts[i, 0]), float(host[i, 1]),
+                                              float(host[i, 2])) for i in range(len(cs)))
+    total_paths = sum(c.path_count for c in cs)
+    res["c4_book_4096"] = {"batched_ms": batched_ms, "per_contract_ms": per_ms,
+                           "speedup": per_ms / batched_ms, "paths": total_paths,
+                           "batched_paths_per_s": total_paths / (batched_ms / 1e3),
+                           "autotuned_schedule": book.schedule(), "autotune_best_ms": tuned_ms,
+                           "bitwise_equal_to_per_contract": same}
+    book.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +203,7 @@ def main():
     ap.add_argument("--paths", type=int, default=N_PATHS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the secondary configs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -159,6 +269,17 @@ def main():
             p64, se64, _ = mcdx.price(cs[k], mcdx.PREC_FP64)
             parity[k].update(price_fp64=p64, bias_fp32_vs_fp64=parity[k]["price_fp32"] - p64,
                              bias_in_se=(parity[k]["price_fp32"] - p64) / se64)
+    ref = load_reference_fixture()
+    ref_keys = {"asian": "c2_asian_arith_call", "barrier_uo": "c2_barrier_uo_call",
+                "lookback_float": "c2_lookback_float_call"}
+    if ref and args.paths == N_PATHS:
+        for k in names:
+            rp = float.fromhex(ref[ref_keys[k]]["price"])
+            parity[k].update(reference_fp64=rp,
+                             bias_vs_reference_in_se=(parity[k]["price_fp32"] - rp) /
+                             parity[k]["se"],
+                             within_3se_of_reference=abs(parity[k]["price_fp32"] - rp) <
+                             3 * parity[k]["se"])
 
     dom = max(per_kernel_ms, key=per_kernel_ms.get)
     ops = (OPS_DRAW + OPS_GBM + OPS_OBSERVE[dom]) * args.paths * STEPS
@@ -191,6 +312,8 @@ def main():
         "roofline": roofline,
         "parity": parity,
     }
+    if not args.no_extras:
+        line["extras"] = extras(torch, mcdx, L, stream, handle, ref)
     if not args.no_cpu:
         cb = cpu_baseline()
         line["cpu_baseline"] = cb
