@@ -94,6 +94,10 @@ def load_reference_fixture():
         return json.load(f)
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def timed_events(torch, stream, fn, reps=1):
     """Average ms of fn() over reps, between HIP events recorded on `stream`."""
     a = torch.cuda.Event(enable_timing=True)
@@ -133,6 +137,7 @@ def extras(torch, mcdx, L, stream, handle, ref):
             rp = fh(ref["c1_european"]["price"])
             entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
         res[f"c1_european_{n}"] = entry
+    progress("C1 done")
     # C2: barrier with the Brownian-bridge correction, 2^24 x 252.
     c = mcdx.contract(mcdx.BARRIER, barrier=120.0, monitoring_points=STEPS, path_count=N_PATHS,
                       seed=SEED, brownian_bridge=1)
@@ -144,6 +149,7 @@ def extras(torch, mcdx, L, stream, handle, ref):
         rp = fh(ref["c2_barrier_uo_call_bb"]["price"])
         entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
     res["c2_barrier_uo_bb"] = entry
+    progress("C2 bridge-corrected barrier done")
     # C2 QMC: Sobol 2^20 points x 252 dims + Brownian bridge (synchronous call, wall clock).
     mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 16, mcdx.PREC_FP32)
     t0 = time.perf_counter()
@@ -152,6 +158,7 @@ def extras(torch, mcdx, L, stream, handle, ref):
     q64 = mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP64)[0]
     res["c2_qmc_sobol_bb_asian"] = {"paths_per_s_wall": (1 << 20) / wall, "ms_wall": wall * 1e3,
                                     "price_fp32": q32, "price_fp64": q64}
+    progress("C2 QMC done")
     # C3: Heston QE, 2^22 x 252, 9 strikes 80..120 from one pass (wall clock).
     par = mcdx.heston()
     ks = [80.0 + 5 * i for i in range(9)]
@@ -166,10 +173,12 @@ def extras(torch, mcdx, L, stream, handle, ref):
         "ms_wall": wall * 1e3,
         "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
         "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)]}
+    progress("C3 done")
     # C4: 4096-contract book, one persistent graph launch vs 4096 per-contract launches.
     cs = mcdx.c4_book()
     book = mcdx.Book(cs, mcdx.PREC_FP32)
     tuned_ms = book.autotune()
+    progress("C4 autotuned")
     book.enqueue(handle.value)
     batched_ms = timed_events(torch, stream, lambda: book.enqueue(handle.value), reps=2)
     outs = torch.zeros((len(cs), 3), dtype=torch.float64, device="cuda")
@@ -281,6 +290,7 @@ def main():
                              within_3se_of_reference=abs(parity[k]["price_fp32"] - rp) <
                              3 * parity[k]["se"])
 
+    progress("headline timed; parity legs done")
     dom = max(per_kernel_ms, key=per_kernel_ms.get)
     ops = (OPS_DRAW + OPS_GBM + OPS_OBSERVE[dom]) * args.paths * STEPS
     t_dom = per_kernel_ms[dom] / 1e3
@@ -315,6 +325,7 @@ def main():
     if not args.no_extras:
         line["extras"] = extras(torch, mcdx, L, stream, handle, ref)
     if not args.no_cpu:
+        progress("CPU baseline (reference, bounded sample)")
         cb = cpu_baseline()
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = value / cb["value"]

@@ -2,9 +2,8 @@
 // types priced by one persistent launch, replayed from a captured HIP graph.
 //
 // Work is split into (contract, tile) items, a tile being the same 1024 consecutive path
-// indices every per-contract launch uses (device_common.hpp). Persistent blocks claim
-// items from an atomic counter and write each tile's Welford partial to its contract's
-// slot; the per-contract fixed-order merge then runs exactly as for a single contract. A
+// indices every per-contract launch uses (device_common.hpp). Persistent blocks walk the
+// item list round robin and write each tile's Welford partial to its contract's slot; the per-contract fixed-order merge then runs exactly as for a single contract. A
 // contract's result therefore does not depend on the schedule and equals a per-contract
 // launch bit for bit; only the item order and grid size change, and those are what the
 // autotuner picks.
@@ -34,7 +33,6 @@ struct mcdx_book {
     uint64_t* d_items64 = nullptr;              // (contract << 40) | tile
     mcdx::Welford* d_partials = nullptr;
     mcdx::Welford* d_results = nullptr;
-    uint32_t* d_counter = nullptr;
     // schedule
     uint32_t blocks_per_cu = 0;                 // 0 = occupancy maximum
     uint32_t order = 1;                         // 0 natural, 1 longest tiles first
@@ -49,10 +47,6 @@ struct mcdx_book {
 
 namespace mcdx {
 
-__global__ void book_reset_kernel(uint32_t* counter) {
-    if (threadIdx.x == 0) counter[0] = 0;
-}
-
 template <bool FP64>
 __device__ __forceinline__ double book_sample(const GbmArgs& a, int kind, bool anti, uint64_t path) {
     switch (kind) {
@@ -64,22 +58,19 @@ __device__ __forceinline__ double book_sample(const GbmArgs& a, int kind, bool a
     }
 }
 
-// Persistent blocks; every block exits once the counter passes n_items.
+// Persistent blocks over the item list, round robin: block b takes items b, b+G, b+2G, ...
+// in the schedule's order (longest tiles first spreads the expensive tiles over all blocks).
+// The loop trip count is uniform per block and the body has no thread-0-only region, so
+// every lane runs every iteration in step with the block's barriers.
 template <bool FP64>
 __global__ __launch_bounds__(kTileLanes) void book_kernel(const GbmArgs* __restrict__ args,
                                                           const uint8_t* __restrict__ kinds,
                                                           const uint64_t* __restrict__ toff,
                                                           const uint64_t* __restrict__ items,
-                                                          uint64_t n_items, uint32_t* counter,
+                                                          uint64_t n_items,
                                                           Welford* __restrict__ partials) {
-    __shared__ uint32_t s_item;
     __shared__ Welford lds4[4];
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads(); // every lane has read s_item before thread 0 claims again
-        if (it >= n_items) break;
+    for (uint64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const uint64_t packed = items[it];
         const uint32_t c = (uint32_t)(packed >> 40);
         const uint64_t tile = packed & ((1ull << 40) - 1);
@@ -99,8 +90,8 @@ __global__ __launch_bounds__(kTileLanes) void book_kernel(const GbmArgs* __restr
                 w.m2 += d * (x - w.mean);
             }
         }
-        const Welford t = block_merge_256(w, lds4);
-        if (threadIdx.x == 0) partials[toff[c] + tile] = t;
+        const Welford t = block_merge_256_all(w, lds4);
+        partials[toff[c] + tile] = t; // every lane stores the same value
     }
 }
 
@@ -151,14 +142,13 @@ int grid_blocks(const mcdx_book& b) {
 }
 
 hipError_t enqueue_raw(mcdx_book& b, hipStream_t s) {
-    hipLaunchKernelGGL(book_reset_kernel, dim3(1), dim3(64), 0, s, b.d_counter);
     const int grid = grid_blocks(b);
     if (b.precision == MCDX_PREC_FP64)
         hipLaunchKernelGGL(book_kernel<true>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args, b.d_kind,
-                           b.d_toff, b.d_items64, b.total_tiles, b.d_counter, b.d_partials);
+                           b.d_toff, b.d_items64, b.total_tiles, b.d_partials);
     else
         hipLaunchKernelGGL(book_kernel<false>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args, b.d_kind,
-                           b.d_toff, b.d_items64, b.total_tiles, b.d_counter, b.d_partials);
+                           b.d_toff, b.d_items64, b.total_tiles, b.d_partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_final_merge(b.d_partials, b.d_toff, 0, b.n, b.d_results, s);
@@ -187,8 +177,7 @@ void free_book(mcdx_book* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->d_args, b->d_kind, b->d_toff, b->d_items64, b->d_partials, b->d_results,
-                    b->d_counter};
+    void* ptrs[] = {b->d_args, b->d_kind, b->d_toff, b->d_items64, b->d_partials, b->d_results};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -253,7 +242,6 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
     if (e == hipSuccess) e = hipMalloc(&b->d_items64, acc * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_partials, acc * sizeof(Welford));
     if (e == hipSuccess) e = hipMalloc(&b->d_results, n * sizeof(Welford));
-    if (e == hipSuccess) e = hipMalloc(&b->d_counter, 256);
     if (e == hipSuccess) e = hipMemcpy(b->d_args, args.data(), n * sizeof(GbmArgs), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_kind, kinds.data(), n, hipMemcpyHostToDevice);
     if (e == hipSuccess)

@@ -292,6 +292,45 @@ __device__ __forceinline__ Welford block_merge_256(Welford w, Welford* lds4) {
                                             welford_merge(lds4[2], lds4[3]));
     return r;
 }
+
+// The same tree, with the tile total returned to every lane and no lane-divergent branch:
+// each wave broadcasts its lane-0 total with readfirstlane, every lane of the wave writes
+// that same value to the wave's LDS slot, and every thread folds the four slots. Used in
+// loops that run the merge more than once, where a thread-0-only region next to the
+// barriers lets the compiler split the loop so lanes would re-run iterations out of step.
+__device__ __forceinline__ uint64_t bcast_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double bcast_f64(double v) {
+    return __builtin_bit_cast(double, bcast_u64(__builtin_bit_cast(uint64_t, v)));
+}
+
+__device__ __forceinline__ Welford block_merge_256_all(Welford w, Welford* lds4) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Welford o;
+        o.n = __shfl_down(w.n, off, 64);
+        o.mean = __shfl_down(w.mean, off, 64);
+        o.m2 = __shfl_down(w.m2, off, 64);
+        const Welford m = welford_merge(w, o);
+        const bool take = (lane & (2 * off - 1)) == 0;
+        w.n = take ? m.n : w.n;
+        w.mean = take ? m.mean : w.mean;
+        w.m2 = take ? m.m2 : w.m2;
+    }
+    Welford wave_total;
+    wave_total.n = bcast_u64(w.n);
+    wave_total.mean = bcast_f64(w.mean);
+    wave_total.m2 = bcast_f64(w.m2);
+    lds4[threadIdx.x >> 6] = wave_total;
+    __syncthreads();
+    const Welford r = welford_merge(welford_merge(lds4[0], lds4[1]), welford_merge(lds4[2], lds4[3]));
+    __syncthreads(); // the slots may be rewritten by the caller's next merge
+    return r;
+}
 #endif
 
 } // namespace mcdx

@@ -147,9 +147,8 @@ __global__ __launch_bounds__(kTileLanes) void heston_tile_kernel(HestonArgs a) {
             w.mean += d * (1.0 / (double)(k + 1));
             w.m2 += d * (x - w.mean);
         }
-        const Welford t = block_merge_256(w, lds4);
-        if (threadIdx.x == 0) a.partials[(uint64_t)s * a.tiles + tile] = t;
-        __syncthreads(); // lds4 is reused by the next strike
+        const Welford t = block_merge_256_all(w, lds4);
+        a.partials[(uint64_t)s * a.tiles + tile] = t; // every lane stores the same value
     }
 }
 
