@@ -299,6 +299,7 @@ def philox_bits_host(seed, index, n):
 
 def welford_finish(n, mean, m2):
     """price, SE, count from a device Welford triple (stats.hpp:35-41)."""
+    import math
     var = 0.0 if n < 2 else m2 / (n - 1)
-    se = 0.0 if n == 0 else (var / n) ** 0.5
+    se = 0.0 if n == 0 else math.sqrt(var / n)
     return mean, se, n
