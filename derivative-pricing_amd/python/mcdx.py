@@ -98,6 +98,11 @@ SIGNATURES = {
                                          C.c_int, P(Result)]),
     "mcdx_heston_path_samples": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, C.c_int, u64, u64,
                                            P(dbl)]),
+    "mcdx_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, P(Result), P(dbl),
+                                                P(u8), P(u8)]),
+    "mcdx_monte_carlo_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(Result)]),
+    "mcdx_lsm_reprice_frozen": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(dbl), P(u8), u8,
+                                                      C.c_int, P(Result)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
@@ -197,6 +202,29 @@ def heston_samples(params, strike, type, steps, seed, precision, path0, n):
                                          path0, n, out.ctypes.data_as(P(dbl))),
           "mcdx_heston_path_samples")
     return out
+
+
+def lsm(spot, strike, rate, carry, vol, time, type, m, n, seed, precision):
+    """(price, se, coefficients[(m+1), 4], has[m+1], exercise_at_inception)"""
+    r = Result()
+    co = np.zeros((m + 1) * 4)
+    has = np.zeros(m + 1, dtype=np.uint8)
+    inc = u8(0)
+    check(lib().mcdx_lsm_american(spot, strike, rate, carry, vol, time, type, m, n, seed, precision,
+                                  C.byref(r), co.ctypes.data_as(P(dbl)), has.ctypes.data_as(P(u8)),
+                                  C.byref(inc)), "mcdx_lsm_american")
+    return r.price, r.standard_error, co.reshape(m + 1, 4), has, inc.value
+
+
+def lsm_reprice(spot, strike, rate, carry, vol, time, type, m, n, seed, coeffs, has, inception,
+                precision):
+    r = Result()
+    co = np.ascontiguousarray(coeffs, dtype=np.float64).ravel()
+    hs = np.ascontiguousarray(has, dtype=np.uint8)
+    check(lib().mcdx_lsm_reprice_frozen(spot, strike, rate, carry, vol, time, type, m, n, seed,
+                                        co.ctypes.data_as(P(dbl)), hs.ctypes.data_as(P(u8)),
+                                        inception, precision, C.byref(r)), "mcdx_lsm_reprice_frozen")
+    return r.price
 
 
 class Book:

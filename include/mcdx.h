@@ -169,6 +169,26 @@ int mcdx_heston_path_samples(const mcdx_heston_params* params, double strike, ui
                              int32_t monitoring_points, uint64_t seed, int precision,
                              uint64_t path0, uint64_t n, double* out);
 
+/* ---- Longstaff-Schwartz (src/pricers/lsm.cpp) ---- */
+/* monte_carlo_lsm_american (src/pricers/lsm.cpp:62-149) at `precision`. The policy
+ * (LsmPolicy, include/mcd/pricers/lsm.hpp:16-24) is returned as coefficients[(M+1)*4]
+ * (row d = date d), has_coefficients[M+1] and exercise_at_inception; any may be NULL. */
+int mcdx_lsm_american(double spot, double strike, double rate, double carry_yield, double vol,
+                      double time, uint8_t type, int32_t monitoring_points, uint64_t path_count,
+                      uint64_t seed, int precision, mcdx_result* out, double* coefficients,
+                      uint8_t* has_coefficients, uint8_t* exercise_at_inception);
+/* Reference signature (src/pricers/lsm.cpp:62-66); precision = process default. */
+int mcdx_monte_carlo_lsm_american(double spot, double strike, double rate, double carry_yield,
+                                  double vol, double time, uint8_t type,
+                                  int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                                  mcdx_result* out);
+/* reprice_against_frozen_policy (src/pricers/lsm.cpp:151-213); out->price is the price. */
+int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, int32_t monitoring_points,
+                            uint64_t path_count, uint64_t seed, const double* coefficients,
+                            const uint8_t* has_coefficients, uint8_t exercise_at_inception,
+                            int precision, mcdx_result* out);
+
 /* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
  * No reference counterpart (the reference prices one contract per call,
  * src/pricers/monte_carlo.cpp:234-338); each contract's result equals mcdx_price_gbm on
