@@ -64,6 +64,18 @@ def build_hip(jobs=8):
     os.replace(tmp, LIB)
 
 
+def build_host_lib():
+    """libmcd.so: the drop-in C++20 API (include/mcd/**) over libmcd_hip.so."""
+    srcs = [s for s in sorted(glob.glob(os.path.join(HOST, "*.cpp")))
+            if not s.endswith("module.cpp")]
+    out = os.path.join(LIBDIR, "libmcd.so")
+    run(["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
+         "-I", os.path.join(ROOT, "include")] + srcs +
+        ["-L", LIBDIR, "-lmcd_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-soname,libmcd.so",
+         "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_pybind():
     hosts = sorted(glob.glob(os.path.join(HOST, "*.cpp")))
     if not hosts:
@@ -88,6 +100,7 @@ def build_oracle():
 
 def main():
     build_hip()
+    build_host_lib()
     build_pybind()
     build_oracle()
 

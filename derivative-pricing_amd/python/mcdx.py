@@ -103,6 +103,13 @@ SIGNATURES = {
     "mcdx_monte_carlo_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(Result)]),
     "mcdx_lsm_reprice_frozen": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(dbl), P(u8), u8,
                                                       C.c_int, P(Result)]),
+    "mcdx_pathwise_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Result)]),
+    "mcdx_pathwise_asian": (C.c_int, [dbl] * 6 + [u8, u8, u8, i32, u64, u64, P(Result)]),
+    "mcdx_likelihood_ratio_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Result)]),
+    "mcdx_likelihood_ratio_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Result)]),
+    "mcdx_likelihood_ratio_barrier": (C.c_int, [dbl] * 7 + [u8, u8, u8, dbl, i32, u64, u64,
+                                                            P(Result)]),
+    "mcdx_fd_european_prices": (C.c_int, [dbl] * 6 + [u8, u64, u64] + [dbl] * 4 + [P(Result)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
@@ -225,6 +232,12 @@ def lsm_reprice(spot, strike, rate, carry, vol, time, type, m, n, seed, coeffs, 
                                         co.ctypes.data_as(P(dbl)), hs.ctypes.data_as(P(u8)),
                                         inception, precision, C.byref(r)), "mcdx_lsm_reprice_frozen")
     return r.price
+
+
+def greeks(name, *args, n_out):
+    out = (Result * n_out)()
+    check(getattr(lib(), name)(*args, out), name)
+    return [(r.price, r.standard_error, r.path_count) for r in out]
 
 
 class Book:

@@ -189,6 +189,41 @@ int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carr
                             const uint8_t* has_coefficients, uint8_t exercise_at_inception,
                             int precision, mcdx_result* out);
 
+/* ---- Greeks (src/greeks), FP64 arithmetic as the reference ---- */
+/* pathwise_european (src/greeks/pathwise.cpp:28-62): out3 = delta, vega, rho, each as
+ * {value, standard_error, path_count}. */
+int mcdx_pathwise_european(double spot, double strike, double rate, double carry_yield,
+                           double vol, double time, uint8_t type, uint64_t path_count,
+                           uint64_t seed, mcdx_result* out3);
+/* pathwise_asian (src/greeks/pathwise.cpp:64-147): out3 = delta, vega, rho. */
+int mcdx_pathwise_asian(double spot, double strike, double rate, double carry_yield, double vol,
+                        double time, uint8_t type, uint8_t strike_style, uint8_t average_style,
+                        int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                        mcdx_result* out3);
+/* likelihood_ratio_european / _digital (src/greeks/likelihood_ratio.cpp:77-99):
+ * out5 = delta, gamma, vega, rho, theta. */
+int mcdx_likelihood_ratio_european(double spot, double strike, double rate, double carry_yield,
+                                   double vol, double time, uint8_t type, uint64_t path_count,
+                                   uint64_t seed, mcdx_result* out5);
+int mcdx_likelihood_ratio_digital(double spot, double strike, double rate, double carry_yield,
+                                  double vol, double time, uint8_t type, uint8_t style,
+                                  double cash_amount, uint64_t path_count, uint64_t seed,
+                                  mcdx_result* out5);
+/* likelihood_ratio_barrier (src/greeks/likelihood_ratio.cpp:105-154): out4 = delta, gamma,
+ * vega, rho (no theta, as the reference). */
+int mcdx_likelihood_ratio_barrier(double spot, double strike, double barrier, double rate,
+                                  double carry_yield, double vol, double time, uint8_t type,
+                                  uint8_t direction, uint8_t knock, double rebate,
+                                  int32_t monitoring_points, uint64_t path_count, uint64_t seed,
+                                  mcdx_result* out4);
+/* The nine common-random-number European prices finite_difference_european needs
+ * (src/greeks/finite_difference.cpp:40-65), from one pass: base, spot+, spot-, vol+, vol-,
+ * rate+, rate-, time+, time-. */
+int mcdx_fd_european_prices(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, uint64_t path_count,
+                            uint64_t seed, double bump_spot, double bump_vol, double bump_rate,
+                            double bump_time, mcdx_result* out9);
+
 /* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
  * No reference counterpart (the reference prices one contract per call,
  * src/pricers/monte_carlo.cpp:234-338); each contract's result equals mcdx_price_gbm on

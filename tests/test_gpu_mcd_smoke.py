@@ -1,0 +1,147 @@
+"""The reference's Python smoke suite (bindings/python/tests/test_smoke.py:15-268), assertion
+for assertion, against this repository's `mcd` module on the GPU."""
+import threading
+import time
+
+import pytest
+
+import mcd
+
+pytestmark = pytest.mark.gpu
+C, P = mcd.OptionType.Call, mcd.OptionType.Put
+
+
+def test_every_product_prices_with_a_confidence_interval():
+    seed, n = 42, 100_000
+    res = [
+        mcd.monte_carlo_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, n, seed),
+        mcd.monte_carlo_digital(100, 100, 0.05, 0.0, 0.2, 1.0, C, mcd.DigitalStyle.CashOrNothing,
+                                1.0, n, seed),
+        mcd.monte_carlo_asian(100, 100, 0.05, 0.0, 0.2, 1.0, C, mcd.StrikeStyle.Fixed,
+                              mcd.AverageStyle.Geometric, 12, n, seed),
+        mcd.monte_carlo_barrier(100, 100, 120, 0.05, 0.0, 0.2, 1.0, C, mcd.BarrierDirection.Up,
+                                mcd.BarrierKnock.Out, 0.0, 20, n, seed),
+        mcd.monte_carlo_lookback(100, 100, 0.05, 0.0, 0.2, 1.0, C, mcd.StrikeStyle.Floating, 20,
+                                 n, seed),
+    ]
+    for r in res:
+        assert r.standard_error > 0.0 and r.path_count == n
+
+
+def test_european_within_three_se_of_analytic():
+    mc = mcd.monte_carlo_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 500_000, 42)
+    analytic = mcd.black_scholes_merton(100, 100, 0.05, 0.0, 0.2, 1.0, C)
+    assert abs(mc.price - analytic) < 3.0 * mc.standard_error
+
+
+def test_american_lsm_prices():
+    r = mcd.monte_carlo_lsm_american(100, 100, 0.05, 0.0, 0.25, 1.0, P, 20, 50_000, 7)
+    assert r.price > 0.0 and r.standard_error >= 0.0
+
+
+def test_finite_difference_european_greeks():
+    bumps = mcd.default_bump_sizes(100, 0.2, 1.0)
+    g = mcd.finite_difference_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 300_000, 42, bumps)
+    assert 0.0 < g.delta < 1.0 and g.vega > 0.0
+
+
+def test_likelihood_ratio_greeks():
+    lr = mcd.likelihood_ratio_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 300_000, 42)
+    assert 0.0 < lr.delta.value < 1.0 and lr.delta.standard_error > 0.0
+    assert lr.theta is not None
+    d = mcd.likelihood_ratio_digital(100, 100, 0.05, 0.0, 0.2, 1.0, C,
+                                     mcd.DigitalStyle.CashOrNothing, 1.0, 200_000, 7)
+    assert d.gamma.standard_error > 0.0
+    b = mcd.likelihood_ratio_barrier(100, 100, 120, 0.05, 0.0, 0.2, 1.0, C,
+                                     mcd.BarrierDirection.Up, mcd.BarrierKnock.Out, 0.0, 50,
+                                     100_000, 11)
+    assert b.gamma.standard_error > 0.0 and b.theta is None
+
+
+def test_lr_gamma_se_below_fd_gamma_se_at_the_money_digital():
+    s, k, r, q, sigma, t = 100.0, 100.0, 0.05, 0.0, 0.25, 1.0
+    n, seed = 200_000, 7
+    lr = mcd.likelihood_ratio_digital(s, k, r, q, sigma, t, C, mcd.DigitalStyle.CashOrNothing,
+                                      1.0, n, seed)
+    b = mcd.default_bump_sizes(s, sigma, t)
+    up = mcd.monte_carlo_digital(s + b.spot, k, r, q, sigma, t, C, mcd.DigitalStyle.CashOrNothing, 1.0, n, seed)
+    mid = mcd.monte_carlo_digital(s, k, r, q, sigma, t, C, mcd.DigitalStyle.CashOrNothing, 1.0, n, seed)
+    dn = mcd.monte_carlo_digital(s - b.spot, k, r, q, sigma, t, C, mcd.DigitalStyle.CashOrNothing, 1.0, n, seed)
+    fd_se = ((up.standard_error ** 2 + 4 * mid.standard_error ** 2 + dn.standard_error ** 2)
+             ** 0.5) / (b.spot ** 2)
+    assert lr.gamma.standard_error < fd_se
+
+
+def test_pathwise_greeks():
+    pw = mcd.pathwise_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 300_000, 42)
+    assert 0.0 < pw.delta.value < 1.0 and pw.delta.standard_error > 0.0
+    assert not hasattr(pw, "gamma") and not hasattr(pw, "theta")
+    pa = mcd.pathwise_asian(100, 100, 0.05, 0.0, 0.2, 1.0, C, mcd.StrikeStyle.Fixed,
+                            mcd.AverageStyle.Arithmetic, 12, 100_000, 42)
+    assert pa.delta.standard_error > 0.0
+
+
+def test_qmc():
+    analytic = mcd.black_scholes_merton(100, 100, 0.05, 0.02, 0.2, 1.0, C)
+    q = mcd.qmc_sobol_european(100, 100, 0.05, 0.02, 0.2, 1.0, C, 200_000)
+    assert abs(q.price - analytic) < 0.02 and not hasattr(q, "standard_error")
+    mc = mcd.monte_carlo_asian(100, 100, 0.05, 0.02, 0.25, 1.0, C, mcd.StrikeStyle.Fixed,
+                               mcd.AverageStyle.Arithmetic, 7, 100_000, 42)
+    qa = mcd.qmc_sobol_asian(100, 100, 0.05, 0.02, 0.25, 1.0, C, mcd.StrikeStyle.Fixed, 7, 100_000)
+    assert abs(qa.price - mc.price) < 3.0 * mc.standard_error + 0.1
+    with pytest.raises(ValueError):
+        mcd.qmc_sobol_asian(100, 100, 0.05, 0.02, 0.25, 1.0, C, mcd.StrikeStyle.Fixed,
+                            mcd.SOBOL_MAX_DIMENSIONS + 1, 10_000)
+
+
+def _lewis():
+    p = mcd.HestonParams()
+    p.spot, p.rate, p.carry_yield = 100.0, 0.01, 0.02
+    p.v0, p.kappa, p.theta, p.xi, p.rho = 0.04, 4.0, 0.25, 1.0, -0.5
+    p.time = 1.0
+    return p
+
+
+def test_heston():
+    p = _lewis()
+    assert abs(mcd.heston_semi_analytic(p, 100.0, C) - 16.070154917028834278) < 1e-6
+    mc = mcd.heston_qe_european(p, 100.0, C, 50, 100_000, 42)
+    assert abs(mc.price - mcd.heston_semi_analytic(p, 100.0, C)) < 3.0 * mc.standard_error
+    a = mcd.heston_qe_european(p, 100.0, C, 20, 10_000, 7)
+    b = mcd.heston_qe_european(p, 100.0, C, 20, 10_000, 7)
+    assert a.price == b.price
+
+
+def test_benchmark_harness():
+    r = mcd.benchmark_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 200_000, 42, mcd.McOptions())
+    for key in ("price", "elapsed_seconds", "paths_per_second"):
+        assert key in r
+    assert r["elapsed_seconds"] > 0.0 and r["paths_per_second"] > 0.0
+    r = mcd.benchmark_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, 10_000, 42, mcd.McOptions())
+    assert r["simd_enabled"] == mcd.HAS_NEON
+    a = mcd.monte_carlo_european(100, 100, 0.05, 0.02, 0.2, 1.0, C, 50_000, 7)
+    b = mcd.monte_carlo_european(100, 100, 0.05, 0.02, 0.2, 1.0, C, 50_000, 7)
+    assert a.price == b.price
+
+
+def test_gil_released_during_pricing():
+    # Each thread gets its own HIP stream; with the GIL released, two threads' calls overlap
+    # their launch/sync latency instead of running back to back.
+    calls, n = 40, 200_000
+
+    def work(seed):
+        for i in range(calls):
+            mcd.monte_carlo_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, n, seed + i)
+
+    work(0)  # warm both the module and the device
+    t0 = time.perf_counter()
+    work(1)
+    baseline = time.perf_counter() - t0
+    threads = [threading.Thread(target=work, args=(100 * (i + 1),)) for i in range(2)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    concurrent = time.perf_counter() - t0
+    assert concurrent < baseline * 1.6, (concurrent, baseline)
