@@ -125,22 +125,31 @@ def test_benchmark_harness():
 
 
 def test_gil_released_during_pricing():
-    # Each thread gets its own HIP stream; with the GIL released, two threads' calls overlap
-    # their launch/sync latency instead of running back to back.
+    # Each calling thread owns a HIP stream and scratch (created on its first call); with the
+    # GIL released, two warmed threads overlap their launch/sync latency instead of running
+    # back to back (reference test_smoke.py:230-264 uses the same 1.6x bar).
     calls, n = 40, 200_000
 
     def work(seed):
         for i in range(calls):
             mcd.monte_carlo_european(100, 100, 0.05, 0.0, 0.2, 1.0, C, n, seed + i)
 
-    work(0)  # warm both the module and the device
+    work(0)
     t0 = time.perf_counter()
     work(1)
     baseline = time.perf_counter() - t0
-    threads = [threading.Thread(target=work, args=(100 * (i + 1),)) for i in range(2)]
-    t0 = time.perf_counter()
+    start = threading.Barrier(3)
+
+    def worker(seed):
+        work(seed)          # per-thread engine state is created here, outside the timing
+        start.wait()
+        work(seed + 1000)
+
+    threads = [threading.Thread(target=worker, args=(100 * (i + 1),)) for i in range(2)]
     for t in threads:
         t.start()
+    start.wait()
+    t0 = time.perf_counter()
     for t in threads:
         t.join()
     concurrent = time.perf_counter() - t0
