@@ -110,6 +110,9 @@ SIGNATURES = {
     "mcdx_likelihood_ratio_barrier": (C.c_int, [dbl] * 7 + [u8, u8, u8, dbl, i32, u64, u64,
                                                             P(Result)]),
     "mcdx_fd_european_prices": (C.c_int, [dbl] * 6 + [u8, u64, u64] + [dbl] * 4 + [P(Result)]),
+    "mcdx_mlp_param_count": (u32, []),
+    "mcdx_mlp_gradients": (C.c_int, [P(C.c_float)] * 4 + [u32, P(C.c_float), P(C.c_float)]),
+    "mcdx_mlp_forward": (C.c_int, [P(C.c_float)] * 3 + [u64, P(C.c_float)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
@@ -238,6 +241,28 @@ def greeks(name, *args, n_out):
     out = (Result * n_out)()
     check(getattr(lib(), name)(*args, out), name)
     return [(r.price, r.standard_error, r.path_count) for r in out]
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(P(C.c_float))
+
+
+def mlp_gradients(w, x, t, y):
+    (w_, pw), (x_, px), (t_, pt), (y_, py) = _f32(w), _f32(x), _f32(t), _f32(y)
+    g = np.empty(lib().mcdx_mlp_param_count(), dtype=np.float32)
+    loss = np.empty(2, dtype=np.float32)
+    check(lib().mcdx_mlp_gradients(pw, px, pt, py, len(x_), g.ctypes.data_as(P(C.c_float)),
+                                   loss.ctypes.data_as(P(C.c_float))), "mcdx_mlp_gradients")
+    return g, loss
+
+
+def mlp_forward(w, x, t):
+    (w_, pw), (x_, px), (t_, pt) = _f32(w), _f32(x), _f32(t)
+    out = np.empty(len(x_), dtype=np.float32)
+    check(lib().mcdx_mlp_forward(pw, px, pt, len(x_), out.ctypes.data_as(P(C.c_float))),
+          "mcdx_mlp_forward")
+    return out
 
 
 class Book:

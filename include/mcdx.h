@@ -224,6 +224,18 @@ int mcdx_fd_european_prices(double spot, double strike, double rate, double carr
                             uint64_t seed, double bump_spot, double bump_vol, double bump_rate,
                             double bump_time, mcdx_result* out9);
 
+/* ---- Learned continuation value (no reference counterpart) ----
+ * A 2 -> 64 -> 64 -> 1 SiLU network over (S/K, t/T) with a flat FP32 parameter vector:
+ * W1[64][2], b1[64], W2[64][64] (out, in), b2[64], W3[64], b3. */
+uint32_t mcdx_mlp_param_count(void);
+/* Gradient of the batch-mean squared error through the device's BF16-MFMA backward pass;
+ * n <= 128 samples. loss2[0..1] = summed squared errors of the two 64-sample halves. */
+int mcdx_mlp_gradients(const float* weights, const float* xs, const float* ts, const float* ys,
+                       uint32_t n, float* grad, float* loss2);
+/* out[i] = network(xs[i], ts[i]) through the device's BF16-MFMA forward pass. */
+int mcdx_mlp_forward(const float* weights, const float* xs, const float* ts, uint64_t n,
+                     float* out);
+
 /* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
  * No reference counterpart (the reference prices one contract per call,
  * src/pricers/monte_carlo.cpp:234-338); each contract's result equals mcdx_price_gbm on
