@@ -22,6 +22,7 @@
 
 #include <vector>
 
+#include "mlp.hpp"
 #include "runtime.hpp"
 
 namespace mcdx {
@@ -102,7 +103,15 @@ struct LsmArgs {
     uint8_t* has;          // [M+1] device
     QrState* qr_partials;  // [blocks]
     Welford* partials;     // [tiles]
+    // Optional capture of the regression samples for the network, [date 1..M-1][path]:
+    // features (4 (S/K - 1), 2 date/M - 1) and target 10 cf/K (NaN when out of the money).
+    float* cap_f1;
+    float* cap_f2;
+    float* cap_y;
 };
+
+constexpr double kFeatScale = 4.0;  // f1 = kFeatScale * (S/K - 1)
+constexpr double kTargetScale = 10.0; // y = kTargetScale * cf / K
 
 template <bool FP64>
 __device__ __forceinline__ double store_get(const LsmArgs& a, int date, uint64_t p) {
@@ -195,11 +204,18 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
         a.cf[p] = cf;
         if (regress) {
             const double s0 = store_get<FP64>(a, date, p);
-            if (intrinsic_v(s0, a.strike, a.type) > 0.0) { // lsm.cpp:88-100
+            const bool itm = intrinsic_v(s0, a.strike, a.type) > 0.0;
+            if (itm) { // lsm.cpp:88-100
                 double b[4];
                 laguerre4(s0 / a.strike, b);
                 qr_add_row(st, b[0], b[1], b[2], b[3], cf);
                 st.rows = 1;
+            }
+            if (a.cap_y) {
+                const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
+                a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
+                a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
+                a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
             }
         }
     }
@@ -285,6 +301,156 @@ __global__ void lsm_clear_policy_kernel(uint8_t* has, double* coeffs, int m) {
     }
 }
 
+// ---- Streaming exercise by a policy (no path store) --------------------------------------
+// The network's continuation value enters as a table over S/K per exercise date: it is the
+// network evaluated (by the MFMA forward kernel) on a uniform grid, read with linear
+// interpolation. Outside the table's range the QR coefficients decide (the fallback).
+struct PolicyArgs {
+    const float* table;           // [date 1..M-1][grid], kTargetScale * continuation / K
+    uint32_t grid;
+    double xlo, xhi, inv_h;       // S/K range of the table and 1/spacing
+    Welford* part_nn;             // [tiles + 1]
+    Welford* part_qr;             // [tiles + 1]
+    unsigned long long* counters; // [3] ITM decisions, disagreements, fallbacks
+};
+
+// Grid features for the table: (4 (x_g - 1), 2 d/M - 1) for d = 1..M-1, g < grid.
+__global__ __launch_bounds__(256) void lsm_grid_kernel(float* f1, float* f2, int m, uint32_t grid,
+                                                       double xlo, double h) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)(m - 1) * grid) return;
+    const int date = (int)(i / grid) + 1;
+    const uint32_t g = (uint32_t)(i % grid);
+    f1[i] = (float)(kFeatScale * (xlo + h * (double)g - 1.0));
+    f2[i] = (float)(2.0 * (double)date / (double)m - 1.0);
+}
+
+__global__ void lsm_zero_counters_kernel(unsigned long long* c) {
+    if (threadIdx.x < 3) c[threadIdx.x] = 0ull;
+}
+
+// One exercise-policy state: stopped flag, the exercise value and the date it was taken.
+struct Stop {
+    bool done;
+    int date;
+    double value;
+};
+
+// Per tile of 1024 paths (canonical layout), regenerate each path exactly as the store
+// kernel does and take the first date at which each policy exercises; the cash flow is then
+// discounted by one multiplication per date, the order in which backward induction applies
+// them, so the QR policy reproduces the backward-induction price bit for bit.
+template <bool FP64>
+__global__ __launch_bounds__(kTileLanes) void lsm_policy_kernel(LsmArgs a, PolicyArgs q) {
+    __shared__ Welford lds_n[4], lds_q[4];
+    const uint64_t tile = blockIdx.x;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    Welford wn = {0, 0.0, 0.0}, wq = {0, 0.0, 0.0};
+    unsigned long long c_dec = 0, c_dis = 0, c_fb = 0;
+#pragma unroll 1
+    for (int k = 0; k < kPathsPerLane; ++k) {
+        const uint64_t p = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
+        if (p >= a.n_paths) break;
+        const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
+        Stop sn = {false, 0, 0.0}, sq = {false, 0, 0.0};
+        auto visit = [&](int date, double s) {
+            const double ex = intrinsic_v(s, a.strike, a.type);
+            if (date == a.m) { // lsm.cpp:76-81
+                if (!sn.done) sn = Stop{true, date, ex};
+                if (!sq.done) sq = Stop{true, date, ex};
+                return;
+            }
+            if (!(ex > 0.0)) return;
+            bool dq = false;
+            if (a.has[date]) { // lsm.cpp:110-122
+                double b[4];
+                laguerre4(s / a.strike, b);
+                const double* beta = a.coeffs + 4 * date;
+                double cont = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
+                dq = ex > cont;
+            }
+            bool dn;
+            const double x = s / a.strike;
+            if (x >= q.xlo && x <= q.xhi) {
+                const double u = (x - q.xlo) * q.inv_h;
+                uint32_t g = (uint32_t)u;
+                if (g > q.grid - 2) g = q.grid - 2;
+                const float* row = q.table + (uint64_t)(date - 1) * q.grid;
+                const double fr = u - (double)g;
+                const double c = (double)row[g] + fr * ((double)row[g + 1] - (double)row[g]);
+                dn = ex > c * (a.strike / kTargetScale);
+            } else {
+                dn = dq;
+                if (!sn.done) ++c_fb;
+            }
+            if (!sn.done && !sq.done) {
+                ++c_dec;
+                c_dis += (dn != dq) ? 1u : 0u;
+            }
+            if (!sq.done && dq) sq = Stop{true, date, ex};
+            if (!sn.done && dn) sn = Stop{true, date, ex};
+        };
+        if constexpr (FP64) {
+            double sp = a.spot;
+            for (int step = 0; step < a.m && !(sn.done && sq.done); ++step) {
+                const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+                sp = sp * exp(a.drift + a.vsd * z);
+                visit(step + 1, sp);
+            }
+        } else {
+            float x = 0.f;
+            const float s0 = (float)a.spot;
+            int step = 0;
+            constexpr int B = 8;
+            for (; step + B <= a.m && !(sn.done && sq.done); step += B) {
+                float z[B];
+                normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    x = x + __builtin_fmaf(a.sig2, z[j], a.mu2);
+                    visit(step + j + 1, (double)(s0 * fast_exp2(x)));
+                }
+            }
+            for (; step < a.m && !(sn.done && sq.done); ++step) {
+                const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
+                x = x + __builtin_fmaf(a.sig2, z, a.mu2);
+                visit(step + 1, (double)(s0 * fast_exp2(x)));
+            }
+        }
+        double cn = sn.value, cq = sq.value;
+        for (int i = 0; i < sn.date; ++i) cn *= a.disc;
+        for (int i = 0; i < sq.date; ++i) cq *= a.disc;
+        const double rk = 1.0 / (double)(k + 1);
+        wn.n += 1;
+        double d = cn - wn.mean;
+        wn.mean += d * rk;
+        wn.m2 += d * (cn - wn.mean);
+        wq.n += 1;
+        d = cq - wq.mean;
+        wq.mean += d * rk;
+        wq.m2 += d * (cq - wq.mean);
+    }
+    const Welford tn = block_merge_256(wn, lds_n);
+    const Welford tq = block_merge_256(wq, lds_q);
+    if (threadIdx.x == 0) {
+        q.part_nn[tile] = tn;
+        q.part_qr[tile] = tq;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        c_dec += __shfl_xor(c_dec, off, 64);
+        c_dis += __shfl_xor(c_dis, off, 64);
+        c_fb += __shfl_xor(c_fb, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(q.counters + 0, c_dec);
+        atomicAdd(q.counters + 1, c_dis);
+        atomicAdd(q.counters + 2, c_fb);
+    }
+}
+
 namespace {
 
 struct LsmRun {
@@ -292,9 +458,13 @@ struct LsmRun {
     uint64_t blocks, tiles;
 };
 
+// Lays out the scratch: [store][cf][coeffs][has][qr partials][tile partials] followed by
+// `extra_bytes` for the caller (returned in *extra when non-null). `with_store` false leaves
+// out the path store and the cash-flow vector (streaming pricing).
 int lsm_setup(double spot, double strike, double rate, double carry, double vol, double time,
               uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, ThreadCtx* ctx,
-              LsmRun* run) {
+              LsmRun* run, size_t extra_bytes = 0, char** extra = nullptr,
+              bool with_store = true) {
     if (type > 1 || m < 1 || n == 0 || n > (1ull << 30)) return MCDX_EINVAL;
     if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
     const bool fp64 = precision == MCDX_PREC_FP64;
@@ -315,19 +485,21 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.type = type;
     const uint64_t blocks = (n + kLsmBlock - 1) / kLsmBlock;
     const uint64_t tiles = tiles_for(n);
-    const size_t store_bytes = (size_t)m * n * (fp64 ? 8 : 4);
+    const size_t store_bytes = with_store ? (size_t)m * n * (fp64 ? 8 : 4) : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
-    const size_t o_store = take(store_bytes), o_cf = take(n * 8),
+    const size_t o_store = take(store_bytes), o_cf = take(with_store ? n * 8 : 0),
                  o_coeffs = take((size_t)(m + 1) * 4 * 8), o_has = take((size_t)m + 1),
-                 o_qr = take(blocks * sizeof(QrState)), o_part = take((tiles + 1) * sizeof(Welford));
+                 o_qr = take(with_store ? blocks * sizeof(QrState) : 0),
+                 o_part = take((tiles + 1) * sizeof(Welford)), o_extra = take(extra_bytes);
     int st = ensure_dev(ctx, off);
     if (st) return st;
     char* base = (char*)ctx->dev;
+    if (extra) *extra = base + o_extra;
     a.store = base + o_store;
     a.cf = (double*)(base + o_cf);
     a.coeffs = (double*)(base + o_coeffs);
@@ -450,6 +622,273 @@ int lsm_reprice(double spot, double strike, double rate, double carry, double vo
     return MCDX_OK;
 }
 
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Table range in S/K: from the money boundary to 4.5 standard deviations of log(S_T/K) into
+// the money (put: below 1, call: above 1), widened when the spot sits far out of the money.
+void table_range(double spot, double strike, double rate, double carry, double vol, double time,
+                 uint8_t type, double* lo, double* hi) {
+    const double x0 = spot / strike, drift = (rate - carry - 0.5 * vol * vol) * time;
+    const double w = 4.5 * vol * sqrt(time);
+    if (type == 1) {
+        *hi = 1.0;
+        *lo = x0 * exp((drift < 0.0 ? drift : 0.0) - w);
+        if (!(*lo < 0.95)) *lo = 0.5;
+    } else {
+        *lo = 1.0;
+        *hi = x0 * exp((drift > 0.0 ? drift : 0.0) + w);
+        if (!(*hi > 1.05)) *hi = 2.0;
+    }
+}
+
+struct PolicyBufs {
+    float *table, *gx, *gt;
+    Welford *pn, *pq;
+    unsigned long long* ctr;
+};
+
+size_t policy_bytes(int32_t m, uint32_t grid, uint64_t tiles) {
+    return 3 * align256((size_t)(m - 1) * grid * 4) + 2 * align256((tiles + 1) * sizeof(Welford)) + 256;
+}
+
+PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
+    PolicyBufs b;
+    const size_t tb = align256((size_t)(m - 1) * grid * 4), wb = align256((tiles + 1) * sizeof(Welford));
+    b.table = (float*)p;
+    b.gx = (float*)(p + tb);
+    b.gt = (float*)(p + 2 * tb);
+    b.pn = (Welford*)(p + 3 * tb);
+    b.pq = (Welford*)(p + 3 * tb + wb);
+    b.ctr = (unsigned long long*)(p + 3 * tb + 2 * wb);
+    return b;
+}
+
+// Table (MFMA evaluation of the network on the grid), then the streaming policy kernel and
+// the two fixed-order merges. Results land in pn[tiles], pq[tiles] and ctr[0..2].
+int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_t grid,
+                   double lo, double hi, bool fp64, hipStream_t s) {
+    const int m = r.a.m;
+    const uint64_t pts = (uint64_t)(m - 1) * grid;
+    const double h = (hi - lo) / (double)(grid - 1);
+    hipLaunchKernelGGL(lsm_grid_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, b.gx,
+                       b.gt, m, grid, lo, h);
+    int st = mlp_eval_device(w, b.gx, b.gt, pts, b.table, s);
+    if (st) return st;
+    hipLaunchKernelGGL(lsm_zero_counters_kernel, dim3(1), dim3(64), 0, s, b.ctr);
+    PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, b.pn, b.pq, b.ctr};
+    if (fp64)
+        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a, q);
+    else
+        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a, q);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_final_merge(b.pn, nullptr, r.tiles, 1, b.pn + r.tiles, s);
+    if (e == hipSuccess) e = launch_final_merge(b.pq, nullptr, r.tiles, 1, b.pq + r.tiles, s);
+    return status_of(e);
+}
+
+// Pinned read-back layout for the policy results.
+struct PolicyHost {
+    Welford qr_lsm, nn, qr;
+    unsigned long long ctr[3];
+    double loss_sum;
+};
+
+// Inception rule (lsm.cpp:141-148) applied to a forward-policy price.
+mcdx_result with_inception(const Welford& w, double spot, double strike, uint8_t type) {
+    const mcdx_result r = finish(w);
+    const double inc = intrinsic_v(spot, strike, type);
+    return inc > r.price ? mcdx_result{inc, 0.0, r.path_count} : r;
+}
+
+// Largest gap in S/K, over dates with coefficients, between the network's and the QR
+// exercise boundaries located on the table grid.
+void boundary_gap(const float* table, const double* coeffs, const uint8_t* has, int32_t m,
+                  uint32_t grid, double lo, double hi, double strike, uint8_t type, double* gap,
+                  uint32_t* dates) {
+    const double h = (hi - lo) / (double)(grid - 1);
+    *gap = 0.0;
+    *dates = 0;
+    for (int d = 1; d <= m - 1; ++d) {
+        if (!has[d]) continue;
+        double bn = NAN, bq = NAN;
+        for (uint32_t g = 0; g < grid; ++g) {
+            const double x = lo + h * (double)g;
+            const double ex = intrinsic_v(x * strike, strike, type);
+            if (!(ex > 0.0)) continue;
+            const bool en = ex > (double)table[(size_t)(d - 1) * grid + g] * (strike / kTargetScale);
+            double b[4];
+            laguerre4(x, b);
+            double cont = 0.0;
+            for (int j = 0; j < 4; ++j) cont += coeffs[4 * d + j] * b[j];
+            const bool eq = ex > cont;
+            if (type == 1) { // put: exercise below the boundary; keep the largest such x
+                if (en) bn = x;
+                if (eq) bq = x;
+            } else { // call: exercise above; keep the smallest
+                if (en && bn != bn) bn = x;
+                if (eq && bq != bq) bq = x;
+            }
+        }
+        if (bn == bn && bq == bq) {
+            const double gdiff = fabs(bn - bq);
+            if (gdiff > *gap) *gap = gdiff;
+            ++*dates;
+        }
+    }
+}
+
+} // namespace
+
+int lsm_learned(double spot, double strike, double rate, double carry, double vol, double time,
+                uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision,
+                const mcdx_lsm_learn_options* o, float* weights_out, double* coeffs_out,
+                uint8_t* has_out, mcdx_lsm_learned_result* out) {
+    if (!out || !o || m < 2 || o->steps < 1 || o->steps > (1 << 22) || o->grid_points < 2 ||
+        o->grid_points > 4096 || !(o->learning_rate > 0.f) || !(o->final_learning_rate > 0.f))
+        return MCDX_EINVAL;
+    if ((uint64_t)(m - 1) * n > (1ull << 34)) return MCDX_EINVAL;
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    const uint32_t grid = o->grid_points;
+    const size_t cap_bytes = align256((size_t)(m - 1) * n * 4);
+    const size_t mlp_bytes = align256(3 * (size_t)kMlpParams * 4) + align256((size_t)o->steps * 4);
+    const uint64_t tiles = tiles_for(n);
+    char* ex = nullptr;
+    LsmRun r;
+    if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r,
+                        3 * cap_bytes + mlp_bytes + policy_bytes(m, grid, tiles), &ex)))
+        return st;
+    r.a.cap_f1 = (float*)ex;
+    r.a.cap_f2 = (float*)(ex + cap_bytes);
+    r.a.cap_y = (float*)(ex + 2 * cap_bytes);
+    float* w = (float*)(ex + 3 * cap_bytes);
+    float* am = w + kMlpParams;
+    float* av = am + kMlpParams;
+    float* losses = (float*)(ex + 3 * cap_bytes + align256(3 * (size_t)kMlpParams * 4));
+    const PolicyBufs pb = carve_policy(ex + 3 * cap_bytes + mlp_bytes, m, grid, tiles);
+    const size_t tbl = (size_t)(m - 1) * grid;
+    const size_t h_tbl = align256(sizeof(PolicyHost)), h_loss = h_tbl + align256(tbl * 4),
+                 h_coef = h_loss + align256((size_t)o->steps * 4),
+                 h_has = h_coef + align256((size_t)(m + 1) * 32),
+                 h_w = h_has + align256((size_t)m + 1), h_end = h_w + align256((size_t)kMlpParams * 4);
+    if ((st = ensure_pinned(ctx, h_end))) return st;
+    const bool fp64 = precision == MCDX_PREC_FP64;
+    hipStream_t s = ctx->stream;
+    hipError_t e = fp64 ? enqueue_paths<true>(r, s) : enqueue_paths<false>(r, s);
+    if (e == hipSuccess) e = fp64 ? enqueue_backward<true>(r, true, s) : enqueue_backward<false>(r, true, s);
+    if (e != hipSuccess) return status_of(e);
+    // Train on every captured (path, date) sample; OTM samples carry NaN targets.
+    if ((st = mlp_init_device(w, am, av, o->init_seed, 0.f, s))) return st;
+    const float decay = (float)pow((double)o->final_learning_rate / (double)o->learning_rate,
+                                   1.0 / (double)o->steps);
+    if ((st = mlp_train_device(w, am, av, r.a.cap_f1, r.a.cap_f2, r.a.cap_y, (uint64_t)(m - 1) * n,
+                               o->steps, o->learning_rate, decay, o->sample_seed, losses, nullptr, s)))
+        return st;
+    double lo, hi;
+    table_range(spot, strike, rate, carry, vol, time, type, &lo, &hi);
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, fp64, s))) return st;
+    char* h = (char*)ctx->pinned;
+    PolicyHost* ph = (PolicyHost*)h;
+    e = hipMemcpyAsync(&ph->qr_lsm, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ph->qr, pb.pq + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_tbl, pb.table, tbl * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_loss, losses, (size_t)o->steps * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_coef, r.a.coeffs, (size_t)(m + 1) * 32, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_has, r.a.has, (size_t)m + 1, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_w, w, (size_t)kMlpParams * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    const double* hc = (const double*)(h + h_coef);
+    const uint8_t* hh = (const uint8_t*)(h + h_has);
+    const float* hl = (const float*)(h + h_loss);
+    mcdx_lsm_learned_result res{};
+    res.qr_lsm = with_inception(ph->qr_lsm, spot, strike, type);
+    res.learned = with_inception(ph->nn, spot, strike, type);
+    res.qr_forward = with_inception(ph->qr, spot, strike, type);
+    res.itm_decisions = ph->ctr[0];
+    res.disagreements = ph->ctr[1];
+    res.fallbacks = ph->ctr[2];
+    const int last = o->steps < 100 ? o->steps : 100;
+    double ls = 0.0;
+    for (int i = o->steps - last; i < o->steps; ++i) ls += (double)hl[i];
+    res.final_loss = ls / (double)last / (kTargetScale * kTargetScale);
+    boundary_gap((const float*)(h + h_tbl), hc, hh, m, grid, lo, hi, strike, type, &res.boundary_gap,
+                 &res.boundary_dates);
+    res.table_lo = lo;
+    res.table_hi = hi;
+    if (coeffs_out)
+        for (int i = 0; i < (m + 1) * 4; ++i) coeffs_out[i] = hc[i];
+    if (has_out)
+        for (int i = 0; i <= m; ++i) has_out[i] = hh[i];
+    if (weights_out)
+        for (int i = 0; i < kMlpParams; ++i) weights_out[i] = ((const float*)(h + h_w))[i];
+    *out = res;
+    return MCDX_OK;
+}
+
+int lsm_learned_price(double spot, double strike, double rate, double carry, double vol,
+                      double time, uint8_t type, int32_t m, uint64_t n, uint64_t seed,
+                      int precision, const float* weights, uint32_t grid, double lo, double hi,
+                      const double* coeffs, const uint8_t* has, mcdx_lsm_learned_result* out) {
+    if (!out || !weights || !coeffs || !has || m < 2 || grid < 2 || grid > 4096 || !(hi > lo))
+        return MCDX_EINVAL;
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    const uint64_t tiles = tiles_for(n);
+    char* ex = nullptr;
+    LsmRun r;
+    if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r,
+                        align256((size_t)kMlpParams * 4) + policy_bytes(m, grid, tiles), &ex, false)))
+        return st;
+    float* w = (float*)ex;
+    const PolicyBufs pb = carve_policy(ex + align256((size_t)kMlpParams * 4), m, grid, tiles);
+    const size_t tbl = (size_t)(m - 1) * grid;
+    const size_t h_tbl = align256(sizeof(PolicyHost)), h_coef = h_tbl + align256(tbl * 4),
+                 h_has = h_coef + align256((size_t)(m + 1) * 32),
+                 h_w = h_has + align256((size_t)m + 1), h_end = h_w + align256((size_t)kMlpParams * 4);
+    if ((st = ensure_pinned(ctx, h_end))) return st;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipStreamSynchronize(s); // the pinned staging is free
+    if (e != hipSuccess) return status_of(e);
+    char* h = (char*)ctx->pinned;
+    for (int i = 0; i < (m + 1) * 4; ++i) ((double*)(h + h_coef))[i] = coeffs[i];
+    for (int i = 0; i <= m; ++i) ((uint8_t*)(h + h_has))[i] = has[i] ? 1 : 0;
+    for (int i = 0; i < kMlpParams; ++i) ((float*)(h + h_w))[i] = weights[i];
+    e = hipMemcpyAsync(r.a.coeffs, h + h_coef, (size_t)(m + 1) * 32, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(r.a.has, h + h_has, (size_t)m + 1, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return status_of(e);
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, s))) return st;
+    e = hipStreamSynchronize(s); // uploads consumed before the staging is reused
+    PolicyHost* ph = (PolicyHost*)h;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ph->qr, pb.pq + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + h_tbl, pb.table, tbl * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    mcdx_lsm_learned_result res{};
+    res.qr_lsm = mcdx_result{NAN, NAN, 0};
+    res.learned = with_inception(ph->nn, spot, strike, type);
+    res.qr_forward = with_inception(ph->qr, spot, strike, type);
+    res.itm_decisions = ph->ctr[0];
+    res.disagreements = ph->ctr[1];
+    res.fallbacks = ph->ctr[2];
+    res.final_loss = NAN;
+    boundary_gap((const float*)(h + h_tbl), coeffs, has, m, grid, lo, hi, strike, type,
+                 &res.boundary_gap, &res.boundary_dates);
+    res.table_lo = lo;
+    res.table_hi = hi;
+    *out = res;
+    return MCDX_OK;
+}
+
 } // namespace mcdx
 
 using namespace mcdx;
@@ -481,6 +920,38 @@ int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carr
     return lsm_reprice(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
                        path_count, seed, coefficients, has_coefficients, exercise_at_inception,
                        precision, out);
+}
+
+void mcdx_lsm_learn_default_options(mcdx_lsm_learn_options* o) {
+    if (!o) return;
+    o->init_seed = 7;
+    o->sample_seed = 11;
+    o->steps = 6000;
+    o->learning_rate = 3e-3f;
+    o->final_learning_rate = 1e-4f;
+    o->grid_points = 1024;
+}
+
+int mcdx_lsm_american_learned(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, int32_t monitoring_points,
+                              uint64_t path_count, uint64_t seed, int precision,
+                              const mcdx_lsm_learn_options* options, float* weights,
+                              double* coefficients, uint8_t* has_coefficients,
+                              mcdx_lsm_learned_result* out) {
+    return lsm_learned(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
+                       path_count, seed, precision, options, weights, coefficients,
+                       has_coefficients, out);
+}
+
+int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry_yield,
+                           double vol, double time, uint8_t type, int32_t monitoring_points,
+                           uint64_t path_count, uint64_t seed, int precision,
+                           const float* weights, uint32_t grid_points, double table_lo,
+                           double table_hi, const double* coefficients,
+                           const uint8_t* has_coefficients, mcdx_lsm_learned_result* out) {
+    return lsm_learned_price(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
+                             path_count, seed, precision, weights, grid_points, table_lo,
+                             table_hi, coefficients, has_coefficients, out);
 }
 
 } // extern "C"

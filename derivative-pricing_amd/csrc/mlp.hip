@@ -18,6 +18,7 @@
 
 #include <vector>
 
+#include "mlp.hpp"
 #include "runtime.hpp"
 
 namespace mcdx {
@@ -25,6 +26,7 @@ namespace mcdx {
 constexpr int kH = 64;          // hidden width
 constexpr int kLd = 68;         // padded LDS row (bf16 elements)
 constexpr int kParams = 2 * kH + kH + kH * kH + kH + kH + 1; // 4417
+static_assert(kParams == kMlpParams, "parameter count");
 // parameter offsets in the flat FP32 vector
 constexpr int oW1 = 0, oB1 = 2 * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
               oB3 = oW3 + kH;
@@ -271,13 +273,29 @@ __global__ __launch_bounds__(128) void mlp_train_kernel(TrainArgs a) {
     for (int step = 0; step < a.steps; ++step) {
         load_weights(a.w, L);
         __syncthreads();
-        uint64_t idx;
-        if (a.grad_out) idx = threadIdx.x < a.n ? threadIdx.x : 0;
-        else idx = (uint64_t)philox_bits(a.seed, (uint64_t)step * 128 + threadIdx.x, 0, 7) % a.n;
-        const float x = a.xs[idx], t = a.ts[idx], y = a.ys[idx];
+        uint64_t idx = 0;
+        float y = 0.f;
+        if (a.grad_out) {
+            idx = threadIdx.x < a.n ? threadIdx.x : 0;
+            y = a.ys[idx];
+        } else {
+            // Rejection sampling over the valid (non-NaN target) samples; the draw index is
+            // the attempt, so every lane's choice is a fixed function of (seed, step, lane).
+            const uint64_t ctr = (uint64_t)step * 128 + threadIdx.x;
+            for (uint32_t tr = 0; tr < 8; ++tr) {
+                uint64_t r = philox_bits(a.seed, ctr, tr, 7);
+                if (a.n > 0xFFFFFFFFull) r |= (uint64_t)philox_bits(a.seed, ctr, tr + 8, 7) << 32;
+                idx = r % a.n;
+                y = a.ys[idx];
+                if (y == y) break;
+            }
+        }
+        const float valid = (y == y) ? 1.f : 0.f;
+        const float x = valid != 0.f ? a.xs[idx] : 0.f, t = valid != 0.f ? a.ts[idx] : 0.f;
+        if (valid == 0.f) y = 0.f;
         f16v acc[2][2];
         const float yhat = forward64(L, W, x, t, acc);
-        const float diff = yhat - y;
+        const float diff = (yhat - y) * valid;
         const float g3 = 2.0f * diff / 128.0f;
         float l2 = diff * diff;
 #pragma unroll
@@ -328,9 +346,35 @@ __global__ __launch_bounds__(128) void mlp_eval_kernel(const float* __restrict__
     if (i < n) out[i] = y;
 }
 
+// w[i] from Philox(seed, i): layer-wise scales (see mlp.hpp); Adam moments zeroed.
+__global__ __launch_bounds__(256) void mlp_init_kernel(float* w, float* m, float* v,
+                                                       uint64_t seed, float y_mean) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    const float z = inv_normal_f32(philox_w0((uint32_t)i, 0u, 0u, 9u, (uint32_t)seed,
+                                             (uint32_t)(seed >> 32)));
+    float val;
+    if (i < oB1) val = z;                      // W1: unit-scale features
+    else if (i < oW2) val = z;                 // b1
+    else if (i < oB2) val = z * 0.125f * 1.41421356f; // W2: He scale sqrt(2/64)
+    else if (i < oW3) val = 0.f;               // b2
+    else if (i < oB3) val = z * 0.125f;        // W3
+    else val = y_mean;                         // b3
+    w[i] = val;
+    m[i] = 0.f;
+    v[i] = 0.f;
+}
+
 namespace {
 size_t lds_bytes() { return sizeof(MlpLds); }
 } // namespace
+
+int mlp_init_device(float* w, float* adam_m, float* adam_v, uint64_t seed, float y_mean,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(mlp_init_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, w, adam_m,
+                       adam_v, seed, y_mean);
+    return status_of(hipGetLastError());
+}
 
 int mlp_train_device(float* w, float* m, float* v, const float* xs, const float* ts,
                      const float* ys, uint64_t n, int steps, float lr, float lr_decay,
@@ -393,6 +437,42 @@ int mcdx_mlp_gradients(const float* weights, const float* xs, const float* ts, c
     e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(grad, dg, pb, hipMemcpyDeviceToHost);
     if (e == hipSuccess && loss2) e = hipMemcpy(loss2, dl, 8, hipMemcpyDeviceToHost);
+    return status_of(e);
+}
+
+// `steps` Adam steps from `weights` (updated in place) with zero initial moments; samples with
+// a NaN target are skipped. losses[step] (optional) = first-half batch MSE per step.
+int mcdx_mlp_train(float* weights, const float* xs, const float* ts, const float* ys,
+                   uint64_t n, int32_t steps, float learning_rate, float lr_decay,
+                   uint64_t seed, float* losses) {
+    if (!weights || !xs || !ts || !ys || n == 0 || steps < 1 || n > (1ull << 28) ||
+        steps > (1 << 24))
+        return MCDX_EINVAL;
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    const size_t pb = kParams * sizeof(float);
+    const size_t bytes = 3 * pb + 3 * n * sizeof(float) + (size_t)steps * sizeof(float) + 1024;
+    if ((st = ensure_dev(ctx, bytes))) return st;
+    float* dw = (float*)ctx->dev;
+    float* dm = dw + kParams;
+    float* dv = dm + kParams;
+    float* dx = dv + kParams;
+    float* dt = dx + n;
+    float* dy = dt + n;
+    float* dl = dy + n;
+    hipError_t e = hipMemcpy(dw, weights, pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dx, xs, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, ts, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dy, ys, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(dm, 0, 2 * pb, ctx->stream);
+    if (e != hipSuccess) return status_of(e);
+    if ((st = mlp_train_device(dw, dm, dv, dx, dt, dy, n, steps, learning_rate, lr_decay, seed,
+                               dl, nullptr, ctx->stream)))
+        return st;
+    e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(weights, dw, pb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && losses) e = hipMemcpy(losses, dl, (size_t)steps * 4, hipMemcpyDeviceToHost);
     return status_of(e);
 }
 

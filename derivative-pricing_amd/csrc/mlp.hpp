@@ -1,0 +1,25 @@
+// Device-side entry points of the continuation-value network (mlp.hip), for pipelines that
+// keep every buffer resident in HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcdx {
+
+constexpr int kMlpParams = 4417; // 2 -> 64 -> 64 -> 1
+
+// Deterministic initialisation from Philox(seed): N(0,1) first layer and biases, He-scaled
+// second layer, 1/8-scaled output layer, output bias = y_mean; Adam moments zeroed.
+int mlp_init_device(float* w, float* adam_m, float* adam_v, uint64_t seed, float y_mean,
+                    hipStream_t s);
+// `steps` Adam steps of 128 samples drawn by Philox(seed) from (xs, ts, ys)[0..n); samples
+// whose target is NaN are redrawn (up to 8 times, then masked). loss[step] receives the
+// mean squared error of the step's first 64 samples when non-null.
+int mlp_train_device(float* w, float* m, float* v, const float* xs, const float* ts,
+                     const float* ys, uint64_t n, int steps, float lr, float lr_decay,
+                     uint64_t seed, float* loss, float* grad, hipStream_t s);
+int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
+                    hipStream_t s);
+
+} // namespace mcdx

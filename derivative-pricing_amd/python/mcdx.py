@@ -55,6 +55,20 @@ def contract(product, spot=100.0, strike=100.0, barrier=0.0, rebate=0.0, rate=0.
     return c
 
 
+class LearnOptions(C.Structure):
+    _fields_ = [("init_seed", C.c_uint64), ("sample_seed", C.c_uint64), ("steps", C.c_int32),
+                ("learning_rate", C.c_float), ("final_learning_rate", C.c_float),
+                ("grid_points", C.c_uint32)]
+
+
+class LearnedResult(C.Structure):
+    _fields_ = [("qr_lsm", Result), ("learned", Result), ("qr_forward", Result),
+                ("itm_decisions", C.c_uint64), ("disagreements", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("final_loss", C.c_double),
+                ("boundary_gap", C.c_double), ("boundary_dates", C.c_uint32),
+                ("reserved", C.c_uint32), ("table_lo", C.c_double), ("table_hi", C.c_double)]
+
+
 class HestonParams(C.Structure):
     _fields_ = [(n, dbl) for n in ("spot", "rate", "carry_yield", "v0", "kappa", "theta", "xi",
                                    "rho", "time")]
@@ -113,6 +127,15 @@ SIGNATURES = {
     "mcdx_mlp_param_count": (u32, []),
     "mcdx_mlp_gradients": (C.c_int, [P(C.c_float)] * 4 + [u32, P(C.c_float), P(C.c_float)]),
     "mcdx_mlp_forward": (C.c_int, [P(C.c_float)] * 3 + [u64, P(C.c_float)]),
+    "mcdx_mlp_train": (C.c_int, [P(C.c_float)] * 4 + [u64, i32, C.c_float, C.c_float, u64,
+                                                      P(C.c_float)]),
+    "mcdx_lsm_learn_default_options": (None, [P(LearnOptions)]),
+    "mcdx_lsm_american_learned": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int,
+                                                         P(LearnOptions), P(C.c_float), P(dbl),
+                                                         P(u8), P(LearnedResult)]),
+    "mcdx_lsm_learned_price": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, P(C.c_float),
+                                                      u32, dbl, dbl, P(dbl), P(u8),
+                                                      P(LearnedResult)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
@@ -263,6 +286,64 @@ def mlp_forward(w, x, t):
     check(lib().mcdx_mlp_forward(pw, px, pt, len(x_), out.ctypes.data_as(P(C.c_float))),
           "mcdx_mlp_forward")
     return out
+
+
+def mlp_train(w, x, t, y, steps, lr, lr_decay, seed):
+    w = np.array(w, dtype=np.float32)
+    (x_, px), (t_, pt), (y_, py) = _f32(x), _f32(t), _f32(y)
+    losses = np.empty(steps, dtype=np.float32)
+    check(lib().mcdx_mlp_train(w.ctypes.data_as(P(C.c_float)), px, pt, py, len(x_), steps, lr,
+                               lr_decay, seed, losses.ctypes.data_as(P(C.c_float))),
+          "mcdx_mlp_train")
+    return w, losses
+
+
+def learn_options(**kw):
+    o = LearnOptions()
+    lib().mcdx_lsm_learn_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _learned_dict(r):
+    res = lambda x: (x.price, x.standard_error, x.path_count)  # noqa: E731
+    return {"qr_lsm": res(r.qr_lsm), "learned": res(r.learned), "qr_forward": res(r.qr_forward),
+            "itm_decisions": r.itm_decisions, "disagreements": r.disagreements,
+            "fallbacks": r.fallbacks, "final_loss": r.final_loss,
+            "boundary_gap": r.boundary_gap, "boundary_dates": r.boundary_dates,
+            "table_lo": r.table_lo, "table_hi": r.table_hi}
+
+
+def lsm_learned(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, **opts):
+    """Fit + train + price on the same paths. Returns (dict, weights, coeffs, has)."""
+    o = learn_options(**opts)
+    r = LearnedResult()
+    w = np.zeros(mlp_param_count(), dtype=np.float32)
+    co = np.zeros((m + 1) * 4)
+    has = np.zeros(m + 1, dtype=np.uint8)
+    check(lib().mcdx_lsm_american_learned(spot, strike, rate, carry, vol, time, type, m, n, seed,
+                                          precision, C.byref(o), w.ctypes.data_as(P(C.c_float)),
+                                          co.ctypes.data_as(P(dbl)), has.ctypes.data_as(P(u8)),
+                                          C.byref(r)), "mcdx_lsm_american_learned")
+    return _learned_dict(r), w, co, has
+
+
+def lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, w,
+                      grid, lo, hi, coeffs, has):
+    r = LearnedResult()
+    w_, pw = _f32(w)
+    co = np.ascontiguousarray(coeffs, dtype=np.float64).ravel()
+    hs = np.ascontiguousarray(has, dtype=np.uint8)
+    check(lib().mcdx_lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed,
+                                       precision, pw, grid, lo, hi, co.ctypes.data_as(P(dbl)),
+                                       hs.ctypes.data_as(P(u8)), C.byref(r)),
+          "mcdx_lsm_learned_price")
+    return _learned_dict(r)
+
+
+def mlp_param_count():
+    return lib().mcdx_mlp_param_count()
 
 
 class Book:

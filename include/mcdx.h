@@ -235,6 +235,60 @@ int mcdx_mlp_gradients(const float* weights, const float* xs, const float* ts, c
 /* out[i] = network(xs[i], ts[i]) through the device's BF16-MFMA forward pass. */
 int mcdx_mlp_forward(const float* weights, const float* xs, const float* ts, uint64_t n,
                      float* out);
+/* `steps` Adam steps (128 samples each, drawn by Philox(seed); NaN targets are skipped) from
+ * `weights`, updated in place, with zero initial moments. losses[step] (optional) receives
+ * the mean squared error of each step's first 64 samples. */
+int mcdx_mlp_train(float* weights, const float* xs, const float* ts, const float* ys,
+                   uint64_t n, int32_t steps, float learning_rate, float lr_decay,
+                   uint64_t seed, float* losses);
+
+/* ---- Learned-continuation LSM (no reference counterpart; extends src/pricers/lsm.cpp) ----
+ * The QR-LSM backward pass captures every (path, date) regression sample; the network
+ * C(4 (S/K - 1), 2 t/T - 1) ~ 10 cf/K is trained on them on the device, tabulated per
+ * exercise date on a uniform S/K grid, and a streaming kernel exercises each path at the
+ * first date where the intrinsic value beats the tabulated continuation (the QR
+ * coefficients decide outside the table range). The same kernel replays the QR policy
+ * forward on the same paths, which reproduces the backward-induction price. */
+typedef struct mcdx_lsm_learn_options {
+    uint64_t init_seed;        /* weight initialisation (Philox) */
+    uint64_t sample_seed;      /* minibatch sampling (Philox) */
+    int32_t steps;             /* Adam steps, 128 samples each */
+    float learning_rate;       /* first step */
+    float final_learning_rate; /* geometric decay to this by the last step */
+    uint32_t grid_points;      /* continuation table points per exercise date, 2..4096 */
+} mcdx_lsm_learn_options;
+
+typedef struct mcdx_lsm_learned_result {
+    mcdx_result qr_lsm;       /* QR-LSM backward induction (NaN from mcdx_lsm_learned_price) */
+    mcdx_result learned;      /* first exercise by the network's continuation value */
+    mcdx_result qr_forward;   /* first exercise by the QR coefficients, same paths */
+    uint64_t itm_decisions;   /* in-the-money path-dates reached by both policies */
+    uint64_t disagreements;   /* of those, where the two exercise decisions differ */
+    uint64_t fallbacks;       /* path-dates decided by QR because S/K left the table range */
+    double final_loss;        /* training MSE over the last min(100, steps) steps, (cf/K)^2 */
+    double boundary_gap;      /* max over dates of |boundary_net - boundary_qr| in S/K */
+    uint32_t boundary_dates;  /* dates where both boundaries lie on the grid */
+    uint32_t reserved;
+    double table_lo, table_hi; /* S/K range of the table */
+} mcdx_lsm_learned_result;
+
+void mcdx_lsm_learn_default_options(mcdx_lsm_learn_options* options);
+/* Fit (QR-LSM on `path_count` paths of `seed`), train, tabulate and price on the same paths.
+ * weights[4417], coefficients[(M+1)*4] and has_coefficients[M+1] receive the trained
+ * network and the QR policy when non-null. */
+int mcdx_lsm_american_learned(double spot, double strike, double rate, double carry_yield,
+                              double vol, double time, uint8_t type, int32_t monitoring_points,
+                              uint64_t path_count, uint64_t seed, int precision,
+                              const mcdx_lsm_learn_options* options, float* weights,
+                              double* coefficients, uint8_t* has_coefficients,
+                              mcdx_lsm_learned_result* out);
+/* Price with a trained network and QR policy on fresh paths: streaming, no path store. */
+int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry_yield,
+                           double vol, double time, uint8_t type, int32_t monitoring_points,
+                           uint64_t path_count, uint64_t seed, int precision,
+                           const float* weights, uint32_t grid_points, double table_lo,
+                           double table_hi, const double* coefficients,
+                           const uint8_t* has_coefficients, mcdx_lsm_learned_result* out);
 
 /* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
  * No reference counterpart (the reference prices one contract per call,

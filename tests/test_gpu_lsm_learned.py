@@ -1,0 +1,108 @@
+"""GPU checks of the learned-continuation LSM (no reference counterpart).
+
+Parity anchors: (1) replaying the QR policy forward in the streaming kernel reproduces the
+QR-LSM backward-induction price bit for bit, which pins the streaming kernel's paths,
+decisions and discounting against the backward pass that is itself checked against the
+oracle; (2) the Adam step of the training kernel against the numpy oracle
+(oracle/mlp_oracle.py) on the batch the oracle's Philox picks; (3) the learned price at the
+American put configuration (S=100 K=100 r=0.06 sigma=0.2 T=1, 50 dates, 2^20 paths) within
+1 SE of QR-LSM on the same paths and within 3 SE + 0.05 of CRR (the reference's LSM bar,
+tests/lsm_test.cpp:76-98; 50 dates price a Bermudan, below the American value)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mcdx
+import oracle_lib as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+FP32, FP64 = mcdx.PREC_FP32, mcdx.PREC_FP64
+PUT = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50)
+
+
+def sample_indices(seed, step, n):
+    """Indices the training kernel draws for one step (first attempt; all targets valid)."""
+    out = []
+    for lane in range(128):
+        b = int(O.philox_bits(seed, step * 128 + lane, 1, 0, 7)[0])
+        out.append(b % n)
+    return np.array(out)
+
+
+def test_adam_first_step_matches_oracle():
+    rng = np.random.default_rng(3)
+    n = 500
+    x = rng.uniform(-2, 0, n).astype(np.float32)
+    t = rng.uniform(-1, 1, n).astype(np.float32)
+    y = (np.maximum(-x, 0) * 2 + 0.1 * rng.normal(size=n)).astype(np.float32)
+    w0 = M.init(11)
+    lr = 1e-3
+    w1, losses = mcdx.mlp_train(w0, x, t, y, 1, lr, 1.0, 77)
+    idx = sample_indices(77, 0, n)
+    g, yhat = M.gradients(w0, x[idx], t[idx], y[idx], bf=True)
+    # zero moments: m_hat = g, v_hat = g^2, so the step is lr * g / (|g| + 1e-8)
+    want = w0 - lr * g / (np.abs(g) + 1e-8)
+    big = np.abs(g) > 1e-2 * np.abs(g).max()
+    assert big.sum() > 1000
+    assert np.max(np.abs((w1 - want)[big])) < 2e-2 * lr
+    assert abs(losses[0] - np.mean((yhat[:64] - y[idx][:64]) ** 2)) < 1e-3 * losses[0] + 1e-6
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(5)
+    n = 1 << 16
+    x = rng.uniform(-2, 0, n).astype(np.float32)
+    t = rng.uniform(-1, 1, n).astype(np.float32)
+    y = (np.sin(2 * x) * (1 + 0.3 * t) + 0.05 * rng.normal(size=n)).astype(np.float32)
+    y[::3] = np.nan  # masked samples are redrawn
+    w, losses = mcdx.mlp_train(M.init(1), x, t, y, 3000, 3e-3, 0.999, 9)
+    assert np.isfinite(w).all()
+    assert losses[-200:].mean() < 0.05 * losses[:20].mean()
+    assert losses[-200:].mean() < 4 * 0.05 ** 2
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64])
+def test_qr_forward_replay_reproduces_backward_induction(prec):
+    n = 20_011  # a ragged last tile
+    d, w, co, has = mcdx.lsm_learned(*PUT, n, 42, prec, steps=200)
+    p, se, co2, has2, _ = mcdx.lsm(*PUT, n, 42, prec)
+    assert d["qr_lsm"][0] == p and d["qr_lsm"][1] == se
+    assert np.array_equal(co, co2.ravel()) and np.array_equal(has, has2)
+    assert d["qr_forward"][:2] == d["qr_lsm"][:2]
+    # serving the same network and policy on the same paths reproduces the pipeline's prices
+    s = mcdx.lsm_learned_price(*PUT, n, 42, prec, w, 1024, d["table_lo"], d["table_hi"], co, has)
+    assert s["learned"] == d["learned"] and s["qr_forward"] == d["qr_forward"]
+    assert s["disagreements"] == d["disagreements"] and s["fallbacks"] == d["fallbacks"]
+
+
+def test_learned_american_put_c5():
+    n = 1 << 20
+    d, w, co, has = mcdx.lsm_learned(*PUT, n, 42, FP32)
+    crr = O.lib().orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 10_000, 1)
+    qp, qse, _ = d["qr_lsm"]
+    lp, lse, _ = d["learned"]
+    print("\nqr", qp, qse, "learned", lp, lse, "crr", crr, "loss", d["final_loss"],
+          "disagree", d["disagreements"], "/", d["itm_decisions"], "fallbacks", d["fallbacks"],
+          "gap", d["boundary_gap"], d["boundary_dates"])
+    assert abs(lp - qp) <= qse
+    assert abs(lp - crr) <= 3 * lse + 0.05
+    assert d["disagreements"] <= 0.03 * d["itm_decisions"]
+    assert d["fallbacks"] <= 1e-3 * d["itm_decisions"]
+    # the trained network is what makes the policy: an untrained one prices far lower
+    u, _, _, _ = mcdx.lsm_learned(*PUT, 1 << 16, 42, FP32, steps=1)
+    assert u["learned"][0] < u["qr_lsm"][0] - 5 * u["qr_lsm"][1]
+    # out of sample (fresh paths, streaming, no path store) and at other spots (features are
+    # S/K): within the same bar of CRR
+    for spot in (90.0, 100.0, 110.0):
+        args = (spot,) + PUT[1:]
+        s = mcdx.lsm_learned_price(*args, 1 << 19, 4242, FP32, w, 1024, d["table_lo"],
+                                   d["table_hi"], co, has)
+        c = O.lib().orc_crr_american(spot, 100, 0.06, 0, 0.2, 1, 10_000, 1)
+        print(spot, s["learned"], s["qr_forward"], c, s["fallbacks"])
+        assert abs(s["learned"][0] - c) <= 3 * s["learned"][1] + 0.05
+        assert abs(s["learned"][0] - s["qr_forward"][0]) <= s["learned"][1]
