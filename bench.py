@@ -113,8 +113,10 @@ def timed_events(torch, stream, fn, reps=1):
 
 def extras(torch, mcdx, L, stream, handle, ref):
     """Secondary configurations: C1 European, C2 bridge-corrected barrier and Sobol QMC,
-    C3 Heston 9 strikes, C4 batched book vs per-contract launches."""
-    import oracle_lib as O
+    C3 Heston 9 strikes, C4 batched book vs per-contract launches, C5 American put by
+    QR-LSM and by the learned continuation value. Closed forms (Black-Scholes, Heston
+    semi-analytic, CRR) come from the engine's own host library (the `mcd` module)."""
+    import mcd
     res = {}
     fh = float.fromhex
     dev = torch.zeros(3, dtype=torch.float64, device="cuda")
@@ -124,7 +126,7 @@ def extras(torch, mcdx, L, stream, handle, ref):
                                             C.c_void_p(dev.data_ptr())), "enqueue")
 
     # C1: European call, 1e6 paths, seed 42 (launch-latency scale) and 2^24 steady state.
-    bsm = O.lib().orc_bsm(100, 100, 0.05, 0.0, 0.2, 1.0, 0)
+    bsm = mcd.black_scholes_merton(100, 100, 0.05, 0.0, 0.2, 1.0, mcd.OptionType.Call)
     for n in (1_000_000, 1 << 24):
         c = mcdx.contract(mcdx.EUROPEAN, path_count=n, seed=42)
         enq(c)
@@ -166,8 +168,11 @@ def extras(torch, mcdx, L, stream, handle, ref):
     t0 = time.perf_counter()
     r = mcdx.heston_strikes(par, ks, 0, STEPS, 1 << 22, SEED, mcdx.PREC_FP32)
     wall = time.perf_counter() - t0
-    h = O.heston()
-    sa = [O.lib().orc_heston_semi_analytic(C.byref(h), k, 0) for k in ks]
+    h = mcd.HestonParams()
+    for k_, v_ in dict(spot=100.0, rate=0.03, carry_yield=0.0, v0=0.04, kappa=2.0, theta=0.04,
+                       xi=0.5, rho=-0.7, time=1.0).items():
+        setattr(h, k_, v_)
+    sa = [mcd.heston_semi_analytic(h, k, mcd.OptionType.Call) for k in ks]
     res["c3_heston_qe_9_strikes"] = {
         "paths_per_s_wall": (1 << 22) / wall, "path_steps_per_s_wall": (1 << 22) * STEPS / wall,
         "ms_wall": wall * 1e3,
@@ -201,7 +206,56 @@ def extras(torch, mcdx, L, stream, handle, ref):
                            "autotuned_schedule": book.schedule(), "autotune_best_ms": tuned_ms,
                            "bitwise_equal_to_per_contract": same}
     book.close()
+    progress("C4 done")
+    res["c5_american_put_lsm"] = c5_lsm(mcdx, mcd)
+    progress("C5 done")
     return res
+
+
+def c5_lsm(mcdx, mcd):
+    """C5: American put S=100 K=100 r=0.06 sigma=0.2 T=1, 50 exercise dates, 2^20 paths,
+    FP32 store. QR-LSM (store + 49 device regressions) against the learned continuation
+    value: the full pipeline (QR fit, capture, training, table, pricing) once, then the
+    trained policy serving fresh paths in one streaming kernel (no path store). Wall clock
+    of the synchronous calls, median of 3 after a warm-up."""
+    put = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50)
+    n = 1 << 20
+
+    def med(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2] * 1e3, out
+
+    qr_ms, q = med(lambda: mcdx.lsm(*put, n, SEED, mcdx.PREC_FP32))
+    pipe_ms, (d, w, co, has) = med(lambda: mcdx.lsm_learned(*put, n, SEED, mcdx.PREC_FP32), reps=1)
+    serve_ms, sv = med(lambda: mcdx.lsm_learned_price(*put, n, SEED + 1, mcdx.PREC_FP32, w, 1024,
+                                                      d["table_lo"], d["table_hi"], co, has))
+    crr = mcd.crr_binomial_american(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 10_000, mcd.OptionType.Put)
+    qp, qse = q[0], q[1]
+    lp, lse, _ = d["learned"]
+    sp, sse, _ = sv["learned"]
+    store_bytes = n * 50 * 4 + n * 49 * 4
+    return {
+        "paths": n, "exercise_dates": 50, "crr_10000": crr,
+        "qr_lsm": {"price": qp, "se": qse, "ms_wall": qr_ms, "paths_per_s": n / (qr_ms / 1e3),
+                   "dev_from_crr_in_se": (qp - crr) / qse,
+                   "store_bytes_algorithmic": store_bytes},
+        "learned_same_paths": {"price": lp, "se": lse, "diff_vs_qr_in_se": (lp - qp) / qse,
+                               "qr_forward_replay": d["qr_forward"][0],
+                               "disagreement_rate": d["disagreements"] / max(1, d["itm_decisions"]),
+                               "fallbacks": d["fallbacks"], "boundary_gap_s_over_k": d["boundary_gap"],
+                               "final_loss": d["final_loss"], "pipeline_ms_wall": pipe_ms},
+        "learned_serving_fresh_paths": {"price": sp, "se": sse, "ms_wall": serve_ms,
+                                        "paths_per_s": n / (serve_ms / 1e3),
+                                        "dev_from_crr_in_se": (sp - crr) / sse,
+                                        "diff_vs_qr_lsm_in_se": (sp - qp) / qse,
+                                        "qr_policy_same_paths": sv["qr_forward"][0]},
+        "serving_speedup_vs_qr_lsm": qr_ms / serve_ms,
+    }
 
 
 def main():
@@ -213,6 +267,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary configs")
+    ap.add_argument("--only", choices=["c5"], help="run one secondary config alone (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -224,6 +279,10 @@ def main():
 
     torch.cuda.init()
     L = mcdx.lib()
+    if args.only == "c5":
+        import mcd
+        print(json.dumps({"c5_american_put_lsm": c5_lsm(mcdx, mcd)}), flush=True)
+        return 0
     cs = contracts(mcdx, args.paths)
     names = list(cs)
     # A dedicated stream (default priority): its handle is non-null, so the engine enqueues

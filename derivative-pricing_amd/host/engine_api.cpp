@@ -7,6 +7,7 @@
 #include <limits>
 
 #include "mcd/core/sobol.hpp"
+#include "mcd/engine.hpp"
 #include "mcd/greeks/finite_difference.hpp"
 #include "mcd/greeks/likelihood_ratio.hpp"
 #include "mcd/greeks/pathwise.hpp"

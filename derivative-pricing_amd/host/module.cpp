@@ -4,6 +4,8 @@
 // this module does too for valid inputs; a device error (no GPU, allocation failure) raises
 // RuntimeError instead of returning NaN.
 #include <pybind11/pybind11.h>
+
+#include "mcd/engine.hpp"
 #include <pybind11/stl.h>
 
 #include <string>
@@ -26,10 +28,6 @@ namespace py = pybind11;
 using namespace mcd;
 namespace pricers = mcd::pricers;
 namespace greeks = mcd::greeks;
-
-namespace mcd {
-int last_engine_status() noexcept;
-}
 
 namespace {
 
