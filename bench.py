@@ -233,7 +233,10 @@ def c5_lsm(mcdx, mcd):
     qr_ms, q = med(lambda: mcdx.lsm(*put, n, SEED, mcdx.PREC_FP32))
     pipe_ms, (d, w, co, has) = med(lambda: mcdx.lsm_learned(*put, n, SEED, mcdx.PREC_FP32), reps=1)
     serve_ms, sv = med(lambda: mcdx.lsm_learned_price(*put, n, SEED + 1, mcdx.PREC_FP32, w, 1024,
-                                                      d["table_lo"], d["table_hi"], co, has))
+                                                      d["table_lo"], d["table_hi"], co, has,
+                                                      judge=False))
+    judged = mcdx.lsm_learned_price(*put, n, SEED + 1, mcdx.PREC_FP32, w, 1024, d["table_lo"],
+                                    d["table_hi"], co, has)
     crr = mcd.crr_binomial_american(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 10_000, mcd.OptionType.Put)
     qp, qse = q[0], q[1]
     lp, lse, _ = d["learned"]
@@ -253,7 +256,10 @@ def c5_lsm(mcdx, mcd):
                                         "paths_per_s": n / (serve_ms / 1e3),
                                         "dev_from_crr_in_se": (sp - crr) / sse,
                                         "diff_vs_qr_lsm_in_se": (sp - qp) / qse,
-                                        "qr_policy_same_paths": sv["qr_forward"][0]},
+                                        "qr_policy_same_paths": judged["qr_forward"][0],
+                                        "disagreement_rate": judged["disagreements"] /
+                                        max(1, judged["itm_decisions"]),
+                                        "fallbacks": sv["fallbacks"]},
         "serving_speedup_vs_qr_lsm": qr_ms / serve_ms,
     }
 

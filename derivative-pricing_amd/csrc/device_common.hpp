@@ -121,6 +121,16 @@ __host__ __device__ __forceinline__ double fast_rcp_f64(double x) {
     return __builtin_fma(y, e, y);
 }
 
+__host__ __device__ __forceinline__ double fast_rsqrt_f64(double x) {
+    // v_rsq_f64 seed refined by two Newton steps y <- y (3 - x y^2) / 2: relative error near
+    // 2^-52 for positive normal x.
+    double y = MCDX_DEV_OR_HOST(__builtin_amdgcn_rsq(x), 1.0 / __builtin_sqrt(x));
+    const double hx = 0.5 * x;
+    y = y * __builtin_fma(-hx * y, y, 1.5);
+    y = y * __builtin_fma(-hx * y, y, 1.5);
+    return y;
+}
+
 __host__ __device__ __forceinline__ bool is_tail(uint32_t bits) {
     return bits < kTailLo || bits > kTailHi;
 }

@@ -50,8 +50,10 @@ __host__ __device__ __forceinline__ void qr_add_row(QrState& st, double a0, doub
         const double aj = a[j];
         if (aj == 0.0) continue;
         const double rjj = st.r[ridx(j, j)];
-        const double h = sqrt(rjj * rjj + aj * aj);
-        const double c = rjj / h, s = aj / h;
+        // c = rjj / h, s = aj / h with h = sqrt(rjj^2 + aj^2): one reciprocal square root
+        // (hardware seed + Newton) instead of a square root and two divisions.
+        const double inv = fast_rsqrt_f64(__builtin_fma(rjj, rjj, aj * aj));
+        const double c = rjj * inv, s = aj * inv;
 #pragma unroll
         for (int k = j; k < kLsmBasis; ++k) {
             const double rk = st.r[ridx(j, k)];
@@ -96,6 +98,8 @@ struct LsmArgs {
     float mu2, sig2;       // FP32: log2-space step
     uint64_t seed, n_paths;
     int32_t m;             // exercise dates (monitoring points)
+    uint32_t blocks;       // backward-induction layout: blocks and paths per lane
+    uint32_t per_lane;
     uint8_t type;
     void* store;           // [date 1..M][path], float or double
     double* cf;            // [path]
@@ -175,51 +179,66 @@ __device__ __forceinline__ void qr_shfl_down(const QrState& s, QrState& o, int o
     o.rows = __shfl_down(s.rows, off, 64);
 }
 
-// Apply the exercise decision of date `date + 1` (if it has coefficients), discount, and
-// fold this date's ITM rows. With `regress` false (the final pass, date 0) only the
-// decision and the discount run. Grid: one lane per path, blocks of 256, every block
-// writes its QR partial.
-template <bool FP64>
-__global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress) {
-    __shared__ QrState lds[4];
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    QrState st = {};
-    if (p < a.n_paths) {
-        double cf = a.cf[p];
-        const int later = date + 1;
-        if (later <= a.m - 1 && a.has[later]) { // lsm.cpp:110-122 at the later date
-            const double s1 = store_get<FP64>(a, later, p);
-            const double ex = intrinsic_v(s1, a.strike, a.type);
-            if (ex > 0.0) {
-                double b[4];
-                laguerre4(s1 / a.strike, b);
-                const double* beta = a.coeffs + 4 * later;
-                double cont = 0.0;
+// ---- Backward induction -------------------------------------------------------------------
+// Path layout: B = min(512, ceil(n / 2048)) blocks of 256 lanes; lane t of block b owns
+// paths ((j * B + b) * 256 + t) for j < P = ceil(n / (256 B)), coalesced per j. Each date
+// runs one kernel (decision at the later date, discount, fold the ITM rows into the lane's
+// QR state, block tree) and one single-block solve kernel (merge the B partials along a
+// fixed tree, back-substitute on the device). The layout depends on n only, so the
+// coefficients are bitwise reproducible.
+constexpr uint32_t kLsmMaxBlocks = 512;
+
+__host__ __device__ __forceinline__ uint64_t lane_path(uint32_t j, uint32_t block,
+                                                       uint32_t blocks, uint32_t t) {
+    return ((uint64_t)j * blocks + block) * kLsmBlock + t;
+}
+
+// Exercise decision at a later date with coefficients beta (lsm.cpp:110-122).
+__device__ __forceinline__ double decide(const LsmArgs& a, double s1, double cf,
+                                         const double* beta) {
+    const double ex = intrinsic_v(s1, a.strike, a.type);
+    if (ex > 0.0) {
+        double b[4];
+        laguerre4(s1 / a.strike, b);
+        double cont = 0.0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
-                if (ex > cont) cf = ex;
-            }
-        }
-        cf *= a.disc; // lsm.cpp:84-86 (and :125-127 after the last date)
-        a.cf[p] = cf;
-        if (regress) {
-            const double s0 = store_get<FP64>(a, date, p);
-            const bool itm = intrinsic_v(s0, a.strike, a.type) > 0.0;
-            if (itm) { // lsm.cpp:88-100
-                double b[4];
-                laguerre4(s0 / a.strike, b);
-                qr_add_row(st, b[0], b[1], b[2], b[3], cf);
-                st.rows = 1;
-            }
-            if (a.cap_y) {
-                const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
-                a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
-                a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
-                a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
-            }
-        }
+        for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
+        if (ex > cont) return ex;
     }
-    if (!regress) return; // uniform across the grid
+    return cf;
+}
+
+// Fold one path's row at `date` (lsm.cpp:88-100) and capture the training sample.
+__device__ __forceinline__ void fold(const LsmArgs& a, QrState& st, int date, uint64_t p,
+                                     double s0, double cf) {
+    const bool itm = intrinsic_v(s0, a.strike, a.type) > 0.0;
+    if (itm) {
+        double b[4];
+        laguerre4(s0 / a.strike, b);
+        qr_add_row(st, b[0], b[1], b[2], b[3], cf);
+        st.rows += 1;
+    }
+    if (a.cap_y) {
+        const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
+        a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
+        a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
+        a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
+    }
+}
+
+__device__ __forceinline__ QrState bcast_qr(const QrState& s) {
+    QrState o;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) o.r[i] = bcast_f64(s.r[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.z[i] = bcast_f64(s.z[i]);
+    o.rows = __builtin_amdgcn_readfirstlane(s.rows);
+    return o;
+}
+
+// Block total of the lanes' QR states along a fixed tree, returned to every lane (no
+// thread-0-only region: the drivers run it inside loops with barriers).
+__device__ __forceinline__ QrState block_tree_all(QrState st, QrState* lds4) {
     const int lane = threadIdx.x & 63;
 #pragma unroll 1
     for (int off = 1; off < 64; off <<= 1) {
@@ -227,49 +246,80 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
         qr_shfl_down(st, o, off);
         if ((lane & (2 * off - 1)) == 0) st = qr_merge(st, o);
     }
-    if (lane == 0) lds[threadIdx.x >> 6] = st;
+    const QrState w = bcast_qr(st);
+    __syncthreads(); // lds4 free from any previous use
+    lds4[threadIdx.x >> 6] = w;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        QrState t = qr_merge(qr_merge(lds[0], lds[1]), qr_merge(lds[2], lds[3]));
-        a.qr_partials[blockIdx.x] = t;
+    return qr_merge(qr_merge(lds4[0], lds4[1]), qr_merge(lds4[2], lds4[3]));
+}
+
+// Merge B block partials (256 groups by static partition, then a pairwise tree) and
+// back-substitute (linalg.cpp:58-69) when at least four ITM rows were folded
+// (lsm.cpp:102-104). Every lane of the 256-lane block receives beta and has.
+__device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t nb,
+                                                QrState* lds, double* beta, int* has) {
+    const uint32_t g = threadIdx.x;
+    const uint32_t q = nb / 256, rem = nb % 256;
+    const uint32_t b = g * q + (g < rem ? g : rem);
+    const uint32_t e = b + q + (g < rem ? 1 : 0);
+    QrState acc = {};
+    for (uint32_t i = b; i < e; ++i) acc = qr_merge(acc, parts[i]);
+    __syncthreads();
+    lds[g] = acc;
+    __syncthreads();
+    for (uint32_t s = 1; s < 256; s <<= 1) {
+        QrState m;
+        const bool act = (g & (2 * s - 1)) == 0;
+        if (act) m = qr_merge(lds[g], lds[g + s]);
+        __syncthreads();
+        if (act) lds[g] = m;
+        __syncthreads();
+    }
+    const QrState t = lds[0];
+    *has = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) beta[i] = 0.0;
+    for (int i = kLsmBasis - 1; i >= 0; --i) {
+        double sum = t.z[i];
+        for (int j = i + 1; j < kLsmBasis; ++j) sum -= t.r[ridx(i, j)] * beta[j];
+        const double diag = t.r[ridx(i, i)];
+        beta[i] = diag != 0.0 ? sum / diag : 0.0;
     }
 }
 
-// One block: merge the block partials along a fixed tree (1024 groups by the static
-// partition, then pairwise), then back-substitute (linalg.cpp:58-69) when at least four
-// ITM rows were folded (lsm.cpp:102-104).
-__global__ __launch_bounds__(1024) void lsm_solve_kernel(LsmArgs a, int date, uint64_t n_blocks) {
+// Multi-kernel driver, one date: apply the later date's decision, discount, fold.
+template <bool FP64>
+__global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress) {
+    __shared__ QrState lds4[4];
+    const int later = date + 1;
+    const bool use = later <= a.m - 1 && a.has[later];
+    double beta[4] = {0.0, 0.0, 0.0, 0.0};
+    if (use)
+        for (int j = 0; j < 4; ++j) beta[j] = a.coeffs[4 * later + j];
+    QrState st = {};
+    for (uint32_t j = 0; j < a.per_lane; ++j) {
+        const uint64_t p = lane_path(j, blockIdx.x, a.blocks, threadIdx.x);
+        if (p >= a.n_paths) break;
+        double cf = a.cf[p];
+        if (use) cf = decide(a, store_get<FP64>(a, later, p), cf, beta);
+        cf *= a.disc; // lsm.cpp:84-86 (and :125-127 after the last date)
+        a.cf[p] = cf;
+        if (regress) fold(a, st, date, p, store_get<FP64>(a, date, p), cf);
+    }
+    if (!regress) return; // uniform across the grid
+    const QrState t = block_tree_all(st, lds4);
+    if (threadIdx.x == 0) a.qr_partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kLsmBlock) void lsm_solve_kernel(LsmArgs a, int date) {
     __shared__ QrState lds[256];
-    const uint32_t g = threadIdx.x;
-    // 256 groups (QrState is large; 256 x 116 B fits LDS comfortably)
-    QrState acc = {};
-    if (g < 256) {
-        const uint64_t q = n_blocks / 256, rem = n_blocks % 256;
-        const uint64_t b = g * q + (g < rem ? g : rem);
-        const uint64_t e = b + q + (g < rem ? 1 : 0);
-        for (uint64_t i = b; i < e; ++i) acc = qr_merge(acc, a.qr_partials[i]);
-        lds[g] = acc;
-    }
-    __syncthreads();
-    for (uint32_t s = 1; s < 256; s <<= 1) {
-        if (g < 256 && (g & (2 * s - 1)) == 0) lds[g] = qr_merge(lds[g], lds[g + s]);
-        __syncthreads();
-    }
-    if (g == 0) {
-        const QrState t = lds[0];
-        if (t.rows < (uint32_t)kLsmBasis) {
-            a.has[date] = 0;
-            return;
-        }
-        double beta[4] = {0, 0, 0, 0};
-        for (int i = kLsmBasis - 1; i >= 0; --i) {
-            double sum = t.z[i];
-            for (int j = i + 1; j < kLsmBasis; ++j) sum -= t.r[ridx(i, j)] * beta[j];
-            const double diag = t.r[ridx(i, i)];
-            beta[i] = diag != 0.0 ? sum / diag : 0.0;
-        }
-        for (int j = 0; j < 4; ++j) a.coeffs[4 * date + j] = beta[j];
-        a.has[date] = 1;
+    double beta[4];
+    int has;
+    merge_solve_all(a.qr_partials, a.blocks, lds, beta, &has);
+    if (threadIdx.x == 0) {
+        a.has[date] = (uint8_t)has;
+        if (has)
+            for (int j = 0; j < 4; ++j) a.coeffs[4 * date + j] = beta[j];
     }
 }
 
@@ -309,10 +359,34 @@ struct PolicyArgs {
     const float* table;           // [date 1..M-1][grid], kTargetScale * continuation / K
     uint32_t grid;
     double xlo, xhi, inv_h;       // S/K range of the table and 1/spacing
+    double inv_strike;            // 1/K for the table lookup
+    double cont_scale;            // K / kTargetScale: table units to cash
     Welford* part_nn;             // [tiles + 1]
     Welford* part_qr;             // [tiles + 1]
-    unsigned long long* counters; // [3] ITM decisions, disagreements, fallbacks
+    uint32_t* tile_counts;        // [3][tiles] ITM decisions, disagreements, fallbacks
+    int judge;                    // 0: the QR policy only decides outside the table range
 };
+
+// counters[c] = sum over tiles of tile_counts[c][tile], c < 3 (one block).
+__global__ __launch_bounds__(256) void lsm_count_kernel(const uint32_t* tile_counts,
+                                                        uint64_t tiles,
+                                                        unsigned long long* counters) {
+    __shared__ unsigned long long part[3][4];
+    unsigned long long acc[3] = {0, 0, 0};
+    for (uint64_t i = threadIdx.x; i < tiles; i += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += tile_counts[(uint64_t)c * tiles + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+        if ((threadIdx.x & 63) == 0) part[c][threadIdx.x >> 6] = acc[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        counters[threadIdx.x] = part[threadIdx.x][0] + part[threadIdx.x][1] +
+                                part[threadIdx.x][2] + part[threadIdx.x][3];
+}
 
 // Grid features for the table: (4 (x_g - 1), 2 d/M - 1) for d = 1..M-1, g < grid.
 __global__ __launch_bounds__(256) void lsm_grid_kernel(float* f1, float* f2, int m, uint32_t grid,
@@ -325,129 +399,182 @@ __global__ __launch_bounds__(256) void lsm_grid_kernel(float* f1, float* f2, int
     f2[i] = (float)(2.0 * (double)date / (double)m - 1.0);
 }
 
-__global__ void lsm_zero_counters_kernel(unsigned long long* c) {
-    if (threadIdx.x < 3) c[threadIdx.x] = 0ull;
-}
-
-// One exercise-policy state: stopped flag, the exercise value and the date it was taken.
-struct Stop {
-    bool done;
-    int date;
-    double value;
+// Both policies' state along one path: stopped flags, exercise dates and values, and the
+// path's counts of joint ITM decisions, disagreements and table-range fallbacks.
+struct PolicyState {
+    bool n_done, q_done;
+    int n_date, q_date;
+    double n_val, q_val;
+    uint32_t dec, dis, fb;
 };
 
-// Per tile of 1024 paths (canonical layout), regenerate each path exactly as the store
-// kernel does and take the first date at which each policy exercises; the cash flow is then
-// discounted by one multiplication per date, the order in which backward induction applies
-// them, so the QR policy reproduces the backward-induction price bit for bit.
+__device__ __forceinline__ void policy_visit(const LsmArgs& a, const PolicyArgs& q,
+                                             PolicyState& ps, int date, double s) {
+    const double ex = intrinsic_v(s, a.strike, a.type);
+    if (date == a.m) { // lsm.cpp:76-81
+        if (!ps.n_done) { ps.n_done = true; ps.n_date = date; ps.n_val = ex; }
+        if (!ps.q_done) { ps.q_done = true; ps.q_date = date; ps.q_val = ex; }
+        return;
+    }
+    if (!(ex > 0.0)) return;
+    const double x = s * q.inv_strike;
+    const bool in_range = x >= q.xlo && x <= q.xhi;
+    const bool both = !ps.n_done && !ps.q_done;
+    bool dq = false;
+    if ((!ps.q_done || both || !in_range) && a.has[date]) { // lsm.cpp:110-122
+        double b0, b1, b2, b3;
+        {
+            double b[4];
+            laguerre4(s / a.strike, b);
+            b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+        }
+        const double* beta = a.coeffs + 4 * date;
+        double cont = 0.0;
+        cont += beta[0] * b0;
+        cont += beta[1] * b1;
+        cont += beta[2] * b2;
+        cont += beta[3] * b3;
+        dq = ex > cont;
+    }
+    bool dn = false;
+    if (!ps.n_done) {
+        if (in_range) {
+            const float u = (float)((x - q.xlo) * q.inv_h);
+            uint32_t g = (uint32_t)u;
+            if (g > q.grid - 2) g = q.grid - 2;
+            const float* row = q.table + (uint64_t)(date - 1) * q.grid;
+            const float lo = row[g], hi = row[g + 1];
+            const float c = __builtin_fmaf(u - (float)g, hi - lo, lo);
+            dn = ex > (double)c * q.cont_scale;
+        } else {
+            dn = dq;
+            ps.fb += 1;
+        }
+    }
+    if (both) {
+        ps.dec += 1;
+        ps.dis += (dn != dq) ? 1u : 0u;
+    }
+    if (!ps.q_done && dq) { ps.q_done = true; ps.q_date = date; ps.q_val = ex; }
+    if (!ps.n_done && dn) { ps.n_done = true; ps.n_date = date; ps.n_val = ex; }
+}
+
+// One path per thread; a block of 1024 threads is one canonical tile (thread t computes path
+// tile * 1024 + t), and the tile's Welford partial is rebuilt from LDS in the canonical
+// order (lane l < 256 adds paths l, l+256, l+512, l+768 in turn, then the 256-lane tree), so
+// it equals the partial of the 256-lane path kernels. Each path is regenerated exactly as the
+// store kernel writes it and stopped at each policy's first exercise date; the cash flow is
+// then discounted by one multiplication per date, the order backward induction applies them
+// in, so the QR policy reproduces the backward-induction price bit for bit.
 template <bool FP64>
-__global__ __launch_bounds__(kTileLanes) void lsm_policy_kernel(LsmArgs a, PolicyArgs q) {
+__global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, PolicyArgs q) {
+    __shared__ double sh_n[kTilePaths], sh_q[kTilePaths];
     __shared__ Welford lds_n[4], lds_q[4];
+    __shared__ uint32_t sh_cnt[3][16];
     const uint64_t tile = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t p = tile * kTilePaths + t;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    Welford wn = {0, 0.0, 0.0}, wq = {0, 0.0, 0.0};
-    unsigned long long c_dec = 0, c_dis = 0, c_fb = 0;
-#pragma unroll 1
-    for (int k = 0; k < kPathsPerLane; ++k) {
-        const uint64_t p = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
-        if (p >= a.n_paths) break;
+    // Without the judge the QR policy starts stopped: it then only decides for the network
+    // outside the table range, and its own price is not formed.
+    PolicyState ps = {false, q.judge == 0, 0, 0, 0.0, 0.0, 0u, 0u, 0u};
+    if (p < a.n_paths) {
         const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
-        Stop sn = {false, 0, 0.0}, sq = {false, 0, 0.0};
-        auto visit = [&](int date, double s) {
-            const double ex = intrinsic_v(s, a.strike, a.type);
-            if (date == a.m) { // lsm.cpp:76-81
-                if (!sn.done) sn = Stop{true, date, ex};
-                if (!sq.done) sq = Stop{true, date, ex};
-                return;
-            }
-            if (!(ex > 0.0)) return;
-            bool dq = false;
-            if (a.has[date]) { // lsm.cpp:110-122
-                double b[4];
-                laguerre4(s / a.strike, b);
-                const double* beta = a.coeffs + 4 * date;
-                double cont = 0.0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
-                dq = ex > cont;
-            }
-            bool dn;
-            const double x = s / a.strike;
-            if (x >= q.xlo && x <= q.xhi) {
-                const double u = (x - q.xlo) * q.inv_h;
-                uint32_t g = (uint32_t)u;
-                if (g > q.grid - 2) g = q.grid - 2;
-                const float* row = q.table + (uint64_t)(date - 1) * q.grid;
-                const double fr = u - (double)g;
-                const double c = (double)row[g] + fr * ((double)row[g + 1] - (double)row[g]);
-                dn = ex > c * (a.strike / kTargetScale);
-            } else {
-                dn = dq;
-                if (!sn.done) ++c_fb;
-            }
-            if (!sn.done && !sq.done) {
-                ++c_dec;
-                c_dis += (dn != dq) ? 1u : 0u;
-            }
-            if (!sq.done && dq) sq = Stop{true, date, ex};
-            if (!sn.done && dn) sn = Stop{true, date, ex};
-        };
         if constexpr (FP64) {
             double sp = a.spot;
-            for (int step = 0; step < a.m && !(sn.done && sq.done); ++step) {
+            for (int step = 0; step < a.m && !(ps.n_done && ps.q_done); ++step) {
                 const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
                 sp = sp * exp(a.drift + a.vsd * z);
-                visit(step + 1, sp);
+                policy_visit(a, q, ps, step + 1, sp);
             }
         } else {
             float x = 0.f;
             const float s0 = (float)a.spot;
             int step = 0;
-            constexpr int B = 8;
-            for (; step + B <= a.m && !(sn.done && sq.done); step += B) {
+            constexpr int B = 4;
+            for (; step + B <= a.m && !(ps.n_done && ps.q_done); step += B) {
                 float z[B];
                 normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     x = x + __builtin_fmaf(a.sig2, z[j], a.mu2);
-                    visit(step + j + 1, (double)(s0 * fast_exp2(x)));
+                    policy_visit(a, q, ps, step + j + 1, (double)(s0 * fast_exp2(x)));
                 }
             }
-            for (; step < a.m && !(sn.done && sq.done); ++step) {
+            for (; step < a.m && !(ps.n_done && ps.q_done); ++step) {
                 const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
                 x = x + __builtin_fmaf(a.sig2, z, a.mu2);
-                visit(step + 1, (double)(s0 * fast_exp2(x)));
+                policy_visit(a, q, ps, step + 1, (double)(s0 * fast_exp2(x)));
             }
         }
-        double cn = sn.value, cq = sq.value;
-        for (int i = 0; i < sn.date; ++i) cn *= a.disc;
-        for (int i = 0; i < sq.date; ++i) cq *= a.disc;
-        const double rk = 1.0 / (double)(k + 1);
-        wn.n += 1;
-        double d = cn - wn.mean;
-        wn.mean += d * rk;
-        wn.m2 += d * (cn - wn.mean);
-        wq.n += 1;
-        d = cq - wq.mean;
-        wq.mean += d * rk;
-        wq.m2 += d * (cq - wq.mean);
     }
-    const Welford tn = block_merge_256(wn, lds_n);
-    const Welford tq = block_merge_256(wq, lds_q);
-    if (threadIdx.x == 0) {
-        q.part_nn[tile] = tn;
-        q.part_qr[tile] = tq;
-    }
+    double cn = ps.n_val, cq = ps.q_val;
+    for (int i = 0; i < ps.n_date; ++i) cn *= a.disc;
+    for (int i = 0; i < ps.q_date; ++i) cq *= a.disc;
+    sh_n[t] = cn;
+    sh_q[t] = cq;
+    uint32_t c_dec = ps.dec, c_dis = ps.dis, c_fb = ps.fb;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         c_dec += __shfl_xor(c_dec, off, 64);
         c_dis += __shfl_xor(c_dis, off, 64);
         c_fb += __shfl_xor(c_fb, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(q.counters + 0, c_dec);
-        atomicAdd(q.counters + 1, c_dis);
-        atomicAdd(q.counters + 2, c_fb);
+    if ((t & 63) == 0) {
+        sh_cnt[0][t >> 6] = c_dec;
+        sh_cnt[1][t >> 6] = c_dis;
+        sh_cnt[2][t >> 6] = c_fb;
+    }
+    __syncthreads();
+    if (t < 3) { // per-tile counts, summed by lsm_count_kernel (no same-address atomics)
+        uint32_t sum = 0;
+        for (int w = 0; w < 16; ++w) sum += sh_cnt[t][w];
+        q.tile_counts[(uint64_t)t * gridDim.x + tile] = sum;
+    }
+    Welford wn = {0, 0.0, 0.0}, wq = {0, 0.0, 0.0};
+    if (t < (uint32_t)kTileLanes) {
+#pragma unroll
+        for (int k = 0; k < kPathsPerLane; ++k) {
+            const uint32_t i = (uint32_t)(k * kTileLanes) + t;
+            if (tile * kTilePaths + i >= a.n_paths) break;
+            const double rk = 1.0 / (double)(k + 1);
+            const double xn = sh_n[i], xq = sh_q[i];
+            wn.n += 1;
+            double d = xn - wn.mean;
+            wn.mean += d * rk;
+            wn.m2 += d * (xn - wn.mean);
+            wq.n += 1;
+            d = xq - wq.mean;
+            wq.mean += d * rk;
+            wq.m2 += d * (xq - wq.mean);
+        }
+    }
+    // the 256-lane tree of block_merge_256, run by every wave (waves 4..15 hold zeros)
+    const int lane = t & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Welford on, oq;
+        on.n = __shfl_down(wn.n, off, 64);
+        on.mean = __shfl_down(wn.mean, off, 64);
+        on.m2 = __shfl_down(wn.m2, off, 64);
+        oq.n = __shfl_down(wq.n, off, 64);
+        oq.mean = __shfl_down(wq.mean, off, 64);
+        oq.m2 = __shfl_down(wq.m2, off, 64);
+        if ((lane & (2 * off - 1)) == 0) {
+            wn = welford_merge(wn, on);
+            wq = welford_merge(wq, oq);
+        }
+    }
+    if (lane == 0 && t < (uint32_t)kTileLanes) {
+        lds_n[t >> 6] = wn;
+        lds_q[t >> 6] = wq;
+    }
+    __syncthreads();
+    if (t == 0) {
+        q.part_nn[tile] = welford_merge(welford_merge(lds_n[0], lds_n[1]),
+                                        welford_merge(lds_n[2], lds_n[3]));
+        q.part_qr[tile] = welford_merge(welford_merge(lds_q[0], lds_q[1]),
+                                        welford_merge(lds_q[2], lds_q[3]));
     }
 }
 
@@ -485,6 +612,10 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.type = type;
     const uint64_t blocks = (n + kLsmBlock - 1) / kLsmBlock;
     const uint64_t tiles = tiles_for(n);
+    uint64_t bb = (n + 2047) / 2048;
+    if (bb > kLsmMaxBlocks) bb = kLsmMaxBlocks;
+    a.blocks = (uint32_t)bb;
+    a.per_lane = (uint32_t)((n + bb * kLsmBlock - 1) / (bb * kLsmBlock));
     const size_t store_bytes = with_store ? (size_t)m * n * (fp64 ? 8 : 4) : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -494,7 +625,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     };
     const size_t o_store = take(store_bytes), o_cf = take(with_store ? n * 8 : 0),
                  o_coeffs = take((size_t)(m + 1) * 4 * 8), o_has = take((size_t)m + 1),
-                 o_qr = take(with_store ? blocks * sizeof(QrState) : 0),
+                 o_qr = take(with_store ? 2 * (size_t)a.blocks * sizeof(QrState) : 0),
                  o_part = take((tiles + 1) * sizeof(Welford)), o_extra = take(extra_bytes);
     int st = ensure_dev(ctx, off);
     if (st) return st;
@@ -525,11 +656,11 @@ hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
 // Backward induction with regression (fit) or with a given policy already in coeffs/has.
 template <bool FP64>
 hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
-    const unsigned g = (unsigned)r.blocks;
+    const unsigned g = r.a.blocks;
     for (int date = r.a.m - 1; date >= 1; --date) {
-        hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, date, fit ? 1 : 0);
-        if (fit)
-            hipLaunchKernelGGL(lsm_solve_kernel, dim3(1), dim3(1024), 0, s, r.a, date, r.blocks);
+        hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, date,
+                           fit ? 1 : 0);
+        if (fit) hipLaunchKernelGGL(lsm_solve_kernel, dim3(1), dim3(kLsmBlock), 0, s, r.a, date);
     }
     hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, 0, 0);
     hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a);
@@ -647,10 +778,12 @@ struct PolicyBufs {
     float *table, *gx, *gt;
     Welford *pn, *pq;
     unsigned long long* ctr;
+    uint32_t* tile_counts;
 };
 
 size_t policy_bytes(int32_t m, uint32_t grid, uint64_t tiles) {
-    return 3 * align256((size_t)(m - 1) * grid * 4) + 2 * align256((tiles + 1) * sizeof(Welford)) + 256;
+    return 3 * align256((size_t)(m - 1) * grid * 4) + 2 * align256((tiles + 1) * sizeof(Welford)) +
+           256 + align256(3 * tiles * 4);
 }
 
 PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
@@ -662,13 +795,14 @@ PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
     b.pn = (Welford*)(p + 3 * tb);
     b.pq = (Welford*)(p + 3 * tb + wb);
     b.ctr = (unsigned long long*)(p + 3 * tb + 2 * wb);
+    b.tile_counts = (uint32_t*)(p + 3 * tb + 2 * wb + 256);
     return b;
 }
 
 // Table (MFMA evaluation of the network on the grid), then the streaming policy kernel and
 // the two fixed-order merges. Results land in pn[tiles], pq[tiles] and ctr[0..2].
 int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_t grid,
-                   double lo, double hi, bool fp64, hipStream_t s) {
+                   double lo, double hi, bool fp64, bool judge, hipStream_t s) {
     const int m = r.a.m;
     const uint64_t pts = (uint64_t)(m - 1) * grid;
     const double h = (hi - lo) / (double)(grid - 1);
@@ -676,13 +810,17 @@ int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_
                        b.gt, m, grid, lo, h);
     int st = mlp_eval_device(w, b.gx, b.gt, pts, b.table, s);
     if (st) return st;
-    hipLaunchKernelGGL(lsm_zero_counters_kernel, dim3(1), dim3(64), 0, s, b.ctr);
-    PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, b.pn, b.pq, b.ctr};
+    PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, 1.0 / r.a.strike, r.a.strike / kTargetScale,
+                 b.pn, b.pq, b.tile_counts, judge ? 1 : 0};
     if (fp64)
-        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a, q);
+        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTilePaths), 0, s, r.a, q);
     else
-        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a, q);
+        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTilePaths), 0, s, r.a, q);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(lsm_count_kernel, dim3(1), dim3(256), 0, s, b.tile_counts, r.tiles, b.ctr);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = launch_final_merge(b.pn, nullptr, r.tiles, 1, b.pn + r.tiles, s);
     if (e == hipSuccess) e = launch_final_merge(b.pq, nullptr, r.tiles, 1, b.pq + r.tiles, s);
     return status_of(e);
@@ -765,8 +903,6 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
     r.a.cap_f2 = (float*)(ex + cap_bytes);
     r.a.cap_y = (float*)(ex + 2 * cap_bytes);
     float* w = (float*)(ex + 3 * cap_bytes);
-    float* am = w + kMlpParams;
-    float* av = am + kMlpParams;
     float* losses = (float*)(ex + 3 * cap_bytes + align256(3 * (size_t)kMlpParams * 4));
     const PolicyBufs pb = carve_policy(ex + 3 * cap_bytes + mlp_bytes, m, grid, tiles);
     const size_t tbl = (size_t)(m - 1) * grid;
@@ -781,15 +917,15 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
     if (e == hipSuccess) e = fp64 ? enqueue_backward<true>(r, true, s) : enqueue_backward<false>(r, true, s);
     if (e != hipSuccess) return status_of(e);
     // Train on every captured (path, date) sample; OTM samples carry NaN targets.
-    if ((st = mlp_init_device(w, am, av, o->init_seed, 0.f, s))) return st;
+    if ((st = mlp_init_device(w, o->init_seed, 0.f, s))) return st;
     const float decay = (float)pow((double)o->final_learning_rate / (double)o->learning_rate,
                                    1.0 / (double)o->steps);
-    if ((st = mlp_train_device(w, am, av, r.a.cap_f1, r.a.cap_f2, r.a.cap_y, (uint64_t)(m - 1) * n,
+    if ((st = mlp_train_device(w, r.a.cap_f1, r.a.cap_f2, r.a.cap_y, (uint64_t)(m - 1) * n,
                                o->steps, o->learning_rate, decay, o->sample_seed, losses, nullptr, s)))
         return st;
     double lo, hi;
     table_range(spot, strike, rate, carry, vol, time, type, &lo, &hi);
-    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, fp64, s))) return st;
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, fp64, true, s))) return st;
     char* h = (char*)ctx->pinned;
     PolicyHost* ph = (PolicyHost*)h;
     e = hipMemcpyAsync(&ph->qr_lsm, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
@@ -834,7 +970,8 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
 int lsm_learned_price(double spot, double strike, double rate, double carry, double vol,
                       double time, uint8_t type, int32_t m, uint64_t n, uint64_t seed,
                       int precision, const float* weights, uint32_t grid, double lo, double hi,
-                      const double* coeffs, const uint8_t* has, mcdx_lsm_learned_result* out) {
+                      const double* coeffs, const uint8_t* has, uint8_t judge,
+                      mcdx_lsm_learned_result* out) {
     if (!out || !weights || !coeffs || !has || m < 2 || grid < 2 || grid > 4096 || !(hi > lo))
         return MCDX_EINVAL;
     ThreadCtx* ctx;
@@ -864,7 +1001,8 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     if (e == hipSuccess) e = hipMemcpyAsync(r.a.has, h + h_has, (size_t)m + 1, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
-    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, s))) return st;
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, judge != 0, s)))
+        return st;
     e = hipStreamSynchronize(s); // uploads consumed before the staging is reused
     PolicyHost* ph = (PolicyHost*)h;
     if (e == hipSuccess) e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
@@ -876,7 +1014,7 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     mcdx_lsm_learned_result res{};
     res.qr_lsm = mcdx_result{NAN, NAN, 0};
     res.learned = with_inception(ph->nn, spot, strike, type);
-    res.qr_forward = with_inception(ph->qr, spot, strike, type);
+    res.qr_forward = judge ? with_inception(ph->qr, spot, strike, type) : mcdx_result{NAN, NAN, 0};
     res.itm_decisions = ph->ctr[0];
     res.disagreements = ph->ctr[1];
     res.fallbacks = ph->ctr[2];
@@ -948,10 +1086,11 @@ int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry
                            uint64_t path_count, uint64_t seed, int precision,
                            const float* weights, uint32_t grid_points, double table_lo,
                            double table_hi, const double* coefficients,
-                           const uint8_t* has_coefficients, mcdx_lsm_learned_result* out) {
+                           const uint8_t* has_coefficients, uint8_t judge,
+                           mcdx_lsm_learned_result* out) {
     return lsm_learned_price(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
                              path_count, seed, precision, weights, grid_points, table_lo,
-                             table_hi, coefficients, has_coefficients, out);
+                             table_hi, coefficients, has_coefficients, judge, out);
 }
 
 } // extern "C"

@@ -86,7 +86,7 @@ struct MlpLds {
     unsigned short w2[kH * kLd];  // W2[o][i]
     unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
     float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
-    float grad[kParams];
+    float grad[2][kParams]; // per wave, summed in wave order by the update
     struct Wave {
         unsigned short h1[kH * kLd];  // H1[s][i]
         unsigned short h1t[kH * kLd]; // [i][s]
@@ -151,9 +151,9 @@ __device__ float forward64(MlpLds& L, MlpLds::Wave& W, float x, float t, f16v ac
     return y;
 }
 
-// Backward of this wave's samples given dL/dy per lane; adds the wave's parameter
-// gradients into L.grad (caller orders the two waves).
-__device__ void backward64(MlpLds& L, MlpLds::Wave& W, float g3, f16v acc[2][2], bool first) {
+// Backward of this wave's samples given dL/dy per lane; writes the wave's parameter
+// gradients into L.grad[wave].
+__device__ void backward64(MlpLds& L, MlpLds::Wave& W, float g3, f16v acc[2][2], int wave) {
     const int lane = threadIdx.x & 63;
     W.g3[lane] = g3;
     wave_sync();
@@ -225,8 +225,8 @@ __device__ void backward64(MlpLds& L, MlpLds::Wave& W, float g3, f16v acc[2][2],
     float gb3 = g3;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) gb3 += __shfl_xor(gb3, off, 64);
-    // accumulate into the shared gradient (wave 0 writes, wave 1 adds after a barrier)
-    auto put = [&](int idx, float v) { L.grad[idx] = first ? v : L.grad[idx] + v; };
+    float* gw = L.grad[wave];
+    auto put = [&](int idx, float v) { gw[idx] = v; };
     if (lane < 32) {
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn) {
@@ -248,12 +248,10 @@ __device__ void backward64(MlpLds& L, MlpLds::Wave& W, float g3, f16v acc[2][2],
 }
 
 struct TrainArgs {
-    float* w;          // [kParams] master weights (in/out)
-    float* adam_m;     // [kParams]
-    float* adam_v;     // [kParams]
-    const float* xs;   // [n] features S/K
-    const float* ts;   // [n] features t/T
-    const float* ys;   // [n] targets cf/K
+    float* w;          // [kParams] weights (in/out)
+    const float* xs;   // [n] first feature
+    const float* ts;   // [n] second feature
+    const float* ys;   // [n] targets (NaN = no sample)
     uint64_t n;
     int32_t steps;
     float lr, lr_decay;
@@ -262,26 +260,68 @@ struct TrainArgs {
     float* grad_out;   // when non-null: one batch (indices 0..127), gradients written, no update
 };
 
-// Single workgroup of 128 lanes (2 waves x 64 samples) running every Adam step in one launch.
-__global__ __launch_bounds__(128) void mlp_train_kernel(TrainArgs a) {
+constexpr int kTrainLanes = 128;
+constexpr int kPer = (kParams + kTrainLanes - 1) / kTrainLanes; // parameters per lane (35)
+
+// Lane-owned parameter i = lane + 128 k into its LDS places (BF16 W2 in both layouts).
+__device__ __forceinline__ void scatter_param(MlpLds& L, int i, float v) {
+    if (i < oB1) L.w1[i] = v;
+    else if (i < oW2) L.b1[i - oB1] = v;
+    else if (i < oB2) {
+        const int j = i - oW2, o = j >> 6, k = j & 63;
+        const unsigned short b = f2bf(v);
+        L.w2[o * kLd + k] = b;
+        L.w2t[k * kLd + o] = b;
+    } else if (i < oW3) L.b2[i - oB2] = v;
+    else if (i < oB3) L.w3[i - oW3] = v;
+    else L.b3 = v;
+}
+
+// Single workgroup of 128 lanes (2 waves x 64 samples) running every Adam step in one
+// launch. Each lane keeps its 35 parameters and their Adam moments in registers; the two
+// waves' backward passes run concurrently into separate gradient arrays.
+static_assert(sizeof(MlpLds) + 2 * kParams * sizeof(float) <= 160 * 1024, "LDS budget");
+
+struct TrainLds {
+    MlpLds net;
+    float adam_m[kParams], adam_v[kParams];
+};
+
+__global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
     extern __shared__ unsigned char smem[];
-    MlpLds& L = *(MlpLds*)smem;
-    const int wave = threadIdx.x >> 6;
+    TrainLds& T = *(TrainLds*)smem;
+    MlpLds& L = T.net;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
     MlpLds::Wave& W = L.wave[wave];
     const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
     float bc1 = 1.f, bc2 = 1.f, lr = a.lr;
+    float wr[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = tid + kTrainLanes * k;
+        wr[k] = i < kParams ? a.w[i] : 0.f;
+        if (i < kParams) {
+            T.adam_m[i] = 0.f;
+            T.adam_v[i] = 0.f;
+        }
+    }
     for (int step = 0; step < a.steps; ++step) {
-        load_weights(a.w, L);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + kTrainLanes * k;
+            if (i < kParams) scatter_param(L, i, wr[k]);
+        }
         __syncthreads();
         uint64_t idx = 0;
         float y = 0.f;
         if (a.grad_out) {
-            idx = threadIdx.x < a.n ? threadIdx.x : 0;
+            idx = (uint64_t)tid < a.n ? (uint64_t)tid : 0;
             y = a.ys[idx];
         } else {
             // Rejection sampling over the valid (non-NaN target) samples; the draw index is
             // the attempt, so every lane's choice is a fixed function of (seed, step, lane).
-            const uint64_t ctr = (uint64_t)step * 128 + threadIdx.x;
+            const uint64_t ctr = (uint64_t)step * kTrainLanes + tid;
             for (uint32_t tr = 0; tr < 8; ++tr) {
                 uint64_t r = philox_bits(a.seed, ctr, tr, 7);
                 if (a.n > 0xFFFFFFFFull) r |= (uint64_t)philox_bits(a.seed, ctr, tr + 8, 7) << 32;
@@ -300,32 +340,38 @@ __global__ __launch_bounds__(128) void mlp_train_kernel(TrainArgs a) {
         float l2 = diff * diff;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) l2 += __shfl_xor(l2, off, 64);
-        if (wave == 0) backward64(L, W, g3, acc, true);
-        __syncthreads();
-        if (wave == 1) backward64(L, W, g3, acc, false);
+        backward64(L, W, g3, acc, wave);
         __syncthreads();
         if (a.grad_out) {
-            for (int i = threadIdx.x; i < kParams; i += blockDim.x) a.grad_out[i] = L.grad[i];
-            if (threadIdx.x == 0 && a.loss_out) a.loss_out[0] = l2;
-            if (threadIdx.x == 64 && a.loss_out) a.loss_out[1] = l2;
+            for (int i = tid; i < kParams; i += kTrainLanes) a.grad_out[i] = L.grad[0][i] + L.grad[1][i];
+            if (tid == 0 && a.loss_out) a.loss_out[0] = l2;
+            if (tid == 64 && a.loss_out) a.loss_out[1] = l2;
             return;
         }
-        if (threadIdx.x == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // wave 0 batch half
-        // Adam (FP32 master weights)
+        if (tid == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // wave 0 batch half
+        // Adam (FP32)
         bc1 *= b1c;
         bc2 *= b2c;
-        for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
-            const float g = L.grad[i];
-            const float m = b1c * a.adam_m[i] + (1.f - b1c) * g;
-            const float v = b2c * a.adam_v[i] + (1.f - b2c) * g * g;
-            a.adam_m[i] = m;
-            a.adam_v[i] = v;
-            const float mh = m / (1.f - bc1), vh = v / (1.f - bc2);
-            a.w[i] -= lr * mh / (sqrtf(vh) + eps);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + kTrainLanes * k;
+            if (i < kParams) {
+                const float g = L.grad[0][i] + L.grad[1][i];
+                const float m = b1c * T.adam_m[i] + (1.f - b1c) * g;
+                const float v = b2c * T.adam_v[i] + (1.f - b2c) * g * g;
+                T.adam_m[i] = m; // lane-owned slots: no other lane touches them
+                T.adam_v[i] = v;
+                const float mh = m / (1.f - bc1), vh = v / (1.f - bc2);
+                wr[k] -= lr * mh / (sqrtf(vh) + eps);
+            }
         }
         lr *= a.lr_decay;
-        __threadfence_block();
-        __syncthreads();
+        __syncthreads(); // every lane's reads of the LDS weights and gradients are done
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = tid + kTrainLanes * k;
+        if (i < kParams) a.w[i] = wr[k];
     }
 }
 
@@ -346,9 +392,8 @@ __global__ __launch_bounds__(128) void mlp_eval_kernel(const float* __restrict__
     if (i < n) out[i] = y;
 }
 
-// w[i] from Philox(seed, i): layer-wise scales (see mlp.hpp); Adam moments zeroed.
-__global__ __launch_bounds__(256) void mlp_init_kernel(float* w, float* m, float* v,
-                                                       uint64_t seed, float y_mean) {
+// w[i] from Philox(seed, i): layer-wise scales (see mlp.hpp).
+__global__ __launch_bounds__(256) void mlp_init_kernel(float* w, uint64_t seed, float y_mean) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
     const float z = inv_normal_f32(philox_w0((uint32_t)i, 0u, 0u, 9u, (uint32_t)seed,
@@ -361,29 +406,27 @@ __global__ __launch_bounds__(256) void mlp_init_kernel(float* w, float* m, float
     else if (i < oB3) val = z * 0.125f;        // W3
     else val = y_mean;                         // b3
     w[i] = val;
-    m[i] = 0.f;
-    v[i] = 0.f;
 }
 
 namespace {
 size_t lds_bytes() { return sizeof(MlpLds); }
 } // namespace
 
-int mlp_init_device(float* w, float* adam_m, float* adam_v, uint64_t seed, float y_mean,
-                    hipStream_t s) {
-    hipLaunchKernelGGL(mlp_init_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, w, adam_m,
-                       adam_v, seed, y_mean);
+int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_init_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, w, seed,
+                       y_mean);
     return status_of(hipGetLastError());
 }
 
-int mlp_train_device(float* w, float* m, float* v, const float* xs, const float* ts,
-                     const float* ys, uint64_t n, int steps, float lr, float lr_decay,
-                     uint64_t seed, float* loss, float* grad, hipStream_t s) {
-    TrainArgs a{w, m, v, xs, ts, ys, n, steps, lr, lr_decay, seed, loss, grad};
+int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
+                     int steps, float lr, float lr_decay, uint64_t seed, float* loss, float* grad,
+                     hipStream_t s) {
+    TrainArgs a{w, xs, ts, ys, n, steps, lr, lr_decay, seed, loss, grad};
+    const int bytes = (int)sizeof(TrainLds);
     hipError_t e = hipFuncSetAttribute((const void*)mlp_train_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes());
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return status_of(e);
-    hipLaunchKernelGGL(mlp_train_kernel, dim3(1), dim3(128), lds_bytes(), s, a);
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(1), dim3(kTrainLanes), bytes, s, a);
     return status_of(hipGetLastError());
 }
 
@@ -433,7 +476,7 @@ int mcdx_mlp_gradients(const float* weights, const float* xs, const float* ts, c
     if (e == hipSuccess) e = hipMemcpy(dt, ts, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dy, ys, n * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) return status_of(e);
-    if ((st = mlp_train_device(dw, dm, dv, dx, dt, dy, n, 1, 0.f, 1.f, 0, dl, dg, ctx->stream))) return st;
+    if ((st = mlp_train_device(dw, dx, dt, dy, n, 1, 0.f, 1.f, 0, dl, dg, ctx->stream))) return st;
     e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(grad, dg, pb, hipMemcpyDeviceToHost);
     if (e == hipSuccess && loss2) e = hipMemcpy(loss2, dl, 8, hipMemcpyDeviceToHost);
@@ -465,10 +508,9 @@ int mcdx_mlp_train(float* weights, const float* xs, const float* ts, const float
     if (e == hipSuccess) e = hipMemcpy(dx, xs, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dt, ts, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dy, ys, n * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(dm, 0, 2 * pb, ctx->stream);
     if (e != hipSuccess) return status_of(e);
-    if ((st = mlp_train_device(dw, dm, dv, dx, dt, dy, n, steps, learning_rate, lr_decay, seed,
-                               dl, nullptr, ctx->stream)))
+    if ((st = mlp_train_device(dw, dx, dt, dy, n, steps, learning_rate, lr_decay, seed, dl,
+                               nullptr, ctx->stream)))
         return st;
     e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(weights, dw, pb, hipMemcpyDeviceToHost);

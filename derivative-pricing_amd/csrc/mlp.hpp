@@ -10,15 +10,15 @@ namespace mcdx {
 constexpr int kMlpParams = 4417; // 2 -> 64 -> 64 -> 1
 
 // Deterministic initialisation from Philox(seed): N(0,1) first layer and biases, He-scaled
-// second layer, 1/8-scaled output layer, output bias = y_mean; Adam moments zeroed.
-int mlp_init_device(float* w, float* adam_m, float* adam_v, uint64_t seed, float y_mean,
-                    hipStream_t s);
-// `steps` Adam steps of 128 samples drawn by Philox(seed) from (xs, ts, ys)[0..n); samples
-// whose target is NaN are redrawn (up to 8 times, then masked). loss[step] receives the
-// mean squared error of the step's first 64 samples when non-null.
-int mlp_train_device(float* w, float* m, float* v, const float* xs, const float* ts,
-                     const float* ys, uint64_t n, int steps, float lr, float lr_decay,
-                     uint64_t seed, float* loss, float* grad, hipStream_t s);
+// second layer, 1/8-scaled output layer, output bias = y_mean.
+int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s);
+// `steps` Adam steps (zero initial moments) of 128 samples drawn by Philox(seed) from
+// (xs, ts, ys)[0..n); samples whose target is NaN are redrawn (up to 8 times, then masked).
+// loss[step] receives the mean squared error of the step's first 64 samples when non-null.
+// With grad non-null: one pass over samples 0..127, gradients to grad, no update.
+int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
+                     int steps, float lr, float lr_decay, uint64_t seed, float* loss, float* grad,
+                     hipStream_t s);
 int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
                     hipStream_t s);
 

@@ -134,7 +134,7 @@ SIGNATURES = {
                                                          P(LearnOptions), P(C.c_float), P(dbl),
                                                          P(u8), P(LearnedResult)]),
     "mcdx_lsm_learned_price": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, P(C.c_float),
-                                                      u32, dbl, dbl, P(dbl), P(u8),
+                                                      u32, dbl, dbl, P(dbl), P(u8), u8,
                                                       P(LearnedResult)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
@@ -330,14 +330,14 @@ def lsm_learned(spot, strike, rate, carry, vol, time, type, m, n, seed, precisio
 
 
 def lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, w,
-                      grid, lo, hi, coeffs, has):
+                      grid, lo, hi, coeffs, has, judge=True):
     r = LearnedResult()
     w_, pw = _f32(w)
     co = np.ascontiguousarray(coeffs, dtype=np.float64).ravel()
     hs = np.ascontiguousarray(has, dtype=np.uint8)
     check(lib().mcdx_lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed,
                                        precision, pw, grid, lo, hi, co.ctypes.data_as(P(dbl)),
-                                       hs.ctypes.data_as(P(u8)), C.byref(r)),
+                                       hs.ctypes.data_as(P(u8)), 1 if judge else 0, C.byref(r)),
           "mcdx_lsm_learned_price")
     return _learned_dict(r)
 

@@ -282,13 +282,16 @@ int mcdx_lsm_american_learned(double spot, double strike, double rate, double ca
                               const mcdx_lsm_learn_options* options, float* weights,
                               double* coefficients, uint8_t* has_coefficients,
                               mcdx_lsm_learned_result* out);
-/* Price with a trained network and QR policy on fresh paths: streaming, no path store. */
+/* Price with a trained network on fresh paths: streaming, no path store. With judge != 0 the
+ * QR policy also runs on the same paths (qr_forward, decisions, disagreements); with judge
+ * == 0 it only decides outside the table range and qr_forward is NaN. */
 int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry_yield,
                            double vol, double time, uint8_t type, int32_t monitoring_points,
                            uint64_t path_count, uint64_t seed, int precision,
                            const float* weights, uint32_t grid_points, double table_lo,
                            double table_hi, const double* coefficients,
-                           const uint8_t* has_coefficients, mcdx_lsm_learned_result* out);
+                           const uint8_t* has_coefficients, uint8_t judge,
+                           mcdx_lsm_learned_result* out);
 
 /* ---- Batched book: many GBM contracts, one persistent graph-captured launch ----
  * No reference counterpart (the reference prices one contract per call,

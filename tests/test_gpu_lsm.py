@@ -70,3 +70,11 @@ def test_exercise_at_inception_and_reference_signature():
     r = mcdx.Result()
     assert mcdx.lib().mcdx_monte_carlo_lsm_american(100, 100, .05, 0, .25, 1, 1, 20, 50_000, 7, r) == 0
     assert r.price > 0 and r.standard_error > 0
+
+
+def test_large_path_count():
+    """2^22 paths: 32 paths per lane in the backward-induction layout."""
+    crr = L.orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 2000, 1)
+    p, se, _, has, _ = mcdx.lsm(100, 100, 0.06, 0, 0.2, 1, 1, 50, 1 << 22, 7, FP32)
+    assert has[1:50].all()
+    assert abs(p - crr) < 3 * se + 0.05

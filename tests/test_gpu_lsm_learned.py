@@ -78,6 +78,11 @@ def test_qr_forward_replay_reproduces_backward_induction(prec):
     s = mcdx.lsm_learned_price(*PUT, n, 42, prec, w, 1024, d["table_lo"], d["table_hi"], co, has)
     assert s["learned"] == d["learned"] and s["qr_forward"] == d["qr_forward"]
     assert s["disagreements"] == d["disagreements"] and s["fallbacks"] == d["fallbacks"]
+    # without the QR judge the network's decisions, and so its price, are unchanged
+    u = mcdx.lsm_learned_price(*PUT, n, 42, prec, w, 1024, d["table_lo"], d["table_hi"], co, has,
+                               judge=False)
+    assert u["learned"] == d["learned"] and u["fallbacks"] == d["fallbacks"]
+    assert u["qr_forward"][0] != u["qr_forward"][0]  # NaN: not formed
 
 
 def test_learned_american_put_c5():
