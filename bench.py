@@ -179,37 +179,79 @@ def extras(torch, mcdx, L, stream, handle, ref):
         "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
         "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)]}
     progress("C3 done")
-    # C4: 4096-contract book, one persistent graph launch vs 4096 per-contract launches.
-    cs = mcdx.c4_book()
-    book = mcdx.Book(cs, mcdx.PREC_FP32)
-    tuned_ms = book.autotune()
-    progress("C4 autotuned")
-    book.enqueue(handle.value)
-    batched_ms = timed_events(torch, stream, lambda: book.enqueue(handle.value), reps=2)
-    outs = torch.zeros((len(cs), 3), dtype=torch.float64, device="cuda")
-
-    def per_contract():
-        for i, c in enumerate(cs):
-            mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
-                                                C.c_void_p(outs[i].data_ptr())), "enqueue")
-    per_contract()
-    per_ms = timed_events(torch, stream, per_contract, reps=1)
-    bres = book.results()
-    host = outs.cpu().numpy()
-    counts = host.view("u8")
-    same = all(bres[i] == mcdx.welford_finish(int(counts[i, 0]), float(host[i, 1]),
-                                              float(host[i, 2])) for i in range(len(cs)))
-    total_paths = sum(c.path_count for c in cs)
-    res["c4_book_4096"] = {"batched_ms": batched_ms, "per_contract_ms": per_ms,
-                           "speedup": per_ms / batched_ms, "paths": total_paths,
-                           "batched_paths_per_s": total_paths / (batched_ms / 1e3),
-                           "autotuned_schedule": book.schedule(), "autotune_best_ms": tuned_ms,
-                           "bitwise_equal_to_per_contract": same}
-    book.close()
+    # C4: 4096-contract book, one persistent graph launch vs 4096 per-contract launches of
+    # the same path kernel. Headline case: one Monte Carlo seed for the whole book (common
+    # random numbers), so contracts of one payoff family and step count share each path's
+    # draws; every contract's result still equals its own launch bit for bit. Also reported:
+    # distinct seeds per contract (no sharing possible) and the book with CRN
+    # finite-difference delta/gamma/vega against 5 per-contract launches per contract.
+    res["c4_book_4096"] = c4_entry(torch, mcdx, L, stream, handle)
     progress("C4 done")
     res["c5_american_put_lsm"] = c5_lsm(mcdx, mcd)
     progress("C5 done")
     return res
+
+
+def c4_entry(torch, mcdx, L, stream, handle):
+    def per_contract(cs, outs):
+        for i, c in enumerate(cs):
+            mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
+                                                C.c_void_p(outs[i].data_ptr())), "enqueue")
+
+    def run(cs, flags=0):
+        book = mcdx.Book(cs, mcdx.PREC_FP32, flags)
+        tuned = book.autotune()
+        book.enqueue(handle.value)
+        batched = timed_events(torch, stream, lambda: book.enqueue(handle.value), reps=2)
+        res = book.results()
+        greeks = book.greeks() if flags & mcdx.Book.GREEKS else None
+        sched = book.schedule()
+        book.close()
+        return batched, tuned, res, greeks, sched
+
+    def per(cs):
+        outs = torch.zeros((len(cs), 3), dtype=torch.float64, device="cuda")
+        per_contract(cs, outs)
+        ms = timed_events(torch, stream, lambda: per_contract(cs, outs), reps=1)
+        host = outs.cpu().numpy()
+        counts = host.view("u8")
+        return ms, [mcdx.welford_finish(int(counts[i, 0]), float(host[i, 1]), float(host[i, 2]))
+                    for i in range(len(cs))]
+
+    out = {}
+    crn = mcdx.c4_book(mc_seed=SEED)
+    b_ms, tuned, bres, _, sched = run(crn)
+    progress("C4 shared-seed book timed")
+    p_ms, pres = per(crn)
+    total = sum(c.path_count for c in crn)
+    out.update(seeds="one seed (42) for the book", batched_ms=b_ms, per_contract_ms=p_ms,
+               speedup=p_ms / b_ms, paths=total, batched_paths_per_s=total / (b_ms / 1e3),
+               autotuned_schedule=sched, autotune_best_ms=tuned,
+               bitwise_equal_to_per_contract=bres == pres)
+    dist = mcdx.c4_book()
+    d_ms, _, dres, _, _ = run(dist)
+    dp_ms, dpres = per(dist)
+    out["distinct_seeds"] = {"batched_ms": d_ms, "per_contract_ms": dp_ms,
+                             "speedup": dp_ms / d_ms, "bitwise_equal_to_per_contract": dres == dpres}
+    progress("C4 distinct-seed book timed")
+    g_ms, _, gres, greeks, _ = run(crn, mcdx.Book.GREEKS)
+    bumped = []
+    for c in crn:
+        hs, hv = max(0.01 * c.spot, 1e-4), max(0.01 * c.vol, 1e-4)
+        for field, val in (("spot", c.spot + hs), ("spot", c.spot - hs), ("vol", c.vol + hv),
+                           ("vol", c.vol - hv)):
+            x = mcdx.GbmContract.from_buffer_copy(c)
+            setattr(x, field, val)
+            bumped.append(x)
+    bp_ms, _ = per(bumped)
+    out["greeks"] = {"what": "price + CRN central-difference delta, gamma, vega per contract "
+                             "(default bumps: 1% spot, 1% vol)",
+                     "batched_ms": g_ms, "per_contract_ms": p_ms + bp_ms,
+                     "per_contract_launches": 5 * len(crn), "speedup": (p_ms + bp_ms) / g_ms,
+                     "prices_bitwise_equal": gres == pres,
+                     "example_contract_0": {"delta": greeks[0][0][:2], "gamma": greeks[0][1][:2],
+                                            "vega": greeks[0][2][:2]}}
+    return out
 
 
 def c5_lsm(mcdx, mcd):
@@ -273,7 +315,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary configs")
-    ap.add_argument("--only", choices=["c5"], help="run one secondary config alone (profiling)")
+    ap.add_argument("--only", choices=["c4", "c5"],
+                    help="run one secondary config alone (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -285,10 +328,6 @@ def main():
 
     torch.cuda.init()
     L = mcdx.lib()
-    if args.only == "c5":
-        import mcd
-        print(json.dumps({"c5_american_put_lsm": c5_lsm(mcdx, mcd)}), flush=True)
-        return 0
     cs = contracts(mcdx, args.paths)
     names = list(cs)
     # A dedicated stream (default priority): its handle is non-null, so the engine enqueues
@@ -297,6 +336,13 @@ def main():
     torch.cuda.set_stream(stream)
     handle = C.c_void_p(stream.cuda_stream)
     assert handle.value, "need a non-default stream handle"
+    if args.only == "c5":
+        import mcd
+        print(json.dumps({"c5_american_put_lsm": c5_lsm(mcdx, mcd)}), flush=True)
+        return 0
+    if args.only == "c4":
+        print(json.dumps({"c4_book_4096": c4_entry(torch, mcdx, L, stream, handle)}), flush=True)
+        return 0
     out = torch.zeros((len(names), 3), dtype=torch.float64, device="cuda")
 
     def step(events=None):

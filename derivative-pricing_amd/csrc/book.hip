@@ -14,23 +14,51 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
 #include "gbm_paths.hpp"
 
+namespace mcdx {
+
+// A shared-draw group: up to kFuseMax contracts of one kind, seed and step count (path counts
+// descending), evaluated on the same paths from one normal per step. With `greeks` the
+// members are a contract and its four bumped scenarios (spot up/down, vol up/down) and the
+// per-path central differences go to three more result slots.
+constexpr int kFuseMax = 8;
+
+struct FusedGroup {
+    uint32_t c[kFuseMax];
+    uint32_t k, m;
+    uint32_t greeks, gslot;
+    double inv_2hs, inv_hs2, inv_2hv;
+    int32_t kind;
+};
+
+} // namespace mcdx
+
 struct mcdx_book {
-    uint32_t n = 0;
+    uint32_t n = 0;                             // user contracts (result slots 0..n-1)
+    uint32_t slots = 0;                         // + bumped scenarios + Greek slots
+    uint32_t flags = 0;
     int precision = MCDX_PREC_FP32;
     uint64_t total_tiles = 0;
-    std::vector<mcdx_gbm_contract> contracts;
-    std::vector<uint64_t> tile_offsets;         // [n+1]
-    std::vector<double> item_cost;              // per contract, cost of one tile
+    std::vector<mcdx_gbm_contract> contracts;   // per slot (Greek slots copy their contract)
+    std::vector<uint64_t> tile_offsets;         // [slots+1]
+    std::vector<double> item_cost;              // per slot, cost of one tile
+    std::vector<int32_t> group_of;              // per slot: fused group or -1
+    std::vector<mcdx::FusedGroup> groups;
+    std::vector<uint32_t> greek_slot;           // per user contract (greeks books)
+    uint64_t n_items = 0;                       // plain items, then the group items of each
+    uint64_t n_fused[4] = {0, 0, 0, 0};         // family (terminal, Asian, barrier, lookback)
+    int max_fused_blocks_per_cu[4] = {1, 1, 1, 1};
     // device
     mcdx::GbmArgs* d_args = nullptr;
     uint8_t* d_kind = nullptr;                  // kind | (antithetic << 4)
     uint64_t* d_toff = nullptr;
-    uint64_t* d_items64 = nullptr;              // (contract << 40) | tile
+    uint64_t* d_items64 = nullptr;              // (contract << 40) | tile, or fused (bit 63)
+    mcdx::FusedGroup* d_groups = nullptr;
     mcdx::Welford* d_partials = nullptr;
     mcdx::Welford* d_results = nullptr;
     // schedule
@@ -57,6 +85,123 @@ __device__ __forceinline__ double book_sample(const GbmArgs& a, int kind, bool a
     default: return anti ? path_sample<kLookback, FP64, true>(a, path) : path_sample<kLookback, FP64, false>(a, path);
     }
 }
+
+__device__ __forceinline__ void welford_add_k(Welford& w, double x, int k) {
+    w.n += 1;
+    const double d = x - w.mean;
+    w.mean += d * (1.0 / (double)(k + 1));
+    w.m2 += d * (x - w.mean);
+}
+
+// One tile of a shared-draw group. Each member's state follows path_f32's arithmetic step for
+// step (the same normals, the same log2-space update, observe and settle functions), so each
+// member's tile partial equals the one its own launch writes.
+template <int KIND>
+__device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, const FusedGroup& g,
+                                           uint64_t tile, const uint64_t* __restrict__ toff,
+                                           Welford* __restrict__ partials, Welford* lds4) {
+#pragma clang fp contract(fast)
+    const GbmArgs* A[kFuseMax];
+    int kact = 0;
+#pragma unroll
+    for (int i = 0; i < kFuseMax; ++i) {
+        A[i] = args + g.c[(uint32_t)i < g.k ? i : 0];
+        if ((uint32_t)i < g.k && A[i]->n_paths > tile * kTilePaths) kact = i + 1;
+    }
+    const uint32_t k0 = (uint32_t)A[0]->seed, k1 = (uint32_t)(A[0]->seed >> 32);
+    Welford w[kFuseMax], gw[3];
+#pragma unroll
+    for (int i = 0; i < kFuseMax; ++i) w[i] = Welford{0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gw[j] = Welford{0, 0.0, 0.0};
+#pragma unroll 1
+    for (int k = 0; k < kPathsPerLane; ++k) {
+        const uint64_t path = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
+        const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+        double v[kFuseMax];
+        if constexpr (KIND == kTerminal) {
+            const float z = inv_normal_f32(philox_w0(plo, phh, 0u, 0u, k0, k1));
+#pragma unroll
+            for (int i = 0; i < kFuseMax; ++i)
+                v[i] = i < kact ? A[i]->discount *
+                                      terminal_payoff(*A[i], A[i]->spot * (double)fast_exp2(
+                                                                 fmaf(A[i]->sig2, z, A[i]->mu2)))
+                                : 0.0;
+        } else {
+            StateF32 s[kFuseMax];
+            bool na[kFuseMax], ng[kFuseMax];
+#pragma unroll
+            for (int i = 0; i < kFuseMax; ++i) {
+                s[i] = StateF32{0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
+                na[i] = A[i]->average_style == 0 || A[i]->cv;
+                ng[i] = A[i]->average_style == 1 || A[i]->cv;
+            }
+            auto apply = [&](float z) {
+#pragma unroll
+                for (int i = 0; i < kFuseMax; ++i) {
+                    if (i < kact) {
+                        s[i].x = s[i].x + fmaf(A[i]->sig2, z, A[i]->mu2);
+                        observe_f32<KIND>(*A[i], s[i], na[i], ng[i]);
+                    }
+                }
+            };
+            const int m = (int)g.m;
+            constexpr int B = 8;
+            int step = 0;
+            for (; step + B <= m; step += B) {
+                float z[B];
+                normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
+#pragma unroll
+                for (int j = 0; j < B; ++j) apply(z[j]);
+            }
+            for (; step < m; ++step) apply(inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+#pragma unroll
+            for (int i = 0; i < kFuseMax; ++i) v[i] = i < kact ? settle_f32<KIND>(*A[i], s[i]) : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kFuseMax; ++i)
+            if (i < kact && path < A[i]->n_paths) welford_add_k(w[i], v[i], k);
+        if (g.greeks && path < A[0]->n_paths) { // finite_difference.cpp:58-62, path by path
+            welford_add_k(gw[0], (v[1] - v[2]) * g.inv_2hs, k);
+            welford_add_k(gw[1], ((v[1] - 2.0 * v[0]) + v[2]) * g.inv_hs2, k);
+            welford_add_k(gw[2], (v[3] - v[4]) * g.inv_2hv, k);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kFuseMax; ++i) {
+        if (i < kact) { // uniform across the block
+            const Welford t = block_merge_256_all(w[i], lds4);
+            partials[toff[g.c[i]] + tile] = t;
+        }
+    }
+    if (g.greeks) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const Welford t = block_merge_256_all(gw[j], lds4);
+            partials[toff[g.gslot + j] + tile] = t;
+        }
+    }
+}
+
+// Persistent blocks over one payoff family's shared-draw items (group << 40 | tile), round
+// robin like book_kernel; one instantiation per family keeps each at its own register
+// budget. FP32 only.
+template <int KIND>
+__global__ __launch_bounds__(kTileLanes) void book_fused_kernel(const GbmArgs* __restrict__ args,
+                                                                const uint64_t* __restrict__ toff,
+                                                                const uint64_t* __restrict__ items,
+                                                                uint64_t n_items,
+                                                                const FusedGroup* __restrict__ groups,
+                                                                Welford* __restrict__ partials) {
+    __shared__ Welford lds4[4];
+    for (uint64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const uint64_t packed = items[it];
+        const FusedGroup g = groups[packed >> 40];
+        fused_item<KIND>(args, g, packed & ((1ull << 40) - 1), toff, partials, lds4);
+    }
+}
+
+constexpr int kFusedKinds[4] = {kTerminal, kAsian, kBarrier, kLookback};
 
 // Persistent blocks over the item list, round robin: block b takes items b, b+G, b+2G, ...
 // in the schedule's order (longest tiles first spreads the expensive tiles over all blocks).
@@ -101,8 +246,11 @@ std::mutex g_tune_mu;
 std::map<uint64_t, std::pair<uint32_t, uint32_t>> g_tuned; // book shape -> (blocks/CU, order)
 
 uint64_t shape_key(const mcdx_book& b) {
-    // product mix, log2 path-count histogram and step histogram (FNV-1a over the counts)
-    uint64_t hist[5 + 41 + 64] = {0};
+    // product mix, log2 path-count histogram and step histogram (FNV-1a over the counts),
+    // plus the sharing / Greeks flags and the group count
+    uint64_t hist[5 + 41 + 64 + 2] = {0};
+    hist[5 + 41 + 64] = b.flags;
+    hist[5 + 41 + 64 + 1] = b.groups.size();
     for (const auto& c : b.contracts) {
         hist[kind_of(c)]++;
         int lg = 0;
@@ -121,37 +269,88 @@ uint64_t shape_key(const mcdx_book& b) {
 
 int upload_items(mcdx_book& b) {
     std::vector<uint64_t> items;
+    std::vector<double> cost;
     items.reserve(b.total_tiles);
-    for (uint32_t c = 0; c < b.n; ++c)
-        for (uint64_t t = 0; t < b.tile_offsets[c + 1] - b.tile_offsets[c]; ++t)
+    for (uint32_t c = 0; c < b.slots; ++c) {
+        if (b.group_of[c] != -1) continue; // in a shared-draw group (>= 0) or a Greek slot (-2)
+        for (uint64_t t = 0; t < b.tile_offsets[c + 1] - b.tile_offsets[c]; ++t) {
             items.push_back(((uint64_t)c << 40) | t);
-    if (b.order == 1)
-        std::stable_sort(items.begin(), items.end(), [&](uint64_t x, uint64_t y) {
-            return b.item_cost[x >> 40] > b.item_cost[y >> 40];
-        });
+            cost.push_back(b.item_cost[c]);
+        }
+    }
+    std::vector<uint64_t> fused[4];
+    std::vector<double> fcost[4];
+    for (uint32_t gi = 0; gi < b.groups.size(); ++gi) {
+        const FusedGroup& g = b.groups[gi];
+        int f = 0;
+        while (kFusedKinds[f] != g.kind) ++f;
+        const uint64_t tiles = b.tile_offsets[g.c[0] + 1] - b.tile_offsets[g.c[0]];
+        const double c1 = (double)g.m * (1.0 + 0.12 * (double)g.k);
+        for (uint64_t t = 0; t < tiles; ++t) {
+            fused[f].push_back(((uint64_t)gi << 40) | t);
+            fcost[f].push_back(c1);
+        }
+    }
+    auto order_by_cost = [&](std::vector<uint64_t>& it, const std::vector<double>& c) {
+        if (b.order != 1) return;
+        std::vector<uint32_t> idx(it.size());
+        for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return c[x] > c[y]; });
+        std::vector<uint64_t> sorted(it.size());
+        for (size_t i = 0; i < idx.size(); ++i) sorted[i] = it[idx[i]];
+        it.swap(sorted);
+    };
+    order_by_cost(items, cost);
+    b.n_items = items.size();
+    for (int f = 0; f < 4; ++f) {
+        order_by_cost(fused[f], fcost[f]);
+        b.n_fused[f] = fused[f].size();
+        items.insert(items.end(), fused[f].begin(), fused[f].end());
+    }
     hipError_t e = hipMemcpy(b.d_items64, items.data(), items.size() * sizeof(uint64_t),
                              hipMemcpyHostToDevice);
     return status_of(e);
 }
 
-int grid_blocks(const mcdx_book& b) {
-    const uint32_t per_cu = b.blocks_per_cu ? std::min<uint32_t>(b.blocks_per_cu, b.max_blocks_per_cu)
-                                            : (uint32_t)b.max_blocks_per_cu;
+// Grid of the plain kernel (family < 0) or of one family's shared-draw kernel.
+int grid_blocks(const mcdx_book& b, int family = -1) {
+    const uint32_t lim = family >= 0 ? (uint32_t)b.max_fused_blocks_per_cu[family]
+                                     : (uint32_t)b.max_blocks_per_cu;
+    const uint32_t per_cu = b.blocks_per_cu ? std::min<uint32_t>(b.blocks_per_cu, lim) : lim;
     const uint64_t g = (uint64_t)per_cu * (uint64_t)b.cus;
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, b.total_tiles));
+    const uint64_t n = family >= 0 ? b.n_fused[family] : b.n_items;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, n));
+}
+
+template <int F>
+void launch_family(const mcdx_book& b, uint64_t offset, hipStream_t s) {
+    if (!b.n_fused[F]) return;
+    hipLaunchKernelGGL(book_fused_kernel<kFusedKinds[F]>, dim3(grid_blocks(b, F)), dim3(kTileLanes),
+                       0, s, b.d_args, b.d_toff, b.d_items64 + offset, b.n_fused[F], b.d_groups,
+                       b.d_partials);
 }
 
 hipError_t enqueue_raw(mcdx_book& b, hipStream_t s) {
-    const int grid = grid_blocks(b);
-    if (b.precision == MCDX_PREC_FP64)
-        hipLaunchKernelGGL(book_kernel<true>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args, b.d_kind,
-                           b.d_toff, b.d_items64, b.total_tiles, b.d_partials);
-    else
-        hipLaunchKernelGGL(book_kernel<false>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args, b.d_kind,
-                           b.d_toff, b.d_items64, b.total_tiles, b.d_partials);
+    if (b.n_items) {
+        const int grid = grid_blocks(b);
+        if (b.precision == MCDX_PREC_FP64)
+            hipLaunchKernelGGL(book_kernel<true>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args,
+                               b.d_kind, b.d_toff, b.d_items64, b.n_items, b.d_partials);
+        else
+            hipLaunchKernelGGL(book_kernel<false>, dim3(grid), dim3(kTileLanes), 0, s, b.d_args,
+                               b.d_kind, b.d_toff, b.d_items64, b.n_items, b.d_partials);
+    }
+    uint64_t off = b.n_items;
+    launch_family<0>(b, off, s);
+    off += b.n_fused[0];
+    launch_family<1>(b, off, s);
+    off += b.n_fused[1];
+    launch_family<2>(b, off, s);
+    off += b.n_fused[2];
+    launch_family<3>(b, off, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_final_merge(b.d_partials, b.d_toff, 0, b.n, b.d_results, s);
+    return launch_final_merge(b.d_partials, b.d_toff, 0, b.slots, b.d_results, s);
 }
 
 int rebuild_graph(mcdx_book& b) {
@@ -177,7 +376,8 @@ void free_book(mcdx_book* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->d_args, b->d_kind, b->d_toff, b->d_items64, b->d_partials, b->d_results};
+    void* ptrs[] = {b->d_args, b->d_kind, b->d_toff, b->d_items64, b->d_groups, b->d_partials,
+                    b->d_results};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -191,27 +391,62 @@ using namespace mcdx;
 
 extern "C" {
 
-int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
-                     mcdx_book** out) {
-    if (!out || (!contracts && n) || n == 0 || n > (1u << 24)) return MCDX_EINVAL;
+int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
+                        uint32_t flags, mcdx_book** out) {
+    if (!out || (!contracts && n) || n == 0 || n > (1u << 22) || (flags & ~3u)) return MCDX_EINVAL;
     *out = nullptr;
+    const bool greeks = (flags & MCDX_BOOK_GREEKS) != 0;
+    const bool share = (flags & MCDX_BOOK_NO_SHARING) == 0 && precision == MCDX_PREC_FP32;
     for (uint32_t i = 0; i < n; ++i) {
         const int st = validate(&contracts[i], precision);
         if (st) return st;
+        const int kind = kind_of(contracts[i]);
+        if (greeks && (precision != MCDX_PREC_FP32 || contracts[i].antithetic ||
+                       kind == kBarrierBB || contracts[i].spot <= 0.0 || contracts[i].vol <= 0.0))
+            return MCDX_EINVAL;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCDX_ENODEV;
     mcdx_book* b = new mcdx_book();
     b->n = n;
+    b->flags = flags;
     b->precision = precision;
     b->contracts.assign(contracts, contracts + n);
-    b->tile_offsets.resize(n + 1);
-    b->item_cost.resize(n);
-    std::vector<GbmArgs> args(n);
-    std::vector<uint8_t> kinds(n);
+    // Greeks: four bumped scenarios per contract (default_bump_sizes,
+    // finite_difference.cpp:31-38), then three result slots (delta, gamma, vega).
+    std::vector<double> hs(n), hv(n);
+    if (greeks) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const mcdx_gbm_contract c = contracts[i];
+            hs[i] = std::max(c.spot * 0.01, 1e-4);
+            hv[i] = std::max(c.vol * 0.01, 1e-4);
+            mcdx_gbm_contract v = c;
+            v.spot = c.spot + hs[i];
+            b->contracts.push_back(v);
+            v.spot = c.spot - hs[i];
+            b->contracts.push_back(v);
+            v = c;
+            v.vol = c.vol + hv[i];
+            b->contracts.push_back(v);
+            v.vol = c.vol - hv[i];
+            b->contracts.push_back(v);
+        }
+        b->greek_slot.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            b->greek_slot[i] = (uint32_t)b->contracts.size();
+            for (int j = 0; j < 3; ++j) b->contracts.push_back(contracts[i]);
+        }
+    }
+    const uint32_t slots = (uint32_t)b->contracts.size();
+    b->slots = slots;
+    b->tile_offsets.resize(slots + 1);
+    b->item_cost.resize(slots);
+    b->group_of.assign(slots, -1);
+    std::vector<GbmArgs> args(slots);
+    std::vector<uint8_t> kinds(slots);
     uint64_t acc = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const mcdx_gbm_contract& c = contracts[i];
+    for (uint32_t i = 0; i < slots; ++i) {
+        const mcdx_gbm_contract& c = b->contracts[i];
         b->tile_offsets[i] = acc;
         acc += tiles_for(c.path_count);
         args[i] = make_args(c);
@@ -220,9 +455,62 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
         const double steps = kind == kTerminal ? 1.0 : (double)c.monitoring_points;
         b->item_cost[i] = steps * (c.antithetic ? 1.3 : 1.0) * (kind == kBarrierBB ? 1.5 : 1.0);
     }
-    b->tile_offsets[n] = acc;
+    b->tile_offsets[slots] = acc;
     b->total_tiles = acc;
     if (acc == 0 || acc >= (1ull << 40)) {
+        delete b;
+        return MCDX_EINVAL;
+    }
+    // Shared-draw groups.
+    auto fusable = [&](const mcdx_gbm_contract& c) {
+        const int kind = kind_of(c);
+        return precision == MCDX_PREC_FP32 && !c.antithetic && kind != kBarrierBB;
+    };
+    if (greeks) {
+        for (uint32_t i = 0; i < n; ++i) {
+            FusedGroup g{};
+            g.c[0] = i;
+            for (int j = 0; j < 4; ++j) g.c[1 + j] = n + 4 * i + j;
+            g.k = 5;
+            g.kind = kind_of(contracts[i]);
+            g.m = g.kind == kTerminal ? 1 : (uint32_t)contracts[i].monitoring_points;
+            g.greeks = 1;
+            g.gslot = b->greek_slot[i];
+            g.inv_2hs = 1.0 / (2.0 * hs[i]);
+            g.inv_hs2 = 1.0 / (hs[i] * hs[i]);
+            g.inv_2hv = 1.0 / (2.0 * hv[i]);
+            for (uint32_t j = 0; j < g.k; ++j) b->group_of[g.c[j]] = (int32_t)b->groups.size();
+            for (int j = 0; j < 3; ++j) b->group_of[g.gslot + j] = -2;
+            b->groups.push_back(g);
+        }
+    } else if (share) {
+        // key (seed, kind, steps) -> members by path count, descending
+        std::map<std::tuple<uint64_t, int, int>, std::vector<uint32_t>> buckets;
+        for (uint32_t i = 0; i < n; ++i) {
+            const mcdx_gbm_contract& c = contracts[i];
+            if (!fusable(c)) continue;
+            const int kind = kind_of(c);
+            buckets[{c.seed, kind, kind == kTerminal ? 1 : c.monitoring_points}].push_back(i);
+        }
+        for (auto& kv : buckets) {
+            auto& mem = kv.second;
+            if (mem.size() < 2) continue;
+            std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
+                return contracts[x].path_count > contracts[y].path_count;
+            });
+            for (size_t o = 0; o < mem.size(); o += kFuseMax) {
+                FusedGroup g{};
+                g.k = (uint32_t)std::min<size_t>(kFuseMax, mem.size() - o);
+                if (g.k < 2) break; // a lone tail member stays a plain item
+                for (uint32_t j = 0; j < g.k; ++j) g.c[j] = mem[o + j];
+                g.kind = std::get<1>(kv.first);
+                g.m = (uint32_t)std::get<2>(kv.first);
+                for (uint32_t j = 0; j < g.k; ++j) b->group_of[g.c[j]] = (int32_t)b->groups.size();
+                b->groups.push_back(g);
+            }
+        }
+    }
+    if (b->groups.size() >= (1u << 23)) {
         delete b;
         return MCDX_EINVAL;
     }
@@ -235,17 +523,29 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
                 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, book_kernel<true>, kTileLanes, 0)
                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, book_kernel<false>, kTileLanes, 0);
     b->max_blocks_per_cu = occ > 0 ? occ : 1;
+    const void* fk[4] = {(const void*)book_fused_kernel<kTerminal>, (const void*)book_fused_kernel<kAsian>,
+                         (const void*)book_fused_kernel<kBarrier>, (const void*)book_fused_kernel<kLookback>};
+    for (int f = 0; f < 4 && e == hipSuccess; ++f) {
+        int occf = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occf, fk[f], kTileLanes, 0);
+        b->max_fused_blocks_per_cu[f] = occf > 0 ? occf : 1;
+    }
+    const size_t ng = std::max<size_t>(1, b->groups.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&b->d_args, n * sizeof(GbmArgs));
-    if (e == hipSuccess) e = hipMalloc(&b->d_kind, n);
-    if (e == hipSuccess) e = hipMalloc(&b->d_toff, (n + 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_args, slots * sizeof(GbmArgs));
+    if (e == hipSuccess) e = hipMalloc(&b->d_kind, slots);
+    if (e == hipSuccess) e = hipMalloc(&b->d_toff, (slots + 1) * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&b->d_items64, acc * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_groups, ng * sizeof(FusedGroup));
     if (e == hipSuccess) e = hipMalloc(&b->d_partials, acc * sizeof(Welford));
-    if (e == hipSuccess) e = hipMalloc(&b->d_results, n * sizeof(Welford));
-    if (e == hipSuccess) e = hipMemcpy(b->d_args, args.data(), n * sizeof(GbmArgs), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_kind, kinds.data(), n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_results, slots * sizeof(Welford));
+    if (e == hipSuccess) e = hipMemcpy(b->d_args, args.data(), slots * sizeof(GbmArgs), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_kind, kinds.data(), slots, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(b->d_toff, b->tile_offsets.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+        e = hipMemcpy(b->d_toff, b->tile_offsets.data(), (slots + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !b->groups.empty())
+        e = hipMemcpy(b->d_groups, b->groups.data(), b->groups.size() * sizeof(FusedGroup),
+                      hipMemcpyHostToDevice);
     int st = status_of(e);
     if (!st) {
         std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -263,6 +563,11 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
     }
     *out = b;
     return MCDX_OK;
+}
+
+int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
+                     mcdx_book** out) {
+    return mcdx_book_create_ex(contracts, n, precision, 0u, out);
 }
 
 int mcdx_book_set_schedule(mcdx_book* b, uint32_t blocks_per_cu, uint32_t order) {
@@ -306,6 +611,20 @@ int mcdx_book_results(mcdx_book* b, mcdx_result* out) {
     return MCDX_OK;
 }
 
+int mcdx_book_greeks(mcdx_book* b, mcdx_result* out) {
+    if (!b || !out || !(b->flags & MCDX_BOOK_GREEKS)) return MCDX_EINVAL;
+    hipStream_t s = b->last_stream ? b->last_stream : b->stream;
+    const uint32_t first = b->greek_slot[0];
+    const uint32_t count = 3 * b->n;
+    std::vector<Welford> h(count);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+        e = hipMemcpy(h.data(), b->d_results + first, count * sizeof(Welford), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return status_of(e);
+    for (uint32_t i = 0; i < count; ++i) out[i] = finish(h[i]);
+    return MCDX_OK;
+}
+
 int mcdx_book_price(mcdx_book* b, mcdx_result* out) {
     int st = mcdx_book_enqueue(b, nullptr);
     if (st) return st;
@@ -326,7 +645,9 @@ int mcdx_book_autotune(mcdx_book* b, float* best_ms) {
     uint32_t best_bpc = b->blocks_per_cu, best_order = b->order;
     int st = MCDX_OK;
     for (uint32_t order = 0; order <= 1 && !st; ++order) {
-        for (uint32_t bpc = 1; bpc <= (uint32_t)b->max_blocks_per_cu && !st; bpc *= 2) {
+        uint32_t top = (uint32_t)b->max_blocks_per_cu;
+        for (int f = 0; f < 4; ++f) top = std::max(top, (uint32_t)b->max_fused_blocks_per_cu[f]);
+        for (uint32_t bpc = 1; bpc <= top && !st; bpc *= 2) {
             if ((st = mcdx_book_set_schedule(b, bpc, order))) break;
             float ms = 0.f;
             hipError_t e = hipGraphLaunch(b->exec, b->stream); // warm-up

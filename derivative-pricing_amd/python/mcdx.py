@@ -136,6 +136,8 @@ SIGNATURES = {
     "mcdx_lsm_learned_price": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, P(C.c_float),
                                                       u32, dbl, dbl, P(dbl), P(u8), u8,
                                                       P(LearnedResult)]),
+    "mcdx_book_create_ex": (C.c_int, [P(GbmContract), u32, C.c_int, u32, P(C.c_void_p)]),
+    "mcdx_book_greeks": (C.c_int, [C.c_void_p, P(Result)]),
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
@@ -349,11 +351,21 @@ def mlp_param_count():
 class Book:
     """Owning handle over an mcdx_book (batched contracts, one persistent launch)."""
 
-    def __init__(self, contracts, precision=PREC_FP32):
+    NO_SHARING, GREEKS = 1, 2
+
+    def __init__(self, contracts, precision=PREC_FP32, flags=0):
         arr = (GbmContract * len(contracts))(*contracts)
         self.n = len(contracts)
         self.h = C.c_void_p()
-        check(lib().mcdx_book_create(arr, self.n, precision, C.byref(self.h)), "mcdx_book_create")
+        check(lib().mcdx_book_create_ex(arr, self.n, precision, flags, C.byref(self.h)),
+              "mcdx_book_create_ex")
+
+    def greeks(self):
+        """[(delta, gamma, vega)] per contract, each (value, se, paths)."""
+        out = (Result * (3 * self.n))()
+        check(lib().mcdx_book_greeks(self.h, out), "mcdx_book_greeks")
+        r = [(x.price, x.standard_error, x.path_count) for x in out]
+        return [tuple(r[3 * i:3 * i + 3]) for i in range(self.n)]
 
     def price(self):
         out = (Result * self.n)()
@@ -395,12 +407,14 @@ class Book:
             pass
 
 
-def c4_book(n_contracts=4096, seed=2026, log2_min=14, log2_max=20, scale_paths=1.0):
+def c4_book(n_contracts=4096, seed=2026, log2_min=14, log2_max=20, scale_paths=1.0,
+            mc_seed=None):
     """The batched-book workload: K ~ U[70,130], T in {0.25,0.5,1,2}, sigma ~ U[0.1,0.5],
     S=100 r=0.05 q=0; type uniform over European, cash digital, arithmetic Asian and the 8
     barriers (H = K*(1 +/- 0.2) by direction, rebate 0); N = floor(2^U(log2_min, log2_max));
     M = 252*T for path-dependent products. Parameters come from Philox (key `seed`, stream
-    2) so the book is reproducible without any library RNG."""
+    2) so the book is reproducible without any library RNG. Monte Carlo seeds are 42 + i per
+    contract, or `mc_seed` for every contract (common random numbers across the book)."""
     import math
     out = []
     for i in range(n_contracts):
@@ -412,7 +426,7 @@ def c4_book(n_contracts=4096, seed=2026, log2_min=14, log2_max=20, scale_paths=1
         n = int(math.floor(2.0 ** (log2_min + (log2_max - log2_min) * u[4]) * scale_paths))
         m = int(round(252 * t))
         common = dict(spot=100.0, strike=k, rate=0.05, carry_yield=0.0, vol=vol, time=t,
-                      path_count=max(n, 1), seed=42 + i)
+                      path_count=max(n, 1), seed=42 + i if mc_seed is None else mc_seed)
         if typ == 0:
             out.append(contract(EUROPEAN, **common))
         elif typ == 1:

@@ -300,6 +300,18 @@ int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry
 typedef struct mcdx_book mcdx_book;
 int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
                      mcdx_book** out);
+/* Flags of mcdx_book_create_ex. By default, FP32 contracts that share a seed, a payoff
+ * family (terminal, Asian, discrete barrier, lookback) and a step count are evaluated up to
+ * eight at a time on the same paths from one normal per step (common random numbers); each
+ * still equals its own mcdx_price_gbm bit for bit. */
+#define MCDX_BOOK_NO_SHARING 1u /* one contract per work item */
+#define MCDX_BOOK_GREEKS 2u     /* also delta, gamma, vega per contract (FP32 books; no
+                                   antithetic or bridge-corrected barriers): central
+                                   differences with the reference's default bumps
+                                   (src/greeks/finite_difference.cpp:31-62), the bumped
+                                   scenarios sharing each path's draws */
+int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
+                        uint32_t flags, mcdx_book** out);
 /* Time every schedule (item order x blocks per CU), keep the fastest, remember it for
  * books of the same shape; best_ms may be NULL. */
 int mcdx_book_autotune(mcdx_book* book, float* best_ms);
@@ -312,6 +324,9 @@ int mcdx_book_enqueue(mcdx_book* book, void* stream);
 /* Wait for the last enqueue and copy one result per contract into out[0..n). */
 int mcdx_book_results(mcdx_book* book, mcdx_result* out);
 int mcdx_book_price(mcdx_book* book, mcdx_result* out);
+/* Books created with MCDX_BOOK_GREEKS: out[3 i + {0, 1, 2}] = delta, gamma, vega of contract
+ * i, each the mean of its per-path central difference with that mean's standard error. */
+int mcdx_book_greeks(mcdx_book* book, mcdx_result* out);
 void mcdx_book_destroy(mcdx_book* book);
 
 #ifdef __cplusplus

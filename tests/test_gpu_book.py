@@ -93,3 +93,83 @@ def test_book_rejects_bad_contracts():
     h = C.c_void_p()
     assert mcdx.lib().mcdx_book_create(arr, 3, FP32, C.byref(h)) == mcdx.EINVAL
     assert mcdx.lib().mcdx_book_create(arr, 0, FP32, C.byref(h)) == mcdx.EINVAL
+
+
+def crn_book(n=120, seed=11):
+    """Common seed across the book, so contracts of one family and step count share draws."""
+    cs = mcdx.c4_book(n_contracts=n, seed=seed, log2_min=8, log2_max=14, mc_seed=5)
+    cs[0].product = mcdx.LOOKBACK
+    cs[0].strike_style = 1
+    for c in cs[1:4]:  # a lookback group with ragged path counts
+        c.product = mcdx.LOOKBACK
+        c.strike_style = 0
+        c.monitoring_points = cs[0].monitoring_points
+    cs[4].control_variate = 1 if cs[4].product == mcdx.ASIAN else 0
+    return cs
+
+
+@pytest.mark.parametrize("flags", [0, mcdx.Book.NO_SHARING])
+def test_shared_draw_book_equals_per_contract_calls_bitwise(flags):
+    cs = crn_book()
+    book = mcdx.Book(cs, FP32, flags)
+    got = book.price()
+    for c, g in zip(cs, got):
+        assert g == mcdx.price(c, FP32), (c.product, c.path_count)
+    for bpc, order in ((1, 0), (2, 1)):
+        book.set_schedule(bpc, order)
+        assert book.price() == got
+    book.close()
+
+
+def test_book_greeks_are_per_path_central_differences():
+    cs = crn_book(24, seed=3)
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    prices = book.price()
+    greeks = book.greeks()
+    for c, p, (d, g, v) in zip(cs, prices, greeks):
+        assert p == mcdx.price(c, FP32)
+        hs, hv = max(0.01 * c.spot, 1e-4), max(0.01 * c.vol, 1e-4)
+
+        def bumped(**kw):
+            b = mcdx.GbmContract.from_buffer_copy(c)
+            for k, x in kw.items():
+                setattr(b, k, x)
+            return mcdx.price(b, FP32)[0]
+        up, dn = bumped(spot=c.spot + hs), bumped(spot=c.spot - hs)
+        vu, vd = bumped(vol=c.vol + hv), bumped(vol=c.vol - hv)
+        # the mean of per-path differences equals the difference of the bumped prices
+        tol = 1e-9 * (abs(up) + abs(dn) + 1.0)
+        assert abs(d[0] - (up - dn) / (2 * hs)) <= tol / hs
+        assert abs(g[0] - (up - 2 * p[0] + dn) / hs ** 2) <= 4 * tol / hs ** 2
+        assert abs(v[0] - (vu - vd) / (2 * hv)) <= tol / hv
+        assert d[2] == g[2] == v[2] == c.path_count
+        assert d[1] >= 0 and v[1] >= 0
+    book.close()
+
+
+def test_book_greeks_european_match_black_scholes():
+    import mcd
+    cs = [mcdx.contract(mcdx.EUROPEAN, strike=k, vol=0.25, time=1.0, path_count=1 << 20, seed=9)
+          for k in (90.0, 100.0, 110.0)]
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    book.price()
+    for c, (d, g, v) in zip(cs, book.greeks()):
+        h = 1e-3
+        f = lambda s=c.spot, vol=c.vol: mcd.black_scholes_merton(s, c.strike, c.rate, 0.0, vol, 1.0,
+                                                                 mcd.OptionType.Call)
+        bd = (f(s=c.spot + h) - f(s=c.spot - h)) / (2 * h)
+        bg = (f(s=c.spot + h) - 2 * f() + f(s=c.spot - h)) / h ** 2
+        bv = (f(vol=c.vol + h) - f(vol=c.vol - h)) / (2 * h)
+        assert abs(d[0] - bd) <= 4 * d[1] + 1e-3
+        assert abs(g[0] - bg) <= 4 * g[1] + 2e-3
+        assert abs(v[0] - bv) <= 4 * v[1] + 0.05
+    book.close()
+
+
+def test_greeks_book_rejects_unsupported_contracts():
+    cs = crn_book(8)[:4]
+    cs[0].antithetic = 1
+    arr = (mcdx.GbmContract * 4)(*cs)
+    h = C.c_void_p()
+    assert mcdx.lib().mcdx_book_create_ex(arr, 4, FP32, 2, C.byref(h)) == mcdx.EINVAL
+    assert mcdx.lib().mcdx_book_create_ex(arr, 4, FP64, 2, C.byref(h)) == mcdx.EINVAL
