@@ -88,12 +88,6 @@ __host__ __device__ __forceinline__ uint32_t philox_bits(uint64_t seed, uint64_t
 #define MCDX_F32(x) ((float)(x))
 #define MCDX_F64(x) ((double)(x))
 
-// Branch thresholds on the raw bits: bits < kTailLo  <=>  (bits+0.5)/2^32 < 0.02425, and
-// bits > kTailHi  <=>  (bits+0.5)/2^32 > 0.97575, both exactly as evaluated in FP64
-// (0.02425*2^32 = 104152956.928, (1-0.02425)*2^32 = 4190814339.072 in FP64).
-constexpr uint32_t kTailLo = 104152957u;  // first bits value with u >= 0.02425
-constexpr uint32_t kTailHi = 4190814338u; // last bits value with u <= 1 - 0.02425
-
 // Hardware-rate FP32 transcendentals (v_rcp_f32, v_log_f32, v_sqrt_f32, v_exp_f32, each
 // within about 1 ulp over the ranges used here: arguments are normal floats, never denormal).
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -131,85 +125,94 @@ __host__ __device__ __forceinline__ double fast_rsqrt_f64(double x) {
     return y;
 }
 
-__host__ __device__ __forceinline__ bool is_tail(uint32_t bits) {
-    return bits < kTailLo || bits > kTailHi;
+// ---- FP32 inverse normal -------------------------------------------------------------------
+// Phi^-1(u), u = (bits + 0.5) / 2^32, for the FP32 generation mode: an approximation of this
+// library (the FP64 mode keeps the reference's Acklam + Halley). With m = min(u, 1 - u) and
+// y = log2(4 m (1 - m)) = log2(1 - 4 q^2), q = u - 1/2:
+//   z = q * P(y)                        when m >= m0 = 4.83e-4 (99.9% of draws),
+//   z = sign(q) * T(sqrt(-y) - kTailC)  otherwise,
+// with P (degree 9) and T (degree 8) weighted minimax polynomials fitted in FP64 against
+// sqrt(2) erfinv(2u - 1) (relative error 7.7e-8 and 6.4e-9; 1.7e-7 of (1 + |z|) end to end
+// in FP32 arithmetic). The change of variable removes the rational's division and moves
+// the branch point from Acklam's u = 0.02425 (4.85% of draws) to m0 (0.097%), so a wave
+// rarely takes the tail branch.
+constexpr uint32_t kTailInt = 2073811u; // tail iff min(bits, ~bits) < kTailInt, i.e. m < m0
+constexpr float kTailC = 4.285285692405615f;
+
+// min(bits, ~bits): (m * 2^32 - 0.5) for m = min(u, 1 - u), since 1 - u = (~bits + 0.5) / 2^32.
+__host__ __device__ __forceinline__ uint32_t fold_half(uint32_t bits) {
+    return bits ^ (uint32_t)((int32_t)bits >> 31);
 }
 
-// FP32-output Acklam on the raw Philox bits (Horner forms as normal.cpp:50-66, fused).
-// The central rational is evaluated in FP64: near the tail break its denominator cancels
-// from terms of size ~30 down to ~0.006, which FP32 Horner cannot carry (errors reach
-// 1e-4). q = u - 0.5 is formed exactly from the integer bits, and FP64 FMAs issue at the
-// FP32 rate on CDNA4, so the cost is a few conversions and a refined reciprocal. The
-// well-conditioned tail stays in FP32. central_f32 is meaningful only for !is_tail(bits).
-__host__ __device__ __forceinline__ float central_f32(uint32_t bits) {
-    MCDX_ACKLAM_CENTRAL(double, MCDX_F64)
-    const int32_t centred = (int32_t)(bits ^ 0x80000000u); // bits - 2^31, exact
-    const double q = ((double)centred + 0.5) * 0x1p-32;     // exact
-    const double r = q * q;
-    double num = __builtin_fma(A1, r, A2);
-    num = __builtin_fma(num, r, A3);
-    num = __builtin_fma(num, r, A4);
-    num = __builtin_fma(num, r, A5);
-    num = __builtin_fma(num, r, A6);
-    double den = __builtin_fma(B1, r, B2);
-    den = __builtin_fma(den, r, B3);
-    den = __builtin_fma(den, r, B4);
-    den = __builtin_fma(den, r, B5);
-    den = __builtin_fma(den, r, 1.0);
-    return (float)(num * q * fast_rcp_f64(den));
+__host__ __device__ __forceinline__ bool is_tail(uint32_t bits) { return fold_half(bits) < kTailInt; }
+
+// y = log2(4 m (1 - m)) from the folded bits (4m formed exactly scaled, one rounding each).
+__host__ __device__ __forceinline__ float half_log2(float mf) {
+    const float m4 = __builtin_fmaf(mf, 0x1p-30f, 0x1p-31f);
+    return fast_log2(__builtin_fmaf(-0.25f * m4, m4, m4));
+}
+
+// z carries the sign of q = u - 1/2: za XOR (sign bit of ~bits).
+__host__ __device__ __forceinline__ float signed_by_bits(float za, uint32_t bits) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, za) ^ (~bits & 0x80000000u));
+}
+
+// Meaningful only for !is_tail(bits); mi = fold_half(bits).
+__host__ __device__ __forceinline__ float central_f32(uint32_t bits, uint32_t mi) {
+    const float mf = (float)mi;
+    const float y = half_log2(mf);
+    float p = __builtin_fmaf(5.28648458342218e-10f, y, 2.0134757861001162e-08f);
+    p = __builtin_fmaf(p, y, 2.446661255817162e-07f);
+    p = __builtin_fmaf(p, y, -1.4628129463289952e-07f);
+    p = __builtin_fmaf(p, y, -2.6267098292009905e-05f);
+    p = __builtin_fmaf(p, y, -9.876400872599334e-05f);
+    p = __builtin_fmaf(p, y, 0.002176771406084299f);
+    p = __builtin_fmaf(p, y, 0.015678392723202705f);
+    p = __builtin_fmaf(p, y, -0.45487093925476074f);
+    p = __builtin_fmaf(p, y, 2.5066280364990234f);
+    return signed_by_bits(__builtin_fmaf(mf, -0x1p-32f, 0.5f) * p, bits); // |q| * P(y)
 }
 
 __host__ __device__ __forceinline__ float tail_f32(uint32_t bits) {
-    MCDX_ACKLAM_TAIL(float, MCDX_F32)
-    const bool lower = bits < kTailLo;
-    const uint32_t t = lower ? bits : ~bits; // ~bits = 2^32-1-bits: 1-u = (~bits+0.5)/2^32
-    const float ut = __builtin_fmaf((float)t, 0x1p-32f, 0x1p-33f);
-    // sqrt(-2 ln ut) = sqrt(-2 ln2 * log2 ut)
-    const float z = fast_sqrt(-1.38629436111989061883f * fast_log2(ut));
-    float num = __builtin_fmaf(C1, z, C2);
-    num = __builtin_fmaf(num, z, C3);
-    num = __builtin_fmaf(num, z, C4);
-    num = __builtin_fmaf(num, z, C5);
-    num = __builtin_fmaf(num, z, C6);
-    float den = __builtin_fmaf(D1, z, D2);
-    den = __builtin_fmaf(den, z, D3);
-    den = __builtin_fmaf(den, z, D4);
-    den = __builtin_fmaf(den, z, 1.0f);
-    const float res = num * fast_rcp(den);
-    return lower ? res : -res;
+    const float y = half_log2((float)fold_half(bits));
+    const float t = fast_sqrt(-y) - kTailC;
+    float p = __builtin_fmaf(-1.7373583887092536e-06f, t, 1.4796261211813544e-06f);
+    p = __builtin_fmaf(p, t, 1.7690343156573363e-05f);
+    p = __builtin_fmaf(p, t, -9.297743963543326e-05f);
+    p = __builtin_fmaf(p, t, 0.0003736739745363593f);
+    p = __builtin_fmaf(p, t, -0.001273712026886642f);
+    p = __builtin_fmaf(p, t, 0.0027184723876416683f);
+    p = __builtin_fmaf(p, t, 1.186617374420166f);
+    p = __builtin_fmaf(p, t, 4.813599109649658f);
+    return signed_by_bits(p, bits);
 }
 
 __host__ __device__ __forceinline__ float inv_normal_f32(uint32_t bits) {
-    return is_tail(bits) ? tail_f32(bits) : central_f32(bits);
+    const uint32_t mi = fold_half(bits);
+    return mi < kTailInt ? tail_f32(bits) : central_f32(bits, mi);
 }
 
-// B consecutive normal draws (draw0 .. draw0+B-1, stream 0) of one path. A wave64 holds a
-// tail draw (probability 4.85% each) on almost every step, so evaluating the tail branch
-// per step would cost the whole wave the tail's log/sqrt/rational nearly every time.
-// Instead every draw takes the central branch, and each lane then works through only its
-// own tail draws in the block: the wave loops max-over-lanes times (about 2.3 for B=8)
-// instead of about 7.7.
+// B consecutive normal draws (draw0 .. draw0+B-1, stream 0) of one path. Every draw takes
+// the central branch and the block keeps only the smallest folded value; a lane re-examines
+// its draws for tails only when that minimum is in the tail range (0.8% of lane-blocks for
+// B = 8 at 0.097% per draw), so the common path pays one min per draw for the tail test.
 template <int B>
 __host__ __device__ __forceinline__ void normal_block_f32(uint32_t plo, uint32_t phh,
                                                           uint32_t draw0, uint32_t k0,
                                                           uint32_t k1, float (&z)[B]) {
     uint32_t bits[B];
-    uint32_t pending = 0;
+    uint32_t lowest = 0xFFFFFFFFu;
 #pragma unroll
     for (int j = 0; j < B; ++j) {
         bits[j] = philox_w0(plo, phh, draw0 + (uint32_t)j, 0u, k0, k1);
-        z[j] = central_f32(bits[j]);
-        pending |= (uint32_t)is_tail(bits[j]) << j;
+        const uint32_t mi = fold_half(bits[j]);
+        z[j] = central_f32(bits[j], mi);
+        lowest = lowest < mi ? lowest : mi;
     }
-    while (pending) {
-        const int j = __builtin_ctz(pending);
-        uint32_t b = bits[0];
+    if (lowest < kTailInt) {
 #pragma unroll
-        for (int jj = 1; jj < B; ++jj) b = (j == jj) ? bits[jj] : b;
-        const float t = tail_f32(b);
-#pragma unroll
-        for (int jj = 0; jj < B; ++jj) z[jj] = (j == jj) ? t : z[jj];
-        pending &= pending - 1;
+        for (int j = 0; j < B; ++j)
+            if (is_tail(bits[j])) z[j] = tail_f32(bits[j]);
     }
 }
 
