@@ -41,8 +41,11 @@ OPS_DRAW = 80 + 33
 OPS_GBM = 3
 OPS_OBSERVE = {"asian": 3, "barrier_uo": 2, "lookback_float": 2}
 
+# HBM bytes per launch of each 2^24 x 252 path kernel: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE per dispatch, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by the same counters'
+# reading of a plain 1 GiB device copy (FETCH 492.7, 231.5, 222.5 KiB; WRITE 768 KiB).
+TRAFFIC_BYTES = {"asian": 1795482, "barrier_uo": 1260544, "lookback_float": 1242112}
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FMA)
-PEAK_ISSUE_TOPS = 39.3     # MI355X non-packed VALU lane-op issue rate (256 CU x 64 x 2.4 GHz)
 
 
 def contracts(mcdx, n):
@@ -412,11 +415,11 @@ def main():
         "bound": "valu", "kernel": f"gbm_tile_kernel[{dom}, fp32]",
         "achieved": round(achieved_tops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved_tops / PEAK_FP32_TFLOPS, 4),
-        "frac_of_issue_rate": round(achieved_tops / PEAK_ISSUE_TOPS, 4),
         "ops_per_path_step": OPS_DRAW + OPS_GBM + OPS_OBSERVE[dom],
         "launch_ms": round(per_kernel_ms[dom], 4),
         "per_launch_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
-        "traffic": None,
+        "traffic": TRAFFIC_BYTES[dom] if args.paths == N_PATHS else None,
+        "traffic_unit": "bytes per launch",
     }
 
     value = len(names) * args.paths / (ms_per_step / 1e3)

@@ -129,19 +129,41 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                                 : 0.0;
         } else {
             StateF32 s[kFuseMax];
-            bool na[kFuseMax], ng[kFuseMax];
+            // Step constants, read once per item for every member (A[i] is always a valid
+            // contract), so the step loop runs on registers: observe_f32's arithmetic with
+            // the member's own constants.
+            bool na[kFuseMax], ng[kFuseMax], up[kFuseMax];
+            float mu2[kFuseMax], sig2[kFuseMax], xb[kFuseMax];
 #pragma unroll
             for (int i = 0; i < kFuseMax; ++i) {
                 s[i] = StateF32{0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
                 na[i] = A[i]->average_style == 0 || A[i]->cv;
                 ng[i] = A[i]->average_style == 1 || A[i]->cv;
+                up[i] = A[i]->direction == 0;
+                mu2[i] = A[i]->mu2;
+                sig2[i] = A[i]->sig2;
+                xb[i] = A[i]->xb;
             }
             auto apply = [&](float z) {
 #pragma unroll
                 for (int i = 0; i < kFuseMax; ++i) {
                     if (i < kact) {
-                        s[i].x = s[i].x + fmaf(A[i]->sig2, z, A[i]->mu2);
-                        observe_f32<KIND>(*A[i], s[i], na[i], ng[i]);
+                        StateF32& t = s[i];
+                        t.x = t.x + fmaf(sig2[i], z, mu2[i]);
+                        if constexpr (KIND == kAsian) {
+                            if (na[i]) {
+                                const float u = fast_exp2(t.x);
+                                t.sum_u += (double)u;
+                                t.last_u = u;
+                            }
+                            if (ng[i]) t.sum_x += (double)t.x;
+                        } else if constexpr (KIND == kBarrier) {
+                            const bool crossed = up[i] ? t.x >= xb[i] : t.x <= xb[i];
+                            t.breached = t.breached || crossed;
+                        } else if constexpr (KIND == kLookback) {
+                            t.xmax = fmaxf(t.xmax, t.x);
+                            t.xmin = fminf(t.xmin, t.x);
+                        }
                     }
                 }
             };
