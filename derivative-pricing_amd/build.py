@@ -3,6 +3,7 @@
 Produces, in-tree:
   derivative-pricing_amd/lib/libmcd_hip.so   HIP kernels + the mcdx C ABI (include/mcdx.h)
   derivative-pricing_amd/python/mcd*.so       pybind11 `mcd` module (drop-in Python API)
+  derivative-pricing_amd/bin/mcd_cli          JSON command-line front end
 and asks oracle/Makefile for the CPU oracle (test infrastructure) and, where the reference
 tree exists, oracle/_ref.
 """
@@ -91,6 +92,17 @@ def build_pybind():
     os.replace(out + ".tmp", out)
 
 
+def build_cli():
+    """bin/mcd_cli: the JSON command-line front end over libmcd.so."""
+    bindir = os.path.join(PKG, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "mcd_cli")
+    run(["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+         os.path.join(PKG, "cli", "mcd_cli.cpp"), "-L", LIBDIR, "-lmcd", "-lmcd_hip",
+         "-Wl,-rpath,$ORIGIN/../lib", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     ref_root = os.environ.get("MCDX_REFERENCE_ROOT", "/root/reference")
@@ -102,6 +114,7 @@ def main():
     build_hip()
     build_host_lib()
     build_pybind()
+    build_cli()
     build_oracle()
 
 
