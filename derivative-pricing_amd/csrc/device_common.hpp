@@ -157,10 +157,8 @@ __host__ __device__ __forceinline__ float signed_by_bits(float za, uint32_t bits
     return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, za) ^ (~bits & 0x80000000u));
 }
 
-// Meaningful only for !is_tail(bits); mi = fold_half(bits).
-__host__ __device__ __forceinline__ float central_f32(uint32_t bits, uint32_t mi) {
-    const float mf = (float)mi;
-    const float y = half_log2(mf);
+// P(y): z = |q| * P(y) in the central range.
+__host__ __device__ __forceinline__ float central_poly(float y) {
     float p = __builtin_fmaf(5.28648458342218e-10f, y, 2.0134757861001162e-08f);
     p = __builtin_fmaf(p, y, 2.446661255817162e-07f);
     p = __builtin_fmaf(p, y, -1.4628129463289952e-07f);
@@ -169,12 +167,17 @@ __host__ __device__ __forceinline__ float central_f32(uint32_t bits, uint32_t mi
     p = __builtin_fmaf(p, y, 0.002176771406084299f);
     p = __builtin_fmaf(p, y, 0.015678392723202705f);
     p = __builtin_fmaf(p, y, -0.45487093925476074f);
-    p = __builtin_fmaf(p, y, 2.5066280364990234f);
-    return signed_by_bits(__builtin_fmaf(mf, -0x1p-32f, 0.5f) * p, bits); // |q| * P(y)
+    return __builtin_fmaf(p, y, 2.5066280364990234f);
 }
 
-__host__ __device__ __forceinline__ float tail_f32(uint32_t bits) {
-    const float y = half_log2((float)fold_half(bits));
+// Meaningful only for !is_tail(bits); mi = fold_half(bits).
+__host__ __device__ __forceinline__ float central_f32(uint32_t bits, uint32_t mi) {
+    const float mf = (float)mi;
+    return signed_by_bits(__builtin_fmaf(mf, -0x1p-32f, 0.5f) * central_poly(half_log2(mf)), bits);
+}
+
+// T: |z| in the tail range, from y.
+__host__ __device__ __forceinline__ float tail_poly(float y) {
     const float t = fast_sqrt(-y) - kTailC;
     float p = __builtin_fmaf(-1.7373583887092536e-06f, t, 1.4796261211813544e-06f);
     p = __builtin_fmaf(p, t, 1.7690343156573363e-05f);
@@ -183,13 +186,19 @@ __host__ __device__ __forceinline__ float tail_f32(uint32_t bits) {
     p = __builtin_fmaf(p, t, -0.001273712026886642f);
     p = __builtin_fmaf(p, t, 0.0027184723876416683f);
     p = __builtin_fmaf(p, t, 1.186617374420166f);
-    p = __builtin_fmaf(p, t, 4.813599109649658f);
-    return signed_by_bits(p, bits);
+    return __builtin_fmaf(p, t, 4.813599109649658f);
 }
 
+__host__ __device__ __forceinline__ float tail_f32(uint32_t bits) {
+    return signed_by_bits(tail_poly(half_log2((float)fold_half(bits))), bits);
+}
+
+// Central branch always; the tail polynomial only in lanes that need it (0.097% of draws).
 __host__ __device__ __forceinline__ float inv_normal_f32(uint32_t bits) {
     const uint32_t mi = fold_half(bits);
-    return mi < kTailInt ? tail_f32(bits) : central_f32(bits, mi);
+    float z = central_f32(bits, mi);
+    if (mi < kTailInt) z = tail_f32(bits);
+    return z;
 }
 
 // B consecutive normal draws (draw0 .. draw0+B-1, stream 0) of one path. Every draw takes

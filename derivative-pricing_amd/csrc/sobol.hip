@@ -282,42 +282,18 @@ __device__ __forceinline__ uint32_t sobol_x(const uint32_t* __restrict__ v, uint
     return xh ^ xl;
 }
 
-// u = x / 2^32 (sobol.cpp:96), inverse normal at the requested precision. In FP32 mode the
-// branch uses integer thresholds equal to the FP64 comparison: x/2^32 >= 0.02425 <=>
-// x >= 104152957, x/2^32 <= 0.97575 <=> x <= 4190814339.
+// u = x / 2^32 (sobol.cpp:96). FP32 mode uses the engine's FP32 inverse normal
+// (device_common.hpp) on m = min(u, 1 - u) = min(x, 2^32 - x) / 2^32, formed exactly from the
+// integer; Sobol points have no half-integer offset, so 4m = m_int * 2^-30.
 __device__ __forceinline__ float sobol_normal_f32(uint32_t x) {
-    if (x >= 104152957u && x <= 4190814339u) {
-        MCDX_ACKLAM_CENTRAL(double, MCDX_F64)
-        const double q = (double)(int32_t)(x ^ 0x80000000u) * 0x1p-32; // u - 0.5, exact
-        const double r = q * q;
-        double num = __builtin_fma(A1, r, A2);
-        num = __builtin_fma(num, r, A3);
-        num = __builtin_fma(num, r, A4);
-        num = __builtin_fma(num, r, A5);
-        num = __builtin_fma(num, r, A6);
-        double den = __builtin_fma(B1, r, B2);
-        den = __builtin_fma(den, r, B3);
-        den = __builtin_fma(den, r, B4);
-        den = __builtin_fma(den, r, B5);
-        den = __builtin_fma(den, r, 1.0);
-        return (float)(num * q * fast_rcp_f64(den));
-    }
-    MCDX_ACKLAM_TAIL(float, MCDX_F32)
-    const bool lower = x < 104152957u;
-    // lower: u = x / 2^32; upper: 1 - u = (2^32 - x) / 2^32 = (~x + 1) / 2^32
-    const float ut = lower ? (float)x * 0x1p-32f : ((float)(~x) + 1.0f) * 0x1p-32f;
-    const float z = fast_sqrt(-1.38629436111989061883f * fast_log2(ut));
-    float num = __builtin_fmaf(C1, z, C2);
-    num = __builtin_fmaf(num, z, C3);
-    num = __builtin_fmaf(num, z, C4);
-    num = __builtin_fmaf(num, z, C5);
-    num = __builtin_fmaf(num, z, C6);
-    float den = __builtin_fmaf(D1, z, D2);
-    den = __builtin_fmaf(den, z, D3);
-    den = __builtin_fmaf(den, z, D4);
-    den = __builtin_fmaf(den, z, 1.0f);
-    const float res = num * fast_rcp(den);
-    return lower ? res : -res;
+    const bool upper = x >= 0x80000000u;
+    const uint32_t mi = upper ? 0u - x : x; // 2^32 - x on the upper half
+    const float mf = (float)mi;
+    const float m4 = mf * 0x1p-30f;
+    const float y = fast_log2(__builtin_fmaf(-0.25f * m4, m4, m4));
+    const float za = mi < kTailInt ? tail_poly(y)
+                                   : __builtin_fmaf(mf, -0x1p-32f, 0.5f) * central_poly(y);
+    return upper ? za : -za;
 }
 
 struct QmcArgs {
