@@ -578,6 +578,154 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
     }
 }
 
+// ---- American Greeks by frozen-policy replay ------------------------------------------------
+// Central differences of frozen-policy prices (reprice_against_frozen_policy,
+// lsm.cpp:151-213, as the reference's tests/lsm_test.cpp:138-170 forms its delta): the QR
+// policy fitted at the base parameters exercises five scenarios of each path -- base, spot
+// up/down, vol up/down -- generated from the same draws in one pass, and the per-path
+// differences accumulate like any Monte Carlo sample.
+struct AmGreekArgs {
+    double spot_up, spot_dn;             // FP64 spot scenarios
+    double drift_u, vsd_u, drift_d, vsd_d; // FP64 vol scenarios (per-date constants)
+    float s0_up, s0_dn;                  // FP32 spot scenarios (shared log2 path)
+    float mu2_u, sig2_u, mu2_d, sig2_d;  // FP32 vol scenarios
+    double inv_2hs, inv_hs2, inv_2hv;
+    Welford* parts;                      // [4][tiles + 1]: price, delta, gamma, vega
+    uint64_t tiles;
+};
+
+// One scenario's stop: whether it has exercised, the date and the exercise value.
+struct Stop {
+    bool done;
+    int date;
+    double value;
+};
+
+// One scenario's exercise by the frozen policy at `date` (lsm.cpp:110-122).
+__device__ __forceinline__ void frozen_visit(const LsmArgs& a, Stop& st, int date, double s) {
+    if (st.done) return;
+    const double ex = intrinsic_v(s, a.strike, a.type);
+    if (date == a.m) { // lsm.cpp:76-81
+        st = Stop{true, date, ex};
+        return;
+    }
+    if (ex > 0.0 && a.has[date]) {
+        double b[4];
+        laguerre4(s / a.strike, b);
+        const double* beta = a.coeffs + 4 * date;
+        double cont = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
+        if (ex > cont) st = Stop{true, date, ex};
+    }
+}
+
+// Cash flow of a stopped scenario discounted one multiplication per date, in backward
+// induction's order.
+__device__ __forceinline__ double discounted(const LsmArgs& a, const Stop& st) {
+    double c = st.value;
+    for (int i = 0; i < st.date; ++i) c *= a.disc;
+    return c;
+}
+
+// One path per thread; a block of 1024 threads is one canonical tile, partials rebuilt in
+// the canonical order as lsm_policy_kernel does.
+template <bool FP64>
+__global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a, AmGreekArgs g) {
+    __shared__ double sh[4][kTilePaths];
+    __shared__ Welford lds[4][4];
+    const uint64_t tile = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint64_t p = tile * kTilePaths + t;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    Stop c0 = {false, 0, 0.0}, c1 = c0, c2 = c0, c3 = c0, c4 = c0;
+    auto all_done = [&] { return c0.done && c1.done && c2.done && c3.done && c4.done; };
+    if (p < a.n_paths) {
+        const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
+        if constexpr (FP64) {
+            double s0 = a.spot, s1 = g.spot_up, s2 = g.spot_dn, s3 = a.spot, s4 = a.spot;
+            for (int step = 0; step < a.m && !all_done(); ++step) {
+                const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+                const double f = exp(a.drift + a.vsd * z); // the store kernel's update per scenario
+                s0 = s0 * f;
+                s1 = s1 * f;
+                s2 = s2 * f;
+                s3 = s3 * exp(g.drift_u + g.vsd_u * z);
+                s4 = s4 * exp(g.drift_d + g.vsd_d * z);
+                const int d = step + 1;
+                frozen_visit(a, c0, d, s0);
+                frozen_visit(a, c1, d, s1);
+                frozen_visit(a, c2, d, s2);
+                frozen_visit(a, c3, d, s3);
+                frozen_visit(a, c4, d, s4);
+            }
+        } else {
+            float x = 0.f, xu = 0.f, xd = 0.f;
+            const float s0 = (float)a.spot;
+            for (int step = 0; step < a.m && !all_done(); ++step) {
+                const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
+                x = x + __builtin_fmaf(a.sig2, z, a.mu2);
+                xu = xu + __builtin_fmaf(g.sig2_u, z, g.mu2_u);
+                xd = xd + __builtin_fmaf(g.sig2_d, z, g.mu2_d);
+                const float e = fast_exp2(x);
+                const int d = step + 1;
+                frozen_visit(a, c0, d, (double)(s0 * e));
+                frozen_visit(a, c1, d, (double)(g.s0_up * e));
+                frozen_visit(a, c2, d, (double)(g.s0_dn * e));
+                frozen_visit(a, c3, d, (double)(s0 * fast_exp2(xu)));
+                frozen_visit(a, c4, d, (double)(s0 * fast_exp2(xd)));
+            }
+        }
+    }
+    const double v[5] = {discounted(a, c0), discounted(a, c1), discounted(a, c2), discounted(a, c3),
+                         discounted(a, c4)};
+    sh[0][t] = v[0];
+    sh[1][t] = (v[1] - v[2]) * g.inv_2hs;
+    sh[2][t] = ((v[1] - 2.0 * v[0]) + v[2]) * g.inv_hs2;
+    sh[3][t] = (v[3] - v[4]) * g.inv_2hv;
+    __syncthreads();
+    Welford w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = Welford{0, 0.0, 0.0};
+    if (t < (uint32_t)kTileLanes) {
+#pragma unroll
+        for (int k = 0; k < kPathsPerLane; ++k) {
+            const uint32_t i = (uint32_t)(k * kTileLanes) + t;
+            if (tile * kTilePaths + i >= a.n_paths) break;
+            const double rk = 1.0 / (double)(k + 1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double xv = sh[q][i];
+                w[q].n += 1;
+                const double dd = xv - w[q].mean;
+                w[q].mean += dd * rk;
+                w[q].m2 += dd * (xv - w[q].mean);
+            }
+        }
+    }
+    const int lane = t & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Welford o;
+            o.n = __shfl_down(w[q].n, off, 64);
+            o.mean = __shfl_down(w[q].mean, off, 64);
+            o.m2 = __shfl_down(w[q].m2, off, 64);
+            if ((lane & (2 * off - 1)) == 0) w[q] = welford_merge(w[q], o);
+        }
+    }
+    if (lane == 0 && t < (uint32_t)kTileLanes)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lds[q][t >> 6] = w[q];
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            g.parts[(uint64_t)q * (g.tiles + 1) + tile] =
+                welford_merge(welford_merge(lds[q][0], lds[q][1]), welford_merge(lds[q][2], lds[q][3]));
+}
+
 namespace {
 
 struct LsmRun {
@@ -879,6 +1027,79 @@ void boundary_gap(const float* table, const double* coeffs, const uint8_t* has, 
 
 } // namespace
 
+int lsm_am_greeks(double spot, double strike, double rate, double carry, double vol, double time,
+                  uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, double hs,
+                  double hv, mcdx_result* out4) {
+    if (!out4 || !(spot > 0.0) || !(vol > 0.0)) return MCDX_EINVAL;
+    if (!(hs > 0.0)) hs = fmax(spot * 0.01, 1e-4); // default_bump_sizes, finite_difference.cpp:31-38
+    if (!(hv > 0.0)) hv = fmax(vol * 0.01, 1e-4);
+    if (!(hs < spot) || !(hv < vol)) return MCDX_EINVAL;
+    ThreadCtx* ctx;
+    int st = thread_ctx(&ctx);
+    if (st) return st;
+    const uint64_t tiles = tiles_for(n);
+    char* ex = nullptr;
+    LsmRun r;
+    if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r,
+                        4 * (tiles + 1) * sizeof(Welford), &ex)))
+        return st;
+    if ((st = ensure_pinned(ctx, 4 * sizeof(Welford) + 64))) return st;
+    const bool fp64 = precision == MCDX_PREC_FP64;
+    hipStream_t s = ctx->stream;
+    hipError_t e = fp64 ? enqueue_paths<true>(r, s) : enqueue_paths<false>(r, s);
+    if (e == hipSuccess) e = fp64 ? enqueue_backward<true>(r, true, s) : enqueue_backward<false>(r, true, s);
+    if (e != hipSuccess) return status_of(e);
+    // Scenario constants exactly as the store kernel derives them for those parameters.
+    const double dt = time / (double)m, log2e = 1.44269504088896340735992468100189214;
+    AmGreekArgs g{};
+    g.spot_up = spot + hs;
+    g.spot_dn = spot - hs;
+    const double vu = vol + hv, vd = vol - hv;
+    g.drift_u = (rate - carry - 0.5 * vu * vu) * dt;
+    g.vsd_u = vu * sqrt(dt);
+    g.drift_d = (rate - carry - 0.5 * vd * vd) * dt;
+    g.vsd_d = vd * sqrt(dt);
+    g.s0_up = (float)g.spot_up;
+    g.s0_dn = (float)g.spot_dn;
+    g.mu2_u = (float)(g.drift_u * log2e);
+    g.sig2_u = (float)(g.vsd_u * log2e);
+    g.mu2_d = (float)(g.drift_d * log2e);
+    g.sig2_d = (float)(g.vsd_d * log2e);
+    g.inv_2hs = 1.0 / (2.0 * hs);
+    g.inv_hs2 = 1.0 / (hs * hs);
+    g.inv_2hv = 1.0 / (2.0 * hv);
+    g.parts = (Welford*)ex;
+    g.tiles = tiles;
+    if (fp64)
+        hipLaunchKernelGGL(lsm_am_greeks_kernel<true>, dim3((unsigned)tiles), dim3(kTilePaths), 0, s, r.a, g);
+    else
+        hipLaunchKernelGGL(lsm_am_greeks_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths), 0, s, r.a, g);
+    e = hipGetLastError();
+    for (int q = 0; q < 4 && e == hipSuccess; ++q) {
+        Welford* part = g.parts + (uint64_t)q * (tiles + 1);
+        e = launch_final_merge(part, nullptr, tiles, 1, part + tiles, s);
+    }
+    Welford* h = (Welford*)ctx->pinned;
+    for (int q = 0; q < 4 && e == hipSuccess; ++q)
+        e = hipMemcpyAsync(h + q, g.parts + (uint64_t)q * (tiles + 1) + tiles, sizeof(Welford),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h + 4, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    const mcdx_result fitted = finish(h[4]);
+    const double inc = intrinsic_v(spot, strike, type);
+    if (inc > fitted.price) { // frozen inception exercise (lsm.cpp:141-148, :207-211)
+        const double up = intrinsic_v(spot + hs, strike, type), dn = intrinsic_v(spot - hs, strike, type);
+        out4[0] = mcdx_result{inc, 0.0, fitted.path_count};
+        out4[1] = mcdx_result{(up - dn) / (2.0 * hs), 0.0, fitted.path_count};
+        out4[2] = mcdx_result{((up - 2.0 * inc) + dn) / (hs * hs), 0.0, fitted.path_count};
+        out4[3] = mcdx_result{0.0, 0.0, fitted.path_count};
+        return MCDX_OK;
+    }
+    for (int q = 0; q < 4; ++q) out4[q] = finish(h[q]);
+    return MCDX_OK;
+}
+
 int lsm_learned(double spot, double strike, double rate, double carry, double vol, double time,
                 uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision,
                 const mcdx_lsm_learn_options* o, float* weights_out, double* coeffs_out,
@@ -1058,6 +1279,14 @@ int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carr
     return lsm_reprice(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
                        path_count, seed, coefficients, has_coefficients, exercise_at_inception,
                        precision, out);
+}
+
+int mcdx_lsm_american_greeks(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, int32_t monitoring_points,
+                            uint64_t path_count, uint64_t seed, int precision, double spot_bump,
+                            double vol_bump, mcdx_result* out4) {
+    return lsm_am_greeks(spot, strike, rate, carry_yield, vol, time, type, monitoring_points,
+                         path_count, seed, precision, spot_bump, vol_bump, out4);
 }
 
 void mcdx_lsm_learn_default_options(mcdx_lsm_learn_options* o) {

@@ -117,6 +117,8 @@ SIGNATURES = {
     "mcdx_monte_carlo_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(Result)]),
     "mcdx_lsm_reprice_frozen": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(dbl), P(u8), u8,
                                                       C.c_int, P(Result)]),
+    "mcdx_lsm_american_greeks": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, dbl, dbl,
+                                                        P(Result)]),
     "mcdx_pathwise_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Result)]),
     "mcdx_pathwise_asian": (C.c_int, [dbl] * 6 + [u8, u8, u8, i32, u64, u64, P(Result)]),
     "mcdx_likelihood_ratio_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Result)]),
@@ -260,6 +262,14 @@ def lsm_reprice(spot, strike, rate, carry, vol, time, type, m, n, seed, coeffs, 
                                         co.ctypes.data_as(P(dbl)), hs.ctypes.data_as(P(u8)),
                                         inception, precision, C.byref(r)), "mcdx_lsm_reprice_frozen")
     return r.price
+
+
+def lsm_greeks(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, hs=0.0, hv=0.0):
+    """[(value, se, paths)] for price, delta, gamma, vega (frozen-policy central differences)."""
+    out = (Result * 4)()
+    check(lib().mcdx_lsm_american_greeks(spot, strike, rate, carry, vol, time, type, m, n, seed,
+                                         precision, hs, hv, out), "mcdx_lsm_american_greeks")
+    return [(r.price, r.standard_error, r.path_count) for r in out]
 
 
 def greeks(name, *args, n_out):

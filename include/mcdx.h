@@ -189,6 +189,17 @@ int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carr
                             const uint8_t* has_coefficients, uint8_t exercise_at_inception,
                             int precision, mcdx_result* out);
 
+/* American Greeks by central differences of frozen-policy prices (reprice_against_frozen_policy,
+ * src/pricers/lsm.cpp:151-213, as the reference's tests/lsm_test.cpp:138-170 forms delta):
+ * one QR-LSM fit at the base parameters, then the base, spot +/- spot_bump and vol +/- vol_bump
+ * scenarios of every path, from the same draws, exercised by the frozen policy in one
+ * streaming pass. out4 = price, delta, gamma, vega, each with the standard error of its
+ * per-path estimator. Bumps <= 0 select default_bump_sizes (1% of spot, 1% of vol). */
+int mcdx_lsm_american_greeks(double spot, double strike, double rate, double carry_yield,
+                            double vol, double time, uint8_t type, int32_t monitoring_points,
+                            uint64_t path_count, uint64_t seed, int precision, double spot_bump,
+                            double vol_bump, mcdx_result* out4);
+
 /* ---- Greeks (src/greeks), FP64 arithmetic as the reference ---- */
 /* pathwise_european (src/greeks/pathwise.cpp:28-62): out3 = delta, vega, rho, each as
  * {value, standard_error, path_count}. */

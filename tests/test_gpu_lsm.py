@@ -78,3 +78,52 @@ def test_large_path_count():
     p, se, _, has, _ = mcdx.lsm(100, 100, 0.06, 0, 0.2, 1, 1, 50, 1 << 22, 7, FP32)
     assert has[1:50].all()
     assert abs(p - crr) < 3 * se + 0.05
+
+
+@pytest.mark.parametrize("prec", [FP32, FP64])
+def test_american_greeks_are_frozen_policy_differences(prec):
+    """One pass over five scenarios equals separate frozen-policy replays
+    (reprice_against_frozen_policy) of the fitted policy at the bumped parameters."""
+    args = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 25)
+    n, seed = 30_011, 5
+    price, delta, gamma, vega = mcdx.lsm_greeks(*args, n, seed, prec)
+    p, se, co, has, inc = mcdx.lsm(*args, n, seed, prec)
+    assert price[0] == p and price[2] == n
+    hs, hv = 1.0, 0.002
+
+    def frozen(s0=100.0, vol=0.2):
+        a = (s0, 100.0, 0.06, 0.0, vol, 1.0, 1, 25)
+        return mcdx.lsm_reprice(*a, n, seed, co, has, inc, prec)
+    up, dn = frozen(s0=100 + hs), frozen(s0=100 - hs)
+    vu, vd = frozen(vol=0.2 + hv), frozen(vol=0.2 - hv)
+    assert abs(delta[0] - (up - dn) / (2 * hs)) <= 1e-9 * (1 + abs(up) + abs(dn)) / hs
+    assert abs(gamma[0] - (up - 2 * p + dn) / hs ** 2) <= 4e-9 * (1 + abs(up) + abs(dn)) / hs ** 2
+    assert abs(vega[0] - (vu - vd) / (2 * hv)) <= 1e-9 * (1 + abs(vu) + abs(vd)) / hv
+    assert delta[1] > 0 and vega[1] > 0
+
+
+def test_american_greeks_near_binomial():
+    price, delta, gamma, vega = mcdx.lsm_greeks(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50,
+                                                1 << 20, 42, FP32)
+    crr = lambda s=100.0, v=0.2: L.orc_crr_american(s, 100, 0.06, 0, v, 1, 4000, 1)  # noqa: E731
+    d = (crr(s=101.0) - crr(s=99.0)) / 2.0
+    g = (crr(s=101.0) - 2 * crr() + crr(s=99.0)) / 1.0
+    v = (crr(v=0.202) - crr(v=0.198)) / 0.004
+    assert abs(delta[0] - d) < 4 * delta[1] + 0.01, (delta, d)
+    assert abs(gamma[0] - g) < 4 * gamma[1] + 0.005, (gamma, g)
+    assert abs(vega[0] - v) < 4 * vega[1] + 0.5, (vega, v)
+
+
+def test_frozen_policy_delta_is_less_noisy_than_refitting():
+    """The reference's tests/lsm_test.cpp:138-170 claim, on the device: across 40 seeds the
+    frozen-boundary delta varies less than refitting the regression at each bump."""
+    args = (100.0, 100.0, 0.05, 0.0, 0.25, 1.0, 1, 20)
+    h, n = 0.1, 5000
+    frozen, naive = [], []
+    for trial in range(40):
+        seed = 909 + trial * 1013
+        frozen.append(mcdx.lsm_greeks(*args, n, seed, FP64, hs=h)[1][0])
+        up = mcdx.lsm(100.0 + h, *args[1:], n, seed, FP64)[0]
+        dn = mcdx.lsm(100.0 - h, *args[1:], n, seed, FP64)[0]
+        naive.append((up - dn) / (2 * h))
+    assert np.std(frozen, ddof=1) < np.std(naive, ddof=1)
