@@ -305,6 +305,10 @@ struct QmcArgs {
     const BridgeOp* ops;   // bridge program (Asian)
     int32_t n_ops, m, slots;
     uint8_t type, strike_style, european;
+    uint8_t product;       // 0 arithmetic Asian, 1 barrier, 2 lookback (over the bridge)
+    uint8_t direction, knock;
+    double barrier, rebate;
+    float barrier_rel;     // FP32: H / S0
     Welford* partials;
 };
 
@@ -327,8 +331,12 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
         }
         return a.discount * smax_d(phi * (st - a.strike), 0.0);
     }
-    // Asian over the bridge (qmc.cpp:44-54): replay the depth-first program.
+    // Asian over the bridge (qmc.cpp:44-54): replay the depth-first program. Barriers and
+    // lookbacks observe the same dates (every date once; breach, maximum and minimum do not
+    // depend on the visiting order; the program's first point is the last date).
     double sum = 0.0, last = 0.0;
+    double mx = FP64 ? a.spot : 1.0, mn = mx; // lookback extrema seeded at S0 (S/S0 in FP32)
+    bool breached = false;
     if constexpr (FP64) lds_d[tid] = 0.0;
     else lds_f[tid] = 0.f;
     for (int k = 0; k < a.n_ops; ++k) {
@@ -344,6 +352,11 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
             const double s = a.spot * exp(op.drift_t + a.vol * w);
             sum += s;
             if (k == 0) last = s;
+            if (a.product == 1) breached = breached || (a.direction == 0 ? s >= a.barrier : s <= a.barrier);
+            if (a.product == 2) {
+                mx = smax_d(mx, s);
+                mn = (s < mn) ? s : mn;
+            }
         } else {
             const float z = sobol_normal_f32(x);
             const float wl = lds_f[op.sl * nthreads + tid];
@@ -353,11 +366,32 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
             const float su = fast_exp2(__builtin_fmaf(a.vol2, w, op.drift2_f));
             sum += (double)su;
             if (k == 0) last = (double)su;
+            if (a.product == 1)
+                breached = breached || (a.direction == 0 ? su >= a.barrier_rel : su <= a.barrier_rel);
+            if (a.product == 2) {
+                mx = smax_d(mx, (double)su);
+                mn = ((double)su < mn) ? (double)su : mn;
+            }
         }
     }
     if constexpr (!FP64) {
         sum *= a.spot;
         last *= a.spot;
+        mx *= a.spot;
+        mn *= a.spot;
+    }
+    if (a.product == 1) { // barrier.hpp:32-37
+        const double vanilla = smax_d(phi * (last - a.strike), 0.0);
+        const bool pays = a.knock == 1 ? !breached : breached;
+        return a.discount * (pays ? vanilla : a.rebate);
+    }
+    if (a.product == 2) { // lookback.hpp:32-35, 54-56
+        double r;
+        if (a.strike_style == 0)
+            r = a.type == 0 ? smax_d(mx - a.strike, 0.0) : smax_d(a.strike - mn, 0.0);
+        else
+            r = a.type == 0 ? (last - mn) : (mx - last);
+        return a.discount * r;
     }
     const double avg = sum / (double)a.m;
     const double ref = a.strike_style == 0 ? (avg - a.strike) : (last - avg);
@@ -495,11 +529,48 @@ int qmc_asian(double spot, double strike, double rate, double carry, double vol,
     return qmc_run(a, precision, ops, slots, out);
 }
 
+int qmc_path(const mcdx_gbm_contract* c, int precision, mcdx_result* out) {
+    if (!out || !c) return MCDX_EINVAL;
+    if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
+    const int32_t m = c->monitoring_points;
+    if (c->type > 1 || c->strike_style > 1 || c->direction > 1 || c->knock > 1 || m < 1 ||
+        m > (int32_t)kSobolDims || c->path_count > (1ull << 32) - 2)
+        return MCDX_EINVAL;
+    uint8_t product;
+    if (c->product == MCDX_ASIAN && c->average_style == 0) product = 0;
+    else if (c->product == MCDX_BARRIER && !c->brownian_bridge) product = 1;
+    else if (c->product == MCDX_LOOKBACK) product = 2;
+    else return MCDX_EINVAL;
+    if (c->path_count == 0) {
+        *out = mcdx_result{NAN, 0.0, 0};
+        return MCDX_OK;
+    }
+    std::vector<BridgeOp> ops;
+    const int slots = build_bridge(m, c->spot, c->rate, c->carry_yield, c->vol, c->time, ops);
+    if (slots < 0) return MCDX_EINVAL;
+    QmcArgs a = qmc_args(c->spot, c->strike, c->rate, c->carry_yield, c->vol, c->time, c->type,
+                         c->path_count);
+    a.m = m;
+    a.strike_style = c->strike_style;
+    a.european = 0;
+    a.product = product;
+    a.direction = c->direction;
+    a.knock = c->knock;
+    a.barrier = c->barrier;
+    a.rebate = c->rebate;
+    a.barrier_rel = (float)(c->barrier / c->spot);
+    return qmc_run(a, precision, ops, slots, out);
+}
+
 } // namespace mcdx
 
 using namespace mcdx;
 
 extern "C" {
+
+int mcdx_qmc_sobol_path(const mcdx_gbm_contract* contract, int precision, mcdx_result* out) {
+    return qmc_path(contract, precision, out);
+}
 
 uint32_t mcdx_sobol_max_dimensions(void) { return kSobolRefDims; }
 uint32_t mcdx_sobol_extended_dimensions(void) { return kSobolDims; }

@@ -107,6 +107,7 @@ SIGNATURES = {
     "mcdx_qmc_sobol_asian": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, P(dbl)]),
     "mcdx_qmc_sobol_asian_extended": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, C.c_int,
                                                             P(Result)]),
+    "mcdx_qmc_sobol_path": (C.c_int, [P(GbmContract), C.c_int, P(Result)]),
     "mcdx_heston_qe_european": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, u64, P(Result)]),
     "mcdx_heston_qe_strikes": (C.c_int, [P(HestonParams), P(dbl), i32, u8, i32, u64, u64,
                                          C.c_int, P(Result)]),
@@ -222,6 +223,12 @@ def qmc_european(spot, strike, rate, carry, vol, time, type, n, precision):
     r = Result()
     check(lib().mcdx_qmc_sobol_european_p(spot, strike, rate, carry, vol, time, type, n,
                                           precision, C.byref(r)), "mcdx_qmc_sobol_european_p")
+    return r.price
+
+
+def qmc_path(c, precision):
+    r = Result()
+    check(lib().mcdx_qmc_sobol_path(C.byref(c), precision, C.byref(r)), "mcdx_qmc_sobol_path")
     return r.price
 
 

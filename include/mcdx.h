@@ -149,6 +149,14 @@ int mcdx_qmc_sobol_asian_extended(double spot, double strike, double rate, doubl
                                   double vol, double time, uint8_t type, uint8_t strike_style,
                                   int32_t monitoring_points, uint64_t path_count, int precision,
                                   mcdx_result* out);
+/* Sobol + Brownian-bridge QMC for a path-dependent contract: arithmetic Asian, discrete
+ * barrier (no bridge correction) or lookback, monitoring_points 1..1024 (one Sobol
+ * dimension per date), Sobol indices 1..path_count. Composes the reference's pieces
+ * (sobol.cpp:83-97, brownian_bridge_path.cpp:20-77, the payoff headers, qmc.cpp:44-56);
+ * the reference prices only the arithmetic Asian this way. The seed is ignored; the
+ * standard_error field is the points' sample standard error, which does not measure the
+ * error of a deterministic point set. */
+int mcdx_qmc_sobol_path(const mcdx_gbm_contract* contract, int precision, mcdx_result* out);
 
 /* ---- Heston QE (src/pricers/heston.cpp) ---- */
 /* HestonParams, include/mcd/models/heston.hpp:8-18 (same field order). */

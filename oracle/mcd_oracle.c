@@ -554,6 +554,50 @@ double orc_qmc_sobol_asian(double spot, double strike, double rate, double carry
     return sum / (double)path_count;
 }
 
+/* QMC over Sobol + Brownian bridge for the path-dependent products: the reference's pieces
+ * composed (sobol.cpp:83-97 points, brownian_bridge_path.cpp:20-77 spots, the payoffs of
+ * asian.hpp / barrier.hpp / lookback.hpp observed at every date, the plain mean of
+ * qmc.cpp:44-56). The reference prices only the arithmetic Asian this way; this is the
+ * engine's extension to barriers and lookbacks. Arithmetic averages only. */
+double orc_qmc_sobol_path(const orc_product* p, uint64_t path_count) {
+    const int m = p->monitoring_points;
+    const double phi = p->type == 0 ? 1.0 : -1.0;
+    const double discount = exp(-p->rate * p->time);
+    double* z = (double*)malloc(sizeof(double) * (size_t)m);
+    double* sp = (double*)malloc(sizeof(double) * (size_t)m);
+    double total = 0.0;
+    for (uint64_t i = 1; i <= path_count; ++i) {
+        for (int d = 0; d < m; ++d) z[d] = orc_inverse_normal_cdf(orc_sobol_point((unsigned)d, i));
+        orc_brownian_bridge_path(p->spot, p->rate, p->carry_yield, p->vol, p->time, m, z, sp);
+        double sum = 0.0, mx = p->spot, mn = p->spot;
+        int breached = 0;
+        for (int d = 0; d < m; ++d) {
+            sum += sp[d];
+            mx = smax(mx, sp[d]);
+            mn = smin(mn, sp[d]);
+            breached = breached || (p->direction == 0 ? sp[d] >= p->barrier : sp[d] <= p->barrier);
+        }
+        const double last = sp[m - 1];
+        double pay;
+        if (p->product == ORC_ASIAN) {
+            const double avg = sum / (double)m;
+            pay = smax(phi * (p->strike_style == 0 ? (avg - p->strike) : (last - avg)), 0.0);
+        } else if (p->product == ORC_BARRIER) {
+            const double vanilla = smax(phi * (last - p->strike), 0.0);
+            const int pays = p->knock == 1 ? !breached : breached;
+            pay = pays ? vanilla : p->rebate;
+        } else if (p->strike_style == 0) {
+            pay = p->type == 0 ? smax(mx - p->strike, 0.0) : smax(p->strike - mn, 0.0);
+        } else {
+            pay = p->type == 0 ? (last - mn) : (mx - last);
+        }
+        total += discount * pay;
+    }
+    free(z);
+    free(sp);
+    return total / (double)path_count;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Heston: src/pricers/heston.cpp:26-156                                     */
 /* ------------------------------------------------------------------------ */

@@ -91,6 +91,9 @@ void orc_brownian_bridge_path(double spot, double rate, double carry, double vol
                               int n, const double* normals, double* spots);
 double orc_qmc_sobol_european(double spot, double strike, double rate, double carry, double vol,
                               double time, int type, uint64_t path_count);
+/* QMC over Sobol + bridge for Asian (arithmetic), barrier and lookback products: the
+ * reference's pieces composed (no reference counterpart for barriers and lookbacks). */
+double orc_qmc_sobol_path(const orc_product* p, uint64_t path_count);
 double orc_qmc_sobol_asian(double spot, double strike, double rate, double carry, double vol,
                            double time, int type, int strike_style, int monitoring_points,
                            uint64_t path_count);

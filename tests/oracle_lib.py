@@ -80,6 +80,7 @@ def lib():
             "orc_brownian_bridge_path": (None, [dbl] * 5 + [i32, P(dbl), P(dbl)]),
             "orc_qmc_sobol_european": (dbl, [dbl] * 6 + [i32, u64]),
             "orc_qmc_sobol_asian": (dbl, [dbl] * 6 + [i32, i32, i32, u64]),
+            "orc_qmc_sobol_path": (dbl, [P(Product), u64]),
             "orc_qe_variance_step": (dbl, [dbl] * 6),
             "orc_heston_path_sample": (dbl, [P(Heston), dbl, i32, i32, u64, u64]),
             "orc_heston_qe_european": (Result, [P(Heston), dbl, i32, i32, u64, u64]),

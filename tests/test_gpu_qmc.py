@@ -4,6 +4,7 @@ Bars: Sobol integers bit-exact for every dimension of the extended table (dims 0
 reference's); FP64 QMC prices within 1e-12 relative of the oracle (the point sum is merged
 in a different order); FP32 generation within 2e-6 relative of FP64 on the same points.
 """
+import ctypes as C
 import json
 import os
 
@@ -78,3 +79,48 @@ def test_qmc_asian_252_converges_to_mc():
     c = mcdx.contract(mcdx.ASIAN, monitoring_points=252, path_count=1 << 20, seed=3)
     p, se, _ = mcdx.price(c, FP32)
     assert abs(q - p) < 3 * se
+
+
+def qmc_contract(product, m, n, **kw):
+    base = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.25, time=1.0,
+                monitoring_points=m, path_count=n)
+    base.update(kw)
+    return mcdx.contract(product, **base)
+
+
+def oracle_qmc(c):
+    p = O.product(c.product, spot=c.spot, strike=c.strike, barrier=c.barrier, rebate=c.rebate,
+                  rate=c.rate, carry_yield=c.carry_yield, vol=c.vol, time=c.time,
+                  monitoring_points=c.monitoring_points, type=c.type, direction=c.direction,
+                  knock=c.knock, strike_style=c.strike_style)
+    return O.lib().orc_qmc_sobol_path(C.byref(p), c.path_count)
+
+
+PATH_CASES = [
+    (mcdx.BARRIER, dict(barrier=120.0, direction=0, knock=1)),
+    (mcdx.BARRIER, dict(barrier=85.0, direction=1, knock=0, type=1, rebate=1.5)),
+    (mcdx.LOOKBACK, dict(strike_style=1)),
+    (mcdx.LOOKBACK, dict(strike_style=0, strike=95.0, type=1)),
+    (mcdx.ASIAN, dict(strike_style=0)),
+]
+
+
+@pytest.mark.parametrize("product,kw", PATH_CASES)
+@pytest.mark.parametrize("m,n", [(7, 3000), (50, 1 << 12), (252, 1025)])
+def test_qmc_path_products_fp64_match_oracle(product, kw, m, n):
+    c = qmc_contract(product, m, n, **kw)
+    got = mcdx.qmc_path(c, FP64)
+    want = oracle_qmc(c)
+    assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (got, want)
+
+
+@pytest.mark.parametrize("product,kw", PATH_CASES)
+def test_qmc_path_products_fp32_close_and_converge_to_mc(product, kw):
+    c = qmc_contract(product, 252, 1 << 18, **kw)
+    q32, q64 = mcdx.qmc_path(c, FP32), mcdx.qmc_path(c, FP64)
+    assert abs(q32 - q64) <= 2e-4 * max(1.0, abs(q64)), (q32, q64)
+    m = mcdx.GbmContract.from_buffer_copy(c)
+    m.path_count = 1 << 22
+    m.seed = 42
+    p, se, _ = mcdx.price(m, FP32)
+    assert abs(q64 - p) <= 4 * se + 1e-3, (q64, p, se)
