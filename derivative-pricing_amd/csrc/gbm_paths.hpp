@@ -161,12 +161,18 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
                     const bool ct = a.direction == 0 ? xt >= a.xb : xt <= a.xb;
                     t.breached = t.breached || ct;
                 }
-                if (!s.breached || (ANTI && !t.breached)) {
+                // u >= 2^-33, so a crossing probability below 2^-34 can never exceed it: the
+                // uniform is drawn only when some alive path has exponent >= -34 (spots near
+                // the barrier); the decisions are exactly those of drawing it every step.
+                const float es = a.bb_c * (a.xb - s.x) * (a.xb - xn);
+                const float et = ANTI ? a.bb_c * (a.xb - t.x) * (a.xb - xt) : -1e30f;
+                const bool need_s = !s.breached && es >= -34.f;
+                const bool need_t = ANTI && !t.breached && et >= -34.f;
+                if (need_s || need_t) {
                     const uint32_t ub = philox_w0(plo, phh, draw, 1u, k0, k1);
                     const float u = fmaf((float)ub, 0x1p-32f, 0x1p-33f);
-                    if (!s.breached) s.breached = u < fast_exp2(a.bb_c * (a.xb - s.x) * (a.xb - xn));
-                    if (ANTI && !t.breached)
-                        t.breached = u < fast_exp2(a.bb_c * (a.xb - t.x) * (a.xb - xt));
+                    if (need_s) s.breached = u < fast_exp2(es);
+                    if (need_t) t.breached = u < fast_exp2(et);
                 }
                 s.x = xn;
                 if (ANTI) t.x = xt;
@@ -254,12 +260,17 @@ __device__ __forceinline__ void bb_step_f64(const GbmArgs& a, StateF64& p, doubl
     if (!p.breached) {
         const double ls = log(p.s);
         const double lsn = log(sn);
-        const double pc = exp(-2.0 * (a.log_barrier - ls) * (a.log_barrier - lsn) / a.bb_denom);
-        if (!have_u) {
-            u = uniform_open01_f64(philox_w0(plo, phh, draw, 1u, k0, k1));
-            have_u = true;
+        const double ex = -2.0 * (a.log_barrier - ls) * (a.log_barrier - lsn) / a.bb_denom;
+        // u >= 2^-33 > exp(-23.5): below that exponent u < pc cannot hold, so the uniform is
+        // not needed (the decision is the one the reference makes after drawing it).
+        if (ex >= -23.5) {
+            const double pc = exp(ex);
+            if (!have_u) {
+                u = uniform_open01_f64(philox_w0(plo, phh, draw, 1u, k0, k1));
+                have_u = true;
+            }
+            p.breached = u < pc;
         }
-        p.breached = u < pc;
     }
     p.s = sn;
 }
