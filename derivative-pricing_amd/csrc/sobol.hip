@@ -398,39 +398,64 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
     return a.discount * smax_d(phi * ref, 0.0);
 }
 
-// Tile of 1024 Sobol indices per block (256 lanes x 4); index i = tile*1024 + k*256 + lane
-// and the points priced are indices 1..n_points (index 0 is skipped, qmc.cpp:24). A wave's
-// 64 indices are 64-aligned, so Gray-code bits >= 6 are wave-uniform. Welford over points as
-// every MC kernel.
+// Tile of 1024 Sobol indices per block, one point per thread: thread t prices index
+// tile*1024 + t, and the tile's Welford partial is then rebuilt in the canonical order (lane
+// l < 256 folds points l, l+256, l+512, l+768, then the 256-lane tree). The points priced are
+// indices 1..n_points (index 0 is skipped, qmc.cpp:24). A wave's 64 indices are 64-aligned,
+// so Gray-code bits >= 6 are wave-uniform.
 template <bool FP64>
-__global__ __launch_bounds__(kTileLanes) void qmc_tile_kernel(QmcArgs a) {
+__global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
     extern __shared__ unsigned char smem[];
+    __shared__ double vals[kTilePaths];
+    __shared__ unsigned char oks[kTilePaths];
     __shared__ Welford lds4[4];
     float* lds_f = (float*)smem;
     double* lds_d = (double*)smem;
     const uint64_t tile = blockIdx.x;
-    Welford w = {0, 0.0, 0.0};
-#pragma unroll 1
-    for (int k = 0; k < kPathsPerLane; ++k) {
-        const uint64_t wave_base = tile * kTilePaths + (uint64_t)(k * kTileLanes) +
-                                   (threadIdx.x & ~63u);
-        if (wave_base > a.n_points) continue; // wave-uniform
-        const uint64_t i_raw = wave_base + (threadIdx.x & 63u);
-        const bool valid = i_raw >= 1 && i_raw <= a.n_points;
+    const uint32_t t = threadIdx.x;
+    const uint64_t wave_base = tile * kTilePaths + (t & ~63u);
+    double x = 0.0;
+    bool valid = false;
+    if (wave_base <= a.n_points) { // wave-uniform
+        const uint64_t i_raw = wave_base + (t & 63u);
+        valid = i_raw >= 1 && i_raw <= a.n_points;
         const uint64_t i = valid ? i_raw : (wave_base | 1u); // finite stand-in, discarded
         const uint64_t gray = i ^ (i >> 1);
         const uint32_t g_lo = (uint32_t)(gray & 63u);
         const uint32_t g_hi = (uint32_t)(((wave_base | 1u) ^ ((wave_base | 1u) >> 1)) >> 6) << 6;
-        const double x = qmc_point<FP64>(a, i, g_hi, g_lo, lds_f, lds_d, threadIdx.x, kTileLanes);
-        if (valid) {
-            w.n += 1;
-            const double d = x - w.mean;
-            w.mean += d / (double)w.n;
-            w.m2 += d * (x - w.mean);
+        x = qmc_point<FP64>(a, i, g_hi, g_lo, lds_f, lds_d, (int)t, kTilePaths);
+    }
+    vals[t] = x;
+    oks[t] = valid ? 1 : 0;
+    __syncthreads();
+    Welford w = {0, 0.0, 0.0};
+    if (t < (uint32_t)kTileLanes) {
+#pragma unroll
+        for (int k = 0; k < kPathsPerLane; ++k) {
+            const uint32_t j = (uint32_t)(k * kTileLanes) + t;
+            if (oks[j]) {
+                const double xv = vals[j];
+                w.n += 1;
+                const double d = xv - w.mean;
+                w.mean += d / (double)w.n;
+                w.m2 += d * (xv - w.mean);
+            }
         }
     }
-    const Welford t = block_merge_256(w, lds4);
-    if (threadIdx.x == 0) a.partials[tile] = t;
+    // block_merge_256's tree over lanes 0..255 (waves 4..15 hold zeros)
+    const int lane = t & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Welford o;
+        o.n = __shfl_down(w.n, off, 64);
+        o.mean = __shfl_down(w.mean, off, 64);
+        o.m2 = __shfl_down(w.m2, off, 64);
+        if ((lane & (2 * off - 1)) == 0) w = welford_merge(w, o);
+    }
+    if (lane == 0 && t < (uint32_t)kTileLanes) lds4[t >> 6] = w;
+    __syncthreads();
+    if (t == 0)
+        a.partials[tile] = welford_merge(welford_merge(lds4[0], lds4[1]), welford_merge(lds4[2], lds4[3]));
 }
 
 __global__ __launch_bounds__(256) void sobol_ints_kernel(const uint32_t* __restrict__ v,
@@ -468,14 +493,19 @@ int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slot
     a.n_ops = (int32_t)ops.size();
     a.slots = slots;
     a.partials = partials;
-    const size_t shmem = (size_t)(slots > 0 ? slots : 1) * kTileLanes *
+    const size_t shmem = (size_t)(slots > 0 ? slots : 1) * kTilePaths *
                          (precision == MCDX_PREC_FP64 ? sizeof(double) : sizeof(float));
+    if (shmem + 16 * 1024 > 160 * 1024) return MCDX_EINVAL; // bridge slots beyond LDS (not reached for <= 1024 dates)
+    const void* kern = precision == MCDX_PREC_FP64 ? (const void*)qmc_tile_kernel<true>
+                                                   : (const void*)qmc_tile_kernel<false>;
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
     if (e == hipSuccess) {
         if (precision == MCDX_PREC_FP64)
-            hipLaunchKernelGGL(qmc_tile_kernel<true>, dim3((unsigned)tiles), dim3(kTileLanes), shmem,
+            hipLaunchKernelGGL(qmc_tile_kernel<true>, dim3((unsigned)tiles), dim3(kTilePaths), shmem,
                                ctx->stream, a);
         else
-            hipLaunchKernelGGL(qmc_tile_kernel<false>, dim3((unsigned)tiles), dim3(kTileLanes), shmem,
+            hipLaunchKernelGGL(qmc_tile_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths), shmem,
                                ctx->stream, a);
         e = hipGetLastError();
     }
