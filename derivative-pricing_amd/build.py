@@ -103,6 +103,18 @@ def build_cli():
     os.replace(out + ".tmp", out)
 
 
+def build_cpp_tests():
+    """tests/cpp/api_test: C++ tests of the drop-in API, linked against libmcd.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
+    if not os.path.exists(src):
+        return
+    out = os.path.join(ROOT, "tests", "cpp", "api_test")
+    run(["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), src,
+         "-L", LIBDIR, "-lmcd", "-lmcd_hip", "-Wl,-rpath,$ORIGIN/../../derivative-pricing_amd/lib",
+         "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     ref_root = os.environ.get("MCDX_REFERENCE_ROOT", "/root/reference")
@@ -115,6 +127,7 @@ def main():
     build_host_lib()
     build_pybind()
     build_cli()
+    build_cpp_tests()
     build_oracle()
 
 
