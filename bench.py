@@ -178,11 +178,16 @@ def extras(torch, mcdx, L, stream, handle, ref):
                        xi=0.5, rho=-0.7, time=1.0).items():
         setattr(h, k_, v_)
     sa = [mcd.heston_semi_analytic(h, k, mcd.OptionType.Call) for k in ks]
+    # FP64 mode on the same paths: the reference's arithmetic (per-path parity with the
+    # oracle is tested), so the FP32-generation bias is measured path for path.
+    r64 = mcdx.heston_strikes(par, ks, 0, STEPS, 1 << 22, SEED, mcdx.PREC_FP64)
     res["c3_heston_qe_9_strikes"] = {
         "paths_per_s_wall": (1 << 22) / wall, "path_steps_per_s_wall": (1 << 22) * STEPS / wall,
         "ms_wall": wall * 1e3,
         "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
-        "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)]}
+        "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)],
+        "prices_fp64": [x[0] for x in r64],
+        "bias_fp32_vs_fp64_in_se": [(x[0] - y[0]) / y[1] for x, y in zip(r, r64)]}
     progress("C3 done")
     # C4: 4096-contract book, one persistent graph launch vs 4096 per-contract launches of
     # the same path kernel. Headline case: one Monte Carlo seed for the whole book (common
@@ -229,6 +234,20 @@ def c4_entry(torch, mcdx, L, stream, handle):
     progress("C4 shared-seed book timed")
     p_ms, pres = per(crn)
     total = sum(c.path_count for c in crn)
+    # Closed-form check of the terminal contracts (Black-Scholes European, cash digital).
+    import mcd
+    ok = tot = 0
+    for c, (p, se, _) in zip(crn, bres):
+        if c.product not in (mcdx.EUROPEAN, mcdx.DIGITAL):
+            continue
+        typ = mcd.OptionType.Call if c.type == 0 else mcd.OptionType.Put
+        cf = (mcd.black_scholes_merton(c.spot, c.strike, c.rate, c.carry_yield, c.vol, c.time, typ)
+              if c.product == mcdx.EUROPEAN else
+              mcd.digital(c.spot, c.strike, c.rate, c.carry_yield, c.vol, c.time, typ,
+                          mcd.DigitalStyle.CashOrNothing, c.cash_amount))
+        tot += 1
+        ok += abs(p - cf) <= 3 * se
+    out["terminal_contracts_within_3se_of_closed_form"] = [ok, tot]
     out.update(seeds="one seed (42) for the book", batched_ms=b_ms, per_contract_ms=p_ms,
                speedup=p_ms / b_ms, paths=total, batched_paths_per_s=total / (b_ms / 1e3),
                autotuned_schedule=sched, autotune_best_ms=tuned,
