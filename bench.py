@@ -91,6 +91,59 @@ def cpu_baseline(budget_s=12.0):
                       f"{dt2:.1f} s, num_threads={cores}"}
 
 
+def cpu_reference_configs(ex, budget_s=2.5):
+    """The reference's own CPU pricers (oracle/_ref) on bounded samples of the C1, C3 and C5
+    workloads, beside the GPU figures: C1 multithreaded (the reference's thread pool), Heston
+    QE and LSM single-threaded (the reference's pricers for those are serial). Each sample is
+    scaled from a probe to about budget_s seconds. Returns None without the reference build."""
+    import oracle_lib as O
+    R = O.ref()
+    if R is None:
+        return None
+    cores = len(os.sched_getaffinity(0))
+    res = O.Result()
+
+    def scaled(fn, n, n_max):
+        while True:  # grow the sample until it takes at least half the budget
+            t0 = time.perf_counter()
+            fn(n)
+            dt = time.perf_counter() - t0
+            if dt >= 0.5 * budget_s or n >= n_max:
+                return n, dt
+            n = int(min(n_max, n * min(16.0, max(2.0, budget_s / max(dt, 1e-4)))))
+
+    out = {}
+    n, dt = scaled(lambda n: R.ref_mc_european(100, 100, .05, 0, .2, 1, 0, n, SEED, 0, cores,
+                                               C.byref(res)), 1 << 16, 1 << 30)
+    out["c1_european"] = {"value": n / dt, "unit": "paths/s", "cores": cores,
+                          "sample": f"{n} paths, num_threads={cores}, {dt:.2f} s"}
+    p9 = (C.c_double * 9)(100.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+    n, dt = scaled(lambda n: R.ref_heston_qe(p9, 100.0, 0, STEPS, n, SEED, C.byref(res)),
+                   1 << 10, 1 << 22)
+    out["c3_heston_qe"] = {"value": n / dt, "unit": "paths/s (one strike)", "cores": 1,
+                           "sample": f"{n} paths x {STEPS} steps, strike 100, single-threaded "
+                                     f"(the reference's Heston pricer), {dt:.2f} s"}
+    co = (C.c_double * (51 * 4))()
+    has = (C.c_uint8 * 51)()
+    inc = C.c_uint8()
+    n, dt = scaled(lambda n: R.ref_lsm(100, 100, .06, 0, .2, 1, 1, 50, n, SEED, C.byref(res),
+                                       co, has, C.byref(inc)), 1 << 12, 1 << 20)
+    out["c5_lsm"] = {"value": n / dt, "unit": "paths/s", "cores": 1,
+                     "sample": f"{n} paths x 50 dates, single-threaded (the reference's LSM "
+                               f"pricer), {dt:.2f} s"}
+    if ex:
+        gpu = {"c1_european": ex.get("c1_european_1000000", {}).get("paths_per_s"),
+               "c3_heston_qe": ex.get("c3_heston_qe_9_strikes", {}).get("paths_per_s_wall"),
+               "c5_lsm": ex.get("c5_american_put_lsm", {}).get("qr_lsm", {}).get("paths_per_s")}
+        for k, g in gpu.items():
+            if g:
+                out[k]["gpu_paths_per_s"] = g
+                out[k]["gpu_over_cpu"] = g / out[k]["value"]
+        out["c3_heston_qe"]["note"] = ("the GPU figure prices 9 strikes per path; the CPU "
+                                       "figure one strike")
+    return out
+
+
 def load_reference_fixture():
     path = os.path.join(ROOT, "tests", "golden", "reference_baseline.json")
     if not os.path.exists(path):
@@ -464,6 +517,8 @@ def main():
         cb = cpu_baseline()
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = value / cb["value"]
+        progress("CPU reference pricers for C1, C3, C5 (bounded samples)")
+        line["cpu_reference_configs"] = cpu_reference_configs(line.get("extras"))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

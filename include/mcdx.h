@@ -302,8 +302,9 @@ int mcdx_lsm_american_learned(double spot, double strike, double rate, double ca
                               double* coefficients, uint8_t* has_coefficients,
                               mcdx_lsm_learned_result* out);
 /* Price with a trained network on fresh paths: streaming, no path store. With judge != 0 the
- * QR policy also runs on the same paths (qr_forward, decisions, disagreements); with judge
- * == 0 it only decides outside the table range and qr_forward is NaN. */
+ * QR policy also runs on the same paths (qr_forward, decisions, disagreements) and the
+ * boundary gap is measured; with judge == 0 it only decides outside the table range,
+ * qr_forward and boundary_gap are NaN and boundary_dates is 0. */
 int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry_yield,
                            double vol, double time, uint8_t type, int32_t monitoring_points,
                            uint64_t path_count, uint64_t seed, int precision,

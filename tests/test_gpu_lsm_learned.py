@@ -83,6 +83,8 @@ def test_qr_forward_replay_reproduces_backward_induction(prec):
                                judge=False)
     assert u["learned"] == d["learned"] and u["fallbacks"] == d["fallbacks"]
     assert u["qr_forward"][0] != u["qr_forward"][0]  # NaN: not formed
+    assert u["boundary_gap"] != u["boundary_gap"] and u["boundary_dates"] == 0
+    assert s["boundary_gap"] == d["boundary_gap"] and s["boundary_dates"] == d["boundary_dates"]
 
 
 def test_learned_american_put_c5():
