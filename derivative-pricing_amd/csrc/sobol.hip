@@ -144,9 +144,11 @@ struct BridgeOp {
     double sd;       // sqrt((t_mid - t_left)(t_right - t_mid) / (t_right - t_left))
     double drift_t;  // (r - q - vol^2/2) * t_{point+1}
     float frac_f, sd_f, drift2_f, pad;
-    uint16_t dim;    // Sobol dimension = the reference's draw index for this split
-    uint8_t sl, sr, so; // LDS slots: left endpoint, right endpoint, output
-    uint8_t pad2[3];
+    // Whole 32-bit words, so the device reads them through the scalar unit (it has no
+    // sub-word loads): the Sobol dimension (= the reference's draw index for this split) and
+    // the LDS slots packed as left | right << 8 | output << 16.
+    uint32_t dim;
+    uint32_t slots;
 };
 
 namespace {
@@ -204,10 +206,8 @@ int build_bridge(int n, double spot_unused, double rate, double carry, double vo
         op.frac_f = (float)frac;
         op.sd_f = (float)sd;
         op.drift2_f = (float)(op.drift_t * log2e);
-        op.dim = (uint16_t)dim;
-        op.sl = (uint8_t)sl;
-        op.sr = (uint8_t)sr;
-        op.so = (uint8_t)so;
+        op.dim = (uint32_t)dim;
+        op.slots = (uint32_t)sl | (uint32_t)sr << 8 | (uint32_t)so << 16;
         ops.push_back(op);
     };
     // Terminal point: w[n-1] = sqrt(t[n]) * z_0 (bridge.cpp:30).
@@ -266,20 +266,44 @@ __device__ __forceinline__ uint32_t sobol_fold(const uint32_t* __restrict__ v, u
     return x;
 }
 
-// Sobol integer for (dim, i) given the wave-uniform high Gray bits and this lane's low bits.
-__device__ __forceinline__ uint32_t sobol_x(const uint32_t* __restrict__ v, uint32_t g_hi,
-                                            uint32_t g_lo) {
-    // g_hi is wave-uniform: read it as such so the fold runs on the scalar unit.
-    const uint32_t gh = __builtin_amdgcn_readfirstlane(g_hi);
-    uint32_t xh = 0;
-    for (uint32_t g = gh; g; g &= g - 1) xh ^= v[__builtin_ctz(g)];
-    uint32_t xl = 0;
+// Sobol integer of (dim, i) split by who shares each Gray-code bit of i: bits >= 10 are the
+// same for a whole 1024-point tile (their XOR per dimension is precomputed once per tile in
+// LDS), bits 6..9 are wave-uniform (one wave of 64 consecutive indices) and bits 0..5 vary
+// by lane. Per dimension that is one LDS broadcast read, ten direction words read through the
+// scalar unit, and ten masked XORs, instead of a loop over the set bits.
+struct SobolLane {
+    uint32_t lo[6]; // all-ones where Gray bit b (b < 6) of this lane's index is set
+    uint32_t mid[4]; // the same for bits 6..9 (wave-uniform values)
+};
+
+__device__ __forceinline__ SobolLane sobol_lane(uint32_t g_lo, uint32_t g_mid) {
+    SobolLane L;
 #pragma unroll
-    for (int b = 0; b < 6; ++b) {
-        const uint32_t mask = 0u - ((g_lo >> b) & 1u);
-        xl ^= mask & v[b];
+    for (int b = 0; b < 6; ++b) L.lo[b] = 0u - ((g_lo >> b) & 1u);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) L.mid[b] = 0u - ((g_mid >> b) & 1u);
+    return L;
+}
+
+__device__ __forceinline__ uint32_t sobol_x(const uint32_t* __restrict__ vd, const SobolLane& L,
+                                            uint32_t tile_part) {
+    uint32_t x = tile_part;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) x ^= L.lo[b] & vd[b];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) x ^= L.mid[b] & vd[6 + b];
+    return x;
+}
+
+// tdir[d] = XOR of direction words 10.. of dimension d over the set bits of the tile's Gray
+// bits (tile_gray = Gray code of the tile index, i.e. Gray bits >= 10 of every index in it).
+__device__ __forceinline__ void sobol_tile_parts(const uint32_t* __restrict__ v, uint32_t dims,
+                                                 uint32_t tile_gray, uint32_t* tdir) {
+    for (uint32_t d = threadIdx.x; d < dims; d += blockDim.x) {
+        uint32_t x = 0;
+        for (uint32_t g = tile_gray; g; g &= g - 1) x ^= v[32 * d + 10 + __builtin_ctz(g)];
+        tdir[d] = x;
     }
-    return xh ^ xl;
 }
 
 // u = x / 2^32 (sobol.cpp:96). FP32 mode uses the engine's FP32 inverse normal
@@ -315,12 +339,17 @@ struct QmcArgs {
 __device__ __forceinline__ double smax_d(double a, double b) { return (a < b) ? b : a; }
 
 // One point of the QMC pricer (index i >= 1). FP64 mode mirrors qmc.cpp / bridge.cpp.
-template <bool FP64>
-__device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_t g_lo,
+// Products of the QMC kernel: one kernel instance each, so the bridge loop carries only its
+// own observation (no per-date product tests).
+enum QmcProduct : int { kQmcAsian = 0, kQmcBarrier = 1, kQmcLookback = 2, kQmcEuropean = 3 };
+
+// One point of the QMC pricer (index i >= 1). FP64 mode mirrors qmc.cpp / bridge.cpp.
+template <bool FP64, int PROD>
+__device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t* tdir,
                             float* lds_f, double* lds_d, int tid, int nthreads) {
     const double phi = a.type == 0 ? 1.0 : -1.0;
-    if (a.european) { // qmc.cpp:24-28, dimension 0
-        const uint32_t x = sobol_x(a.v, g_hi, g_lo);
+    if constexpr (PROD == kQmcEuropean) { // qmc.cpp:24-28, dimension 0
+        const uint32_t x = sobol_x(a.v, L, tdir[0]);
         double st;
         if constexpr (FP64) {
             const double z = inv_normal_f64((double)x / 4294967296.0);
@@ -330,72 +359,73 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
             st = a.spot * (double)fast_exp2(__builtin_fmaf(a.sig2, z, a.mu2));
         }
         return a.discount * smax_d(phi * (st - a.strike), 0.0);
-    }
-    // Asian over the bridge (qmc.cpp:44-54): replay the depth-first program. Barriers and
-    // lookbacks observe the same dates (every date once; breach, maximum and minimum do not
-    // depend on the visiting order; the program's first point is the last date).
-    double sum = 0.0, last = 0.0;
-    double mx = FP64 ? a.spot : 1.0, mn = mx; // lookback extrema seeded at S0 (S/S0 in FP32)
-    bool breached = false;
-    if constexpr (FP64) lds_d[tid] = 0.0;
-    else lds_f[tid] = 0.f;
-    for (int k = 0; k < a.n_ops; ++k) {
-        const BridgeOp& op = a.ops[k];
-        const uint32_t x = sobol_x(a.v + 32 * op.dim, g_hi, g_lo);
-        if constexpr (FP64) {
-            const double z = inv_normal_f64((double)x / 4294967296.0);
-            const double wl = lds_d[op.sl * nthreads + tid];
-            const double wr = lds_d[op.sr * nthreads + tid];
-            const double mean = wl + op.frac * (wr - wl);
-            const double w = mean + op.sd * z;
-            lds_d[op.so * nthreads + tid] = w;
-            const double s = a.spot * exp(op.drift_t + a.vol * w);
-            sum += s;
-            if (k == 0) last = s;
-            if (a.product == 1) breached = breached || (a.direction == 0 ? s >= a.barrier : s <= a.barrier);
-            if (a.product == 2) {
+    } else {
+        // Asian over the bridge (qmc.cpp:44-54): replay the depth-first program. Barriers and
+        // lookbacks observe the same dates (every date once; breach, maximum and minimum do
+        // not depend on the visiting order; the program's first point is the last date).
+        // FP32 mode observes s / S0 (log2-space state) and scales by S0 at the end.
+        double sum = 0.0, last = 0.0;
+        double mx = FP64 ? a.spot : 1.0, mn = mx; // lookback extrema seeded at S0
+        bool breached = false;
+        const bool up = a.direction == 0;
+        if constexpr (FP64) lds_d[tid] = 0.0;
+        else lds_f[tid] = 0.f;
+        for (int k = 0; k < a.n_ops; ++k) {
+            const BridgeOp& op = a.ops[k];
+            const uint32_t x = sobol_x(a.v + 32 * op.dim, L, tdir[op.dim]);
+            const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
+            double s;
+            if constexpr (FP64) {
+                const double z = inv_normal_f64((double)x / 4294967296.0);
+                const double wl = lds_d[sl * nthreads + tid];
+                const double wr = lds_d[sr * nthreads + tid];
+                const double mean = wl + op.frac * (wr - wl);
+                const double w = mean + op.sd * z;
+                lds_d[so * nthreads + tid] = w;
+                s = a.spot * exp(op.drift_t + a.vol * w);
+                if constexpr (PROD == kQmcBarrier)
+                    breached = breached || (up ? s >= a.barrier : s <= a.barrier);
+            } else {
+                const float z = sobol_normal_f32(x);
+                const float wl = lds_f[sl * nthreads + tid];
+                const float wr = lds_f[sr * nthreads + tid];
+                const float w = __builtin_fmaf(op.sd_f, z, __builtin_fmaf(op.frac_f, wr - wl, wl));
+                lds_f[so * nthreads + tid] = w;
+                const float su = fast_exp2(__builtin_fmaf(a.vol2, w, op.drift2_f));
+                if constexpr (PROD == kQmcBarrier)
+                    breached = breached || (up ? su >= a.barrier_rel : su <= a.barrier_rel);
+                s = (double)su;
+            }
+            if (k == 0) last = s; // uniform: the first op is the last date
+            if constexpr (PROD == kQmcAsian) sum += s;
+            if constexpr (PROD == kQmcLookback) {
                 mx = smax_d(mx, s);
                 mn = (s < mn) ? s : mn;
             }
+        }
+        if constexpr (!FP64) {
+            sum *= a.spot;
+            last *= a.spot;
+            mx *= a.spot;
+            mn *= a.spot;
+        }
+        if constexpr (PROD == kQmcBarrier) { // barrier.hpp:32-37
+            const double vanilla = smax_d(phi * (last - a.strike), 0.0);
+            const bool pays = a.knock == 1 ? !breached : breached;
+            return a.discount * (pays ? vanilla : a.rebate);
+        } else if constexpr (PROD == kQmcLookback) { // lookback.hpp:32-35, 54-56
+            double r;
+            if (a.strike_style == 0)
+                r = a.type == 0 ? smax_d(mx - a.strike, 0.0) : smax_d(a.strike - mn, 0.0);
+            else
+                r = a.type == 0 ? (last - mn) : (mx - last);
+            return a.discount * r;
         } else {
-            const float z = sobol_normal_f32(x);
-            const float wl = lds_f[op.sl * nthreads + tid];
-            const float wr = lds_f[op.sr * nthreads + tid];
-            const float w = __builtin_fmaf(op.sd_f, z, __builtin_fmaf(op.frac_f, wr - wl, wl));
-            lds_f[op.so * nthreads + tid] = w;
-            const float su = fast_exp2(__builtin_fmaf(a.vol2, w, op.drift2_f));
-            sum += (double)su;
-            if (k == 0) last = (double)su;
-            if (a.product == 1)
-                breached = breached || (a.direction == 0 ? su >= a.barrier_rel : su <= a.barrier_rel);
-            if (a.product == 2) {
-                mx = smax_d(mx, (double)su);
-                mn = ((double)su < mn) ? (double)su : mn;
-            }
+            const double avg = sum / (double)a.m;
+            const double ref = a.strike_style == 0 ? (avg - a.strike) : (last - avg);
+            return a.discount * smax_d(phi * ref, 0.0);
         }
     }
-    if constexpr (!FP64) {
-        sum *= a.spot;
-        last *= a.spot;
-        mx *= a.spot;
-        mn *= a.spot;
-    }
-    if (a.product == 1) { // barrier.hpp:32-37
-        const double vanilla = smax_d(phi * (last - a.strike), 0.0);
-        const bool pays = a.knock == 1 ? !breached : breached;
-        return a.discount * (pays ? vanilla : a.rebate);
-    }
-    if (a.product == 2) { // lookback.hpp:32-35, 54-56
-        double r;
-        if (a.strike_style == 0)
-            r = a.type == 0 ? smax_d(mx - a.strike, 0.0) : smax_d(a.strike - mn, 0.0);
-        else
-            r = a.type == 0 ? (last - mn) : (mx - last);
-        return a.discount * r;
-    }
-    const double avg = sum / (double)a.m;
-    const double ref = a.strike_style == 0 ? (avg - a.strike) : (last - avg);
-    return a.discount * smax_d(phi * ref, 0.0);
 }
 
 // Tile of 1024 Sobol indices per block, one point per thread: thread t prices index
@@ -403,17 +433,23 @@ __device__ double qmc_point(const QmcArgs& a, uint64_t i, uint32_t g_hi, uint32_
 // l < 256 folds points l, l+256, l+512, l+768, then the 256-lane tree). The points priced are
 // indices 1..n_points (index 0 is skipped, qmc.cpp:24). A wave's 64 indices are 64-aligned,
 // so Gray-code bits >= 6 are wave-uniform.
-template <bool FP64>
+template <bool FP64, int PROD>
 __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
     extern __shared__ unsigned char smem[];
     __shared__ double vals[kTilePaths];
     __shared__ unsigned char oks[kTilePaths];
     __shared__ Welford lds4[4];
+    __shared__ uint32_t tdir[kSobolDims];
     float* lds_f = (float*)smem;
     double* lds_d = (double*)smem;
     const uint64_t tile = blockIdx.x;
     const uint32_t t = threadIdx.x;
-    const uint64_t wave_base = tile * kTilePaths + (t & ~63u);
+    // The wave's first index, read as wave-uniform so that the branch below and the bridge
+    // loop inside it stay scalar (the op table is then read through the scalar unit).
+    const uint64_t wave_base = tile * kTilePaths + __builtin_amdgcn_readfirstlane(t & ~63u);
+    sobol_tile_parts(a.v, PROD == kQmcEuropean ? 1u : (uint32_t)a.m, (uint32_t)(tile ^ (tile >> 1)),
+                     tdir);
+    __syncthreads();
     double x = 0.0;
     bool valid = false;
     if (wave_base <= a.n_points) { // wave-uniform
@@ -421,9 +457,9 @@ __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
         valid = i_raw >= 1 && i_raw <= a.n_points;
         const uint64_t i = valid ? i_raw : (wave_base | 1u); // finite stand-in, discarded
         const uint64_t gray = i ^ (i >> 1);
-        const uint32_t g_lo = (uint32_t)(gray & 63u);
-        const uint32_t g_hi = (uint32_t)(((wave_base | 1u) ^ ((wave_base | 1u) >> 1)) >> 6) << 6;
-        x = qmc_point<FP64>(a, i, g_hi, g_lo, lds_f, lds_d, (int)t, kTilePaths);
+        const SobolLane L = sobol_lane((uint32_t)(gray & 63u),
+                                       __builtin_amdgcn_readfirstlane((uint32_t)((gray >> 6) & 15u)));
+        x = qmc_point<FP64, PROD>(a, L, tdir, lds_f, lds_d, (int)t, kTilePaths);
     }
     vals[t] = x;
     oks[t] = valid ? 1 : 0;
@@ -472,6 +508,31 @@ __global__ __launch_bounds__(256) void sobol_ints_kernel(const uint32_t* __restr
 namespace mcdx {
 namespace {
 
+template <bool FP64, int PROD>
+hipError_t launch_qmc_as(unsigned tiles, size_t shmem, hipStream_t s, const QmcArgs& a) {
+    hipError_t e = hipFuncSetAttribute((const void*)qmc_tile_kernel<FP64, PROD>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((qmc_tile_kernel<FP64, PROD>), dim3(tiles), dim3(kTilePaths), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <bool FP64>
+hipError_t launch_qmc_prec(int prod, unsigned tiles, size_t shmem, hipStream_t s, const QmcArgs& a) {
+    switch (prod) {
+    case kQmcEuropean: return launch_qmc_as<FP64, kQmcEuropean>(tiles, shmem, s, a);
+    case kQmcBarrier: return launch_qmc_as<FP64, kQmcBarrier>(tiles, shmem, s, a);
+    case kQmcLookback: return launch_qmc_as<FP64, kQmcLookback>(tiles, shmem, s, a);
+    default: return launch_qmc_as<FP64, kQmcAsian>(tiles, shmem, s, a);
+    }
+}
+
+hipError_t launch_qmc(bool fp64, int prod, unsigned tiles, size_t shmem, hipStream_t s,
+                      const QmcArgs& a) {
+    return fp64 ? launch_qmc_prec<true>(prod, tiles, shmem, s, a)
+                : launch_qmc_prec<false>(prod, tiles, shmem, s, a);
+}
+
 int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slots,
             mcdx_result* out) {
     ThreadCtx* ctx;
@@ -496,19 +557,11 @@ int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slot
     const size_t shmem = (size_t)(slots > 0 ? slots : 1) * kTilePaths *
                          (precision == MCDX_PREC_FP64 ? sizeof(double) : sizeof(float));
     if (shmem + 16 * 1024 > 160 * 1024) return MCDX_EINVAL; // bridge slots beyond LDS (not reached for <= 1024 dates)
-    const void* kern = precision == MCDX_PREC_FP64 ? (const void*)qmc_tile_kernel<true>
-                                                   : (const void*)qmc_tile_kernel<false>;
+    const int prod = a.european ? kQmcEuropean : a.product == 1 ? kQmcBarrier
+                                               : a.product == 2 ? kQmcLookback : kQmcAsian;
+    const bool fp64 = precision == MCDX_PREC_FP64;
     if (e == hipSuccess)
-        e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e == hipSuccess) {
-        if (precision == MCDX_PREC_FP64)
-            hipLaunchKernelGGL(qmc_tile_kernel<true>, dim3((unsigned)tiles), dim3(kTilePaths), shmem,
-                               ctx->stream, a);
-        else
-            hipLaunchKernelGGL(qmc_tile_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths), shmem,
-                               ctx->stream, a);
-        e = hipGetLastError();
-    }
+        e = launch_qmc(fp64, prod, (unsigned)tiles, shmem, ctx->stream, a);
     if (e == hipSuccess) e = launch_final_merge(partials, nullptr, tiles, 1, partials + tiles, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ctx->pinned, partials + tiles, sizeof(Welford), hipMemcpyDeviceToHost,
