@@ -41,55 +41,71 @@ __host__ __device__ __forceinline__ int ridx(int i, int j) { // i <= j
     return i * kLsmBasis - (i * (i - 1)) / 2 + (j - i);
 }
 
-// Fold one row (a[J0..3], y; zero before column J0) into (R, z) with Givens rotations.
-// Branch-free: a zero a[j] selects c = 1, s = 0, which reproduces R's row and the remaining
-// entries exactly (the skip of the textbook loop), also when R(j,j) = 0 makes the reciprocal
-// square root infinite. Without branches the compiler interleaves the rotations of
-// consecutive rows, which is what bounds the latency of the merge trees.
-template <int J0>
-__host__ __device__ __forceinline__ void givens_fold(QrState& st, double* a, double y) {
+// One Givens rotation at column J between R's row J and a row being streamed in (a[0..3],
+// y; its entries before J are already zero). c = rjj / h, s = aj / h with
+// h = sqrt(rjj^2 + aj^2): one reciprocal square root (hardware seed + Newton) instead of a
+// square root and two divisions; the updates are fused multiply-adds (four FP64 operations
+// per column instead of six: the regression kernels issue FP64 work back to back, so their
+// operation count is their time). Branch-free: a zero a[J] selects c = 1, s = 0, which
+// reproduces R's row and the remaining entries exactly (the skip of the textbook loop), also
+// when R(J,J) = 0 makes the reciprocal square root infinite.
+template <int J>
+__host__ __device__ __forceinline__ void givens_rot(QrState& st, double* a, double& y) {
+    const double aj = a[J];
+    const double rjj = st.r[ridx(J, J)];
+    const double inv = fast_rsqrt_f64(__builtin_fma(rjj, rjj, aj * aj));
+    const bool skip = aj == 0.0;
+    const double c = skip ? 1.0 : rjj * inv, s = skip ? 0.0 : aj * inv;
 #pragma unroll
-    for (int j = J0; j < kLsmBasis; ++j) {
-        const double aj = a[j];
-        const double rjj = st.r[ridx(j, j)];
-        // c = rjj / h, s = aj / h with h = sqrt(rjj^2 + aj^2): one reciprocal square root
-        // (hardware seed + Newton) instead of a square root and two divisions.
-        const double inv = fast_rsqrt_f64(__builtin_fma(rjj, rjj, aj * aj));
-        const bool skip = aj == 0.0;
-        const double c = skip ? 1.0 : rjj * inv, s = skip ? 0.0 : aj * inv;
+    for (int k = J; k < kLsmBasis; ++k) {
+        const double rk = st.r[ridx(J, k)];
+        st.r[ridx(J, k)] = __builtin_fma(c, rk, s * a[k]);
+        a[k] = __builtin_fma(c, a[k], -s * rk);
+    }
+    const double zj = st.z[J];
+    st.z[J] = __builtin_fma(c, zj, s * y);
+    y = __builtin_fma(c, y, -s * zj);
+}
+
+template <int J>
+__host__ __device__ __forceinline__ void givens_rot_if(int j, QrState& st, double* a, double& y) {
+    if (j == J) givens_rot<J>(st, a, y);
+}
+
+// Fold NR rows into (R, z) in anti-diagonal order: step t applies the rotation (row r,
+// column t - r) of every row r. Rotation (r, j) needs (r, j - 1) and (r - 1, j), both done
+// at step t - 1, so the rotations of one step are independent of each other; each rotation
+// still sees exactly the operands of the row-by-row order, so R and z are bit-identical to
+// it. With TRI, row r starts at column r (the rows of another R factor, zero before their
+// diagonal).
+template <int NR, bool TRI>
+__host__ __device__ __forceinline__ void givens_fold_rows(QrState& st, double (&rows)[NR][4],
+                                                          double (&ys)[NR]) {
 #pragma unroll
-        for (int k = j; k < kLsmBasis; ++k) {
-            const double rk = st.r[ridx(j, k)];
-            st.r[ridx(j, k)] = c * rk + s * a[k];
-            a[k] = -s * rk + c * a[k];
+    for (int t = 0; t < NR + kLsmBasis - 1; ++t) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int j = t - r;
+            if (j < (TRI ? r : 0) || j >= kLsmBasis) continue;
+            givens_rot_if<0>(j, st, rows[r], ys[r]);
+            givens_rot_if<1>(j, st, rows[r], ys[r]);
+            givens_rot_if<2>(j, st, rows[r], ys[r]);
+            givens_rot_if<3>(j, st, rows[r], ys[r]);
         }
-        const double zj = st.z[j];
-        st.z[j] = c * zj + s * y;
-        y = -s * zj + c * y;
     }
 }
 
-__host__ __device__ __forceinline__ void qr_add_row(QrState& st, double a0, double a1, double a2,
-                                                    double a3, double y) {
-    double a[4] = {a0, a1, a2, a3};
-    givens_fold<0>(st, a, y);
-}
-
-// Stream row I of b's [R_b | z_b] into a; its entries before column I are structural zeros.
-template <int I>
-__host__ __device__ __forceinline__ void merge_row(QrState& a, const QrState& b) {
-    double row[4];
-#pragma unroll
-    for (int j = 0; j < kLsmBasis; ++j) row[j] = j >= I ? b.r[ridx(I, j)] : 0.0;
-    givens_fold<I>(a, row, b.z[I]);
-}
-
+// Stream b's rows [R_b | z_b] into a (row i is zero before column i).
 __host__ __device__ __forceinline__ QrState qr_merge(QrState a, const QrState& b) {
-    static_assert(kLsmBasis == 4, "merge_row calls are written out for four columns");
-    merge_row<0>(a, b);
-    merge_row<1>(a, b);
-    merge_row<2>(a, b);
-    merge_row<3>(a, b);
+    static_assert(kLsmBasis == 4, "the merge streams four rows");
+    double rows[4][4], ys[4];
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+        for (int j = 0; j < kLsmBasis; ++j) rows[i][j] = j >= i ? b.r[ridx(i, j)] : 0.0;
+        ys[i] = b.z[i];
+    }
+    givens_fold_rows<4, true>(a, rows, ys);
     a.rows += b.rows;
     return a;
 }
@@ -225,22 +241,23 @@ __device__ __forceinline__ double decide(const LsmArgs& a, double s1, double cf,
     return cf;
 }
 
-// Fold one path's row at `date` (lsm.cpp:88-100) and capture the training sample.
-__device__ __forceinline__ void fold(const LsmArgs& a, QrState& st, int date, uint64_t p,
-                                     double s0, double cf) {
-    const bool itm = intrinsic_v(s0, a.strike, a.type) > 0.0;
-    if (itm) {
-        double b[4];
-        laguerre4(s0 / a.strike, b);
-        qr_add_row(st, b[0], b[1], b[2], b[3], cf);
-        st.rows += 1;
-    }
-    if (a.cap_y) {
+// One path's regression row at `date` (lsm.cpp:88-100): the Laguerre basis of S/K when the
+// path is in range and in the money, else zeros; returns whether it is. Captures the
+// network's training sample for every path in range.
+__device__ __forceinline__ bool fold_row(const LsmArgs& a, int date, uint64_t p, bool in,
+                                         double s0, double cf, double* row) {
+    const bool itm = in && intrinsic_v(s0, a.strike, a.type) > 0.0;
+    double b[4];
+    laguerre4(s0 / a.strike, b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) row[k] = itm ? b[k] : 0.0;
+    if (a.cap_y && in) {
         const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
         a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
         a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
         a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
     }
+    return itm;
 }
 
 __device__ __forceinline__ QrState bcast_qr(const QrState& s) {
@@ -339,7 +356,25 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
             if (use) c = decide(a, s1[u], c, beta);
             c *= a.disc; // lsm.cpp:84-86 (and :125-127 after the last date)
             a.cf[p] = c;
-            if (regress) fold(a, st, date, p, s0[u], c);
+            cf[u] = c;
+        }
+        if (!regress) continue; // uniform across the grid
+        // The chunk's rows in ascending path order, in groups of four folded in
+        // anti-diagonal order; out-of-the-money and past-the-end paths fold as zero rows,
+        // which leave R and z unchanged.
+#pragma unroll
+        for (uint32_t g = 0; g < kLsmChunk; g += 4) {
+            double rows[4][4], ys[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t u = g + r;
+                const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
+                const bool in = j0 + u < a.per_lane && p < a.n_paths;
+                const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[r]);
+                ys[r] = itm ? cf[u] : 0.0;
+                st.rows += itm ? 1u : 0u;
+            }
+            givens_fold_rows<4, false>(st, rows, ys);
         }
     }
     if (!regress) return; // uniform across the grid
