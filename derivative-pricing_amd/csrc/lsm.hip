@@ -212,13 +212,13 @@ __device__ __forceinline__ void qr_shfl_down(const QrState& s, QrState& o, int o
 }
 
 // ---- Backward induction -------------------------------------------------------------------
-// Path layout: B = min(512, ceil(n / 2048)) blocks of 256 lanes; lane t of block b owns
+// Path layout: B = min(256, ceil(n / 4096)) blocks of 256 lanes; lane t of block b owns
 // paths ((j * B + b) * 256 + t) for j < P = ceil(n / (256 B)), coalesced per j. Each date
 // runs one kernel (decision at the later date, discount, fold the ITM rows into the lane's
 // QR state, block tree) and one single-block solve kernel (merge the B partials along a
 // fixed tree, back-substitute on the device). The layout depends on n only, so the
 // coefficients are bitwise reproducible.
-constexpr uint32_t kLsmMaxBlocks = 512;
+constexpr uint32_t kLsmMaxBlocks = 256;
 constexpr uint32_t kLsmChunk = 8; // paths per lane whose loads are issued together
 
 __host__ __device__ __forceinline__ uint64_t lane_path(uint32_t j, uint32_t block,
@@ -831,7 +831,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.type = type;
     const uint64_t blocks = (n + kLsmBlock - 1) / kLsmBlock;
     const uint64_t tiles = tiles_for(n);
-    uint64_t bb = (n + 2047) / 2048;
+    uint64_t bb = (n + 4095) / 4096;
     if (bb > kLsmMaxBlocks) bb = kLsmMaxBlocks;
     a.blocks = (uint32_t)bb;
     a.per_lane = (uint32_t)((n + bb * kLsmBlock - 1) / (bb * kLsmBlock));
