@@ -333,7 +333,9 @@ struct QmcArgs {
     uint8_t direction, knock;
     double barrier, rebate;
     float barrier_rel;     // FP32: H / S0
-    Welford* partials;
+    Welford* partials;     // [replicate][tile]
+    const uint32_t* shifts; // randomized runs: digital shifts [replicate][dims]
+    uint32_t dims;          // dimensions per point (1 for the European)
 };
 
 __device__ __forceinline__ double smax_d(double a, double b) { return (a < b) ? b : a; }
@@ -344,7 +346,19 @@ __device__ __forceinline__ double smax_d(double a, double b) { return (a < b) ? 
 enum QmcProduct : int { kQmcAsian = 0, kQmcBarrier = 1, kQmcLookback = 2, kQmcEuropean = 3 };
 
 // One point of the QMC pricer (index i >= 1). FP64 mode mirrors qmc.cpp / bridge.cpp.
-template <bool FP64, int PROD>
+// Normal of a Sobol integer. Plain runs: u = x / 2^32 (sobol.cpp:96). Randomized runs
+// (digitally shifted x): u = (x + 0.5) / 2^32, the Philox uniform's mapping (rng.cpp:34-37),
+// so a shifted point never reaches u = 0; the FP32 inverse normal is written for it.
+template <bool FP64, bool SHIFTED>
+__device__ __forceinline__ double qmc_normal_f64(uint32_t x) {
+    return SHIFTED ? inv_normal_f64(uniform_open01_f64(x)) : inv_normal_f64((double)x / 4294967296.0);
+}
+template <bool SHIFTED>
+__device__ __forceinline__ float qmc_normal_f32(uint32_t x) {
+    return SHIFTED ? inv_normal_f32(x) : sobol_normal_f32(x);
+}
+
+template <bool FP64, int PROD, bool SHIFTED>
 __device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t* tdir,
                             float* lds_f, double* lds_d, int tid, int nthreads) {
     const double phi = a.type == 0 ? 1.0 : -1.0;
@@ -352,10 +366,10 @@ __device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t
         const uint32_t x = sobol_x(a.v, L, tdir[0]);
         double st;
         if constexpr (FP64) {
-            const double z = inv_normal_f64((double)x / 4294967296.0);
+            const double z = qmc_normal_f64<true, SHIFTED>(x);
             st = a.spot * exp(a.drift + a.vsd * z);
         } else {
-            const float z = sobol_normal_f32(x);
+            const float z = qmc_normal_f32<SHIFTED>(x);
             st = a.spot * (double)fast_exp2(__builtin_fmaf(a.sig2, z, a.mu2));
         }
         return a.discount * smax_d(phi * (st - a.strike), 0.0);
@@ -376,7 +390,7 @@ __device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t
             const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
             double s;
             if constexpr (FP64) {
-                const double z = inv_normal_f64((double)x / 4294967296.0);
+                const double z = qmc_normal_f64<true, SHIFTED>(x);
                 const double wl = lds_d[sl * nthreads + tid];
                 const double wr = lds_d[sr * nthreads + tid];
                 const double mean = wl + op.frac * (wr - wl);
@@ -386,7 +400,7 @@ __device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t
                 if constexpr (PROD == kQmcBarrier)
                     breached = breached || (up ? s >= a.barrier : s <= a.barrier);
             } else {
-                const float z = sobol_normal_f32(x);
+                const float z = qmc_normal_f32<SHIFTED>(x);
                 const float wl = lds_f[sl * nthreads + tid];
                 const float wr = lds_f[sr * nthreads + tid];
                 const float w = __builtin_fmaf(op.sd_f, z, __builtin_fmaf(op.frac_f, wr - wl, wl));
@@ -433,7 +447,7 @@ __device__ double qmc_point(const QmcArgs& a, const SobolLane& L, const uint32_t
 // l < 256 folds points l, l+256, l+512, l+768, then the 256-lane tree). The points priced are
 // indices 1..n_points (index 0 is skipped, qmc.cpp:24). A wave's 64 indices are 64-aligned,
 // so Gray-code bits >= 6 are wave-uniform.
-template <bool FP64, int PROD>
+template <bool FP64, int PROD, bool SHIFTED>
 __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
     extern __shared__ unsigned char smem[];
     __shared__ double vals[kTilePaths];
@@ -447,8 +461,12 @@ __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
     // The wave's first index, read as wave-uniform so that the branch below and the bridge
     // loop inside it stay scalar (the op table is then read through the scalar unit).
     const uint64_t wave_base = tile * kTilePaths + __builtin_amdgcn_readfirstlane(t & ~63u);
-    sobol_tile_parts(a.v, PROD == kQmcEuropean ? 1u : (uint32_t)a.m, (uint32_t)(tile ^ (tile >> 1)),
-                     tdir);
+    sobol_tile_parts(a.v, a.dims, (uint32_t)(tile ^ (tile >> 1)), tdir);
+    if (SHIFTED) { // replicate blockIdx.y: its digital shift joins the tile part
+        __syncthreads();
+        for (uint32_t d = t; d < a.dims; d += blockDim.x)
+            tdir[d] ^= a.shifts[(uint64_t)blockIdx.y * a.dims + d];
+    }
     __syncthreads();
     double x = 0.0;
     bool valid = false;
@@ -459,7 +477,7 @@ __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
         const uint64_t gray = i ^ (i >> 1);
         const SobolLane L = sobol_lane((uint32_t)(gray & 63u),
                                        __builtin_amdgcn_readfirstlane((uint32_t)((gray >> 6) & 15u)));
-        x = qmc_point<FP64, PROD>(a, L, tdir, lds_f, lds_d, (int)t, kTilePaths);
+        x = qmc_point<FP64, PROD, SHIFTED>(a, L, tdir, lds_f, lds_d, (int)t, kTilePaths);
     }
     vals[t] = x;
     oks[t] = valid ? 1 : 0;
@@ -491,7 +509,8 @@ __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
     if (lane == 0 && t < (uint32_t)kTileLanes) lds4[t >> 6] = w;
     __syncthreads();
     if (t == 0)
-        a.partials[tile] = welford_merge(welford_merge(lds4[0], lds4[1]), welford_merge(lds4[2], lds4[3]));
+        a.partials[(uint64_t)blockIdx.y * gridDim.x + tile] =
+            welford_merge(welford_merge(lds4[0], lds4[1]), welford_merge(lds4[2], lds4[3]));
 }
 
 __global__ __launch_bounds__(256) void sobol_ints_kernel(const uint32_t* __restrict__ v,
@@ -508,52 +527,84 @@ __global__ __launch_bounds__(256) void sobol_ints_kernel(const uint32_t* __restr
 namespace mcdx {
 namespace {
 
-template <bool FP64, int PROD>
-hipError_t launch_qmc_as(unsigned tiles, size_t shmem, hipStream_t s, const QmcArgs& a) {
-    hipError_t e = hipFuncSetAttribute((const void*)qmc_tile_kernel<FP64, PROD>,
+template <bool FP64, int PROD, bool SHIFTED>
+hipError_t launch_qmc_as(unsigned tiles, unsigned reps, size_t shmem, hipStream_t s,
+                         const QmcArgs& a) {
+    hipError_t e = hipFuncSetAttribute((const void*)qmc_tile_kernel<FP64, PROD, SHIFTED>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((qmc_tile_kernel<FP64, PROD>), dim3(tiles), dim3(kTilePaths), shmem, s, a);
+    hipLaunchKernelGGL((qmc_tile_kernel<FP64, PROD, SHIFTED>), dim3(tiles, reps), dim3(kTilePaths),
+                       shmem, s, a);
     return hipGetLastError();
 }
 
-template <bool FP64>
-hipError_t launch_qmc_prec(int prod, unsigned tiles, size_t shmem, hipStream_t s, const QmcArgs& a) {
+template <bool FP64, bool SHIFTED>
+hipError_t launch_qmc_prec(int prod, unsigned tiles, unsigned reps, size_t shmem, hipStream_t s,
+                           const QmcArgs& a) {
     switch (prod) {
-    case kQmcEuropean: return launch_qmc_as<FP64, kQmcEuropean>(tiles, shmem, s, a);
-    case kQmcBarrier: return launch_qmc_as<FP64, kQmcBarrier>(tiles, shmem, s, a);
-    case kQmcLookback: return launch_qmc_as<FP64, kQmcLookback>(tiles, shmem, s, a);
-    default: return launch_qmc_as<FP64, kQmcAsian>(tiles, shmem, s, a);
+    case kQmcEuropean: return launch_qmc_as<FP64, kQmcEuropean, SHIFTED>(tiles, reps, shmem, s, a);
+    case kQmcBarrier: return launch_qmc_as<FP64, kQmcBarrier, SHIFTED>(tiles, reps, shmem, s, a);
+    case kQmcLookback: return launch_qmc_as<FP64, kQmcLookback, SHIFTED>(tiles, reps, shmem, s, a);
+    default: return launch_qmc_as<FP64, kQmcAsian, SHIFTED>(tiles, reps, shmem, s, a);
     }
 }
 
-hipError_t launch_qmc(bool fp64, int prod, unsigned tiles, size_t shmem, hipStream_t s,
-                      const QmcArgs& a) {
-    return fp64 ? launch_qmc_prec<true>(prod, tiles, shmem, s, a)
-                : launch_qmc_prec<false>(prod, tiles, shmem, s, a);
+hipError_t launch_qmc(bool fp64, bool shifted, int prod, unsigned tiles, unsigned reps,
+                      size_t shmem, hipStream_t s, const QmcArgs& a) {
+    if (shifted)
+        return fp64 ? launch_qmc_prec<true, true>(prod, tiles, reps, shmem, s, a)
+                    : launch_qmc_prec<false, true>(prod, tiles, reps, shmem, s, a);
+    return fp64 ? launch_qmc_prec<true, false>(prod, tiles, reps, shmem, s, a)
+                : launch_qmc_prec<false, false>(prod, tiles, reps, shmem, s, a);
 }
 
+// Runs `reps` replicates of the point set (reps = 1 and no shifts: the plain QMC estimate;
+// otherwise replicate r digitally shifted by shifts[r * dims ..]) and returns each
+// replicate's Welford result in out[r].
 int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slots,
-            mcdx_result* out) {
+            uint32_t reps, const std::vector<uint32_t>& shifts, mcdx_result* out) {
     ThreadCtx* ctx;
     int st = thread_ctx(&ctx);
     if (st) return st;
     if ((st = device_tables(&a.v))) return st;
+    a.dims = a.european ? 1u : (uint32_t)a.m;
+    const bool shifted = !shifts.empty();
+    if (shifted && shifts.size() != (size_t)reps * a.dims) return MCDX_EINVAL;
+    if (reps == 0 || (!shifted && reps != 1)) return MCDX_EINVAL;
     // Sobol indices 0 .. n_points live in tiles of 1024 (index 0 skipped).
     const uint64_t tiles = (a.n_points + 1 + kTilePaths - 1) / kTilePaths;
-    const size_t ops_off = (tiles + 1) * sizeof(Welford);
-    const size_t ops_bytes = ops.size() * sizeof(BridgeOp);
-    if ((st = ensure_dev(ctx, ops_off + ops_bytes + 256))) return st;
-    if ((st = ensure_pinned(ctx, sizeof(Welford)))) return st;
-    Welford* partials = (Welford*)ctx->dev;
-    BridgeOp* dops = (BridgeOp*)((char*)ctx->dev + ops_off);
-    hipError_t e = hipSuccess;
-    if (ops_bytes)
-        e = hipMemcpyAsync(dops, ops.data(), ops_bytes, hipMemcpyHostToDevice, ctx->stream);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t part_bytes = al(((size_t)reps * tiles + reps) * sizeof(Welford));
+    const size_t off_bytes = al((size_t)(reps + 1) * sizeof(uint64_t));
+    const size_t ops_bytes = al(ops.size() * sizeof(BridgeOp));
+    const size_t shift_bytes = al(shifts.size() * sizeof(uint32_t));
+    if ((st = ensure_dev(ctx, part_bytes + off_bytes + ops_bytes + shift_bytes))) return st;
+    if ((st = ensure_pinned(ctx, off_bytes + shift_bytes + reps * sizeof(Welford)))) return st;
+    char* dev = (char*)ctx->dev;
+    Welford* partials = (Welford*)dev;
+    Welford* merged = partials + (size_t)reps * tiles;
+    uint64_t* doffs = (uint64_t*)(dev + part_bytes);
+    BridgeOp* dops = (BridgeOp*)(dev + part_bytes + off_bytes);
+    uint32_t* dshift = (uint32_t*)(dev + part_bytes + off_bytes + ops_bytes);
+    hipError_t e = hipStreamSynchronize(ctx->stream); // the pinned staging is free
+    char* h = (char*)ctx->pinned;
+    uint64_t* hoffs = (uint64_t*)h;
+    for (uint32_t r = 0; r <= reps; ++r) hoffs[r] = (uint64_t)r * tiles;
+    if (shifted) memcpy(h + off_bytes, shifts.data(), shifts.size() * sizeof(uint32_t));
+    if (e == hipSuccess && reps > 1)
+        e = hipMemcpyAsync(doffs, hoffs, (size_t)(reps + 1) * sizeof(uint64_t),
+                           hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && !ops.empty())
+        e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(BridgeOp), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess && shifted)
+        e = hipMemcpyAsync(dshift, h + off_bytes, shifts.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, ctx->stream);
     a.ops = dops;
     a.n_ops = (int32_t)ops.size();
     a.slots = slots;
     a.partials = partials;
+    a.shifts = shifted ? dshift : nullptr;
     const size_t shmem = (size_t)(slots > 0 ? slots : 1) * kTilePaths *
                          (precision == MCDX_PREC_FP64 ? sizeof(double) : sizeof(float));
     if (shmem + 16 * 1024 > 160 * 1024) return MCDX_EINVAL; // bridge slots beyond LDS (not reached for <= 1024 dates)
@@ -561,15 +612,21 @@ int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slot
                                                : a.product == 2 ? kQmcLookback : kQmcAsian;
     const bool fp64 = precision == MCDX_PREC_FP64;
     if (e == hipSuccess)
-        e = launch_qmc(fp64, prod, (unsigned)tiles, shmem, ctx->stream, a);
-    if (e == hipSuccess) e = launch_final_merge(partials, nullptr, tiles, 1, partials + tiles, ctx->stream);
+        e = launch_qmc(fp64, shifted, prod, (unsigned)tiles, reps, shmem, ctx->stream, a);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(ctx->pinned, partials + tiles, sizeof(Welford), hipMemcpyDeviceToHost,
-                           ctx->stream);
+        e = launch_final_merge(partials, reps > 1 ? doffs : nullptr, tiles, reps, merged, ctx->stream);
+    Welford* hres = (Welford*)(h + off_bytes + shift_bytes);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hres, merged, reps * sizeof(Welford), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return status_of(e);
-    *out = finish(*(const Welford*)ctx->pinned);
+    for (uint32_t r = 0; r < reps; ++r) out[r] = finish(hres[r]);
     return MCDX_OK;
+}
+
+int qmc_run(QmcArgs a, int precision, const std::vector<BridgeOp>& ops, int slots,
+            mcdx_result* out) {
+    return qmc_run(a, precision, ops, slots, 1, {}, out);
 }
 
 QmcArgs qmc_args(double spot, double strike, double rate, double carry, double vol, double time,
@@ -645,6 +702,70 @@ int qmc_path(const mcdx_gbm_contract* c, int precision, mcdx_result* out) {
     return qmc_run(a, precision, ops, slots, out);
 }
 
+constexpr uint32_t kMaxReplicates = 4096;
+
+// Randomized QMC: `replicates` independent digital shifts of the same point set; the price
+// is the mean of the replicate means and its standard error their spread, the error
+// estimate plain QMC does not give.
+int rqmc_path(const mcdx_gbm_contract* c, int precision, uint32_t replicates,
+              uint64_t shift_seed, mcdx_result* out, double* replicate_prices) {
+    if (!out || !c || replicates < 2 || replicates > kMaxReplicates) return MCDX_EINVAL;
+    if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
+    if (c->type > 1 || c->path_count > (1ull << 32) - 2) return MCDX_EINVAL;
+    const bool european = c->product == MCDX_EUROPEAN;
+    const int32_t m = european ? 1 : c->monitoring_points;
+    uint8_t product = 0;
+    if (!european) {
+        if (c->strike_style > 1 || c->direction > 1 || c->knock > 1 || m < 1 ||
+            m > (int32_t)kSobolDims)
+            return MCDX_EINVAL;
+        if (c->product == MCDX_ASIAN && c->average_style == 0) product = 0;
+        else if (c->product == MCDX_BARRIER && !c->brownian_bridge) product = 1;
+        else if (c->product == MCDX_LOOKBACK) product = 2;
+        else return MCDX_EINVAL;
+    }
+    if (c->path_count == 0) {
+        *out = mcdx_result{NAN, 0.0, 0};
+        return MCDX_OK;
+    }
+    std::vector<BridgeOp> ops;
+    int slots = 0;
+    if (!european) {
+        slots = build_bridge(m, c->spot, c->rate, c->carry_yield, c->vol, c->time, ops);
+        if (slots < 0) return MCDX_EINVAL;
+    }
+    QmcArgs a = qmc_args(c->spot, c->strike, c->rate, c->carry_yield, c->vol, c->time, c->type,
+                         c->path_count);
+    a.m = m;
+    a.european = european ? 1 : 0;
+    a.strike_style = c->strike_style;
+    a.product = product;
+    a.direction = c->direction;
+    a.knock = c->knock;
+    a.barrier = c->barrier;
+    a.rebate = c->rebate;
+    a.barrier_rel = (float)(c->barrier / c->spot);
+    // shift of replicate r, dimension d: Philox4x32-10 word 0 at key shift_seed, counter
+    // {r, 0, d, 3} (stream 3: no reference stream uses it)
+    std::vector<uint32_t> shifts((size_t)replicates * (size_t)m);
+    for (uint32_t r = 0; r < replicates; ++r)
+        for (int32_t d = 0; d < m; ++d)
+            shifts[(size_t)r * m + d] = philox_bits(shift_seed, r, (uint32_t)d, 3u);
+    std::vector<mcdx_result> reps(replicates);
+    const int st = qmc_run(a, precision, ops, slots, replicates, shifts, reps.data());
+    if (st) return st;
+    double sum = 0.0;
+    for (uint32_t r = 0; r < replicates; ++r) sum += reps[r].price;
+    const double mean = sum / (double)replicates;
+    double ss = 0.0;
+    for (uint32_t r = 0; r < replicates; ++r) ss += (reps[r].price - mean) * (reps[r].price - mean);
+    const double se = sqrt(ss / (double)(replicates - 1) / (double)replicates);
+    if (replicate_prices)
+        for (uint32_t r = 0; r < replicates; ++r) replicate_prices[r] = reps[r].price;
+    *out = mcdx_result{mean, se, c->path_count * replicates};
+    return MCDX_OK;
+}
+
 } // namespace mcdx
 
 using namespace mcdx;
@@ -653,6 +774,11 @@ extern "C" {
 
 int mcdx_qmc_sobol_path(const mcdx_gbm_contract* contract, int precision, mcdx_result* out) {
     return qmc_path(contract, precision, out);
+}
+
+int mcdx_rqmc_sobol_path(const mcdx_gbm_contract* contract, int precision, uint32_t replicates,
+                         uint64_t shift_seed, mcdx_result* out, double* replicate_prices) {
+    return rqmc_path(contract, precision, replicates, shift_seed, out, replicate_prices);
 }
 
 uint32_t mcdx_sobol_max_dimensions(void) { return kSobolRefDims; }

@@ -108,6 +108,7 @@ SIGNATURES = {
     "mcdx_qmc_sobol_asian_extended": (C.c_int, [dbl] * 6 + [u8, u8, i32, u64, C.c_int,
                                                             P(Result)]),
     "mcdx_qmc_sobol_path": (C.c_int, [P(GbmContract), C.c_int, P(Result)]),
+    "mcdx_rqmc_sobol_path": (C.c_int, [P(GbmContract), C.c_int, u32, u64, P(Result), P(dbl)]),
     "mcdx_heston_qe_european": (C.c_int, [P(HestonParams), dbl, u8, i32, u64, u64, P(Result)]),
     "mcdx_heston_qe_strikes": (C.c_int, [P(HestonParams), P(dbl), i32, u8, i32, u64, u64,
                                          C.c_int, P(Result)]),
@@ -230,6 +231,15 @@ def qmc_path(c, precision):
     r = Result()
     check(lib().mcdx_qmc_sobol_path(C.byref(c), precision, C.byref(r)), "mcdx_qmc_sobol_path")
     return r.price
+
+
+def rqmc_path(c, precision, replicates, shift_seed):
+    """Randomized QMC: (price, standard error, total points, replicate prices)."""
+    r = Result()
+    reps = np.zeros(replicates, dtype=np.float64)
+    check(lib().mcdx_rqmc_sobol_path(C.byref(c), precision, replicates, shift_seed, C.byref(r),
+                                     reps.ctypes.data_as(P(dbl))), "mcdx_rqmc_sobol_path")
+    return r.price, r.standard_error, r.path_count, reps
 
 
 def heston_strikes(params, strikes, type, steps, n, seed, precision):

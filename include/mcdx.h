@@ -157,6 +157,16 @@ int mcdx_qmc_sobol_asian_extended(double spot, double strike, double rate, doubl
  * standard_error field is the points' sample standard error, which does not measure the
  * error of a deterministic point set. */
 int mcdx_qmc_sobol_path(const mcdx_gbm_contract* contract, int precision, mcdx_result* out);
+/* Randomized QMC (no reference counterpart: the reference's QMC returns a plain mean with no
+ * error estimate, qmc.hpp:9-16). `replicates` (2..4096) digital shifts of the Sobol indices
+ * 1..path_count: replicate r XORs dimension d's integer with Philox4x32-10 word 0 at key
+ * shift_seed, counter {r, 0, d, 3}, and maps it to u = (x + 0.5) / 2^32. out->price = mean of
+ * the replicate means, out->standard_error = their sample deviation / sqrt(replicates),
+ * out->path_count = path_count * replicates; replicate_prices (optional, [replicates])
+ * receives each replicate's mean. Products: European (dimension 0), arithmetic Asian, discrete
+ * barrier and lookback over the bridge (as mcdx_qmc_sobol_path). */
+int mcdx_rqmc_sobol_path(const mcdx_gbm_contract* contract, int precision, uint32_t replicates,
+                         uint64_t shift_seed, mcdx_result* out, double* replicate_prices);
 
 /* ---- Heston QE (src/pricers/heston.cpp) ---- */
 /* HestonParams, include/mcd/models/heston.hpp:8-18 (same field order). */

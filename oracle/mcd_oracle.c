@@ -559,43 +559,80 @@ double orc_qmc_sobol_asian(double spot, double strike, double rate, double carry
  * asian.hpp / barrier.hpp / lookback.hpp observed at every date, the plain mean of
  * qmc.cpp:44-56). The reference prices only the arithmetic Asian this way; this is the
  * engine's extension to barriers and lookbacks. Arithmetic averages only. */
-double orc_qmc_sobol_path(const orc_product* p, uint64_t path_count) {
-    const int m = p->monitoring_points;
+/* One point's discounted payoff from its normals z[0..m-1] (bridge order = Sobol dimension). */
+static double qmc_path_payoff(const orc_product* p, int m, const double* z, double* sp) {
     const double phi = p->type == 0 ? 1.0 : -1.0;
     const double discount = exp(-p->rate * p->time);
+    if (p->product == ORC_EUROPEAN) { /* qmc.cpp:24-28 */
+        const double st = orc_gbm_terminal_spot(p->spot, p->rate, p->carry_yield, p->vol, p->time, z[0]);
+        return discount * smax(phi * (st - p->strike), 0.0);
+    }
+    orc_brownian_bridge_path(p->spot, p->rate, p->carry_yield, p->vol, p->time, m, z, sp);
+    double sum = 0.0, mx = p->spot, mn = p->spot;
+    int breached = 0;
+    for (int d = 0; d < m; ++d) {
+        sum += sp[d];
+        mx = smax(mx, sp[d]);
+        mn = smin(mn, sp[d]);
+        breached = breached || (p->direction == 0 ? sp[d] >= p->barrier : sp[d] <= p->barrier);
+    }
+    const double last = sp[m - 1];
+    double pay;
+    if (p->product == ORC_ASIAN) {
+        const double avg = sum / (double)m;
+        pay = smax(phi * (p->strike_style == 0 ? (avg - p->strike) : (last - avg)), 0.0);
+    } else if (p->product == ORC_BARRIER) {
+        const double vanilla = smax(phi * (last - p->strike), 0.0);
+        const int pays = p->knock == 1 ? !breached : breached;
+        pay = pays ? vanilla : p->rebate;
+    } else if (p->strike_style == 0) {
+        pay = p->type == 0 ? smax(mx - p->strike, 0.0) : smax(p->strike - mn, 0.0);
+    } else {
+        pay = p->type == 0 ? (last - mn) : (mx - last);
+    }
+    return discount * pay;
+}
+
+/* Plain mean over indices 1..path_count. shift == NULL: the reference's point x / 2^32
+ * (sobol.cpp:96). Otherwise the digitally shifted point x ^ shift[d], taken at the middle of
+ * its 2^-32 cell, u = (x ^ shift[d] + 0.5) / 2^32 (the Philox uniform's mapping, rng.cpp:34-37),
+ * so that no shifted point reaches u = 0. */
+static double qmc_mean(const orc_product* p, uint64_t path_count, const uint32_t* shift) {
+    const int m = p->product == ORC_EUROPEAN ? 1 : p->monitoring_points;
     double* z = (double*)malloc(sizeof(double) * (size_t)m);
     double* sp = (double*)malloc(sizeof(double) * (size_t)m);
     double total = 0.0;
     for (uint64_t i = 1; i <= path_count; ++i) {
-        for (int d = 0; d < m; ++d) z[d] = orc_inverse_normal_cdf(orc_sobol_point((unsigned)d, i));
-        orc_brownian_bridge_path(p->spot, p->rate, p->carry_yield, p->vol, p->time, m, z, sp);
-        double sum = 0.0, mx = p->spot, mn = p->spot;
-        int breached = 0;
         for (int d = 0; d < m; ++d) {
-            sum += sp[d];
-            mx = smax(mx, sp[d]);
-            mn = smin(mn, sp[d]);
-            breached = breached || (p->direction == 0 ? sp[d] >= p->barrier : sp[d] <= p->barrier);
+            const double u = shift ? ((double)(orc_sobol_int((unsigned)d, i) ^ shift[d]) + 0.5) / 4294967296.0
+                                   : orc_sobol_point((unsigned)d, i);
+            z[d] = orc_inverse_normal_cdf(u);
         }
-        const double last = sp[m - 1];
-        double pay;
-        if (p->product == ORC_ASIAN) {
-            const double avg = sum / (double)m;
-            pay = smax(phi * (p->strike_style == 0 ? (avg - p->strike) : (last - avg)), 0.0);
-        } else if (p->product == ORC_BARRIER) {
-            const double vanilla = smax(phi * (last - p->strike), 0.0);
-            const int pays = p->knock == 1 ? !breached : breached;
-            pay = pays ? vanilla : p->rebate;
-        } else if (p->strike_style == 0) {
-            pay = p->type == 0 ? smax(mx - p->strike, 0.0) : smax(p->strike - mn, 0.0);
-        } else {
-            pay = p->type == 0 ? (last - mn) : (mx - last);
-        }
-        total += discount * pay;
+        total += qmc_path_payoff(p, m, z, sp);
     }
     free(z);
     free(sp);
     return total / (double)path_count;
+}
+
+double orc_qmc_sobol_path(const orc_product* p, uint64_t path_count) {
+    return qmc_mean(p, path_count, NULL);
+}
+
+/* Randomized QMC: shift[d] = Philox4x32-10 word 0 at key shift_seed, counter
+ * {replicate, 0, d, 3} (stream 3: no reference stream uses it). */
+uint32_t orc_rqmc_shift(uint64_t shift_seed, uint32_t replicate, unsigned dim) {
+    return orc_philox_bits(shift_seed, replicate, dim, 3);
+}
+
+double orc_rqmc_replicate(const orc_product* p, uint64_t path_count, uint64_t shift_seed,
+                          uint32_t replicate) {
+    const int m = p->product == ORC_EUROPEAN ? 1 : p->monitoring_points;
+    uint32_t* shift = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)m);
+    for (int d = 0; d < m; ++d) shift[d] = orc_rqmc_shift(shift_seed, replicate, (unsigned)d);
+    const double r = qmc_mean(p, path_count, shift);
+    free(shift);
+    return r;
 }
 
 /* ------------------------------------------------------------------------ */

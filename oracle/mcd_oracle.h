@@ -94,6 +94,13 @@ double orc_qmc_sobol_european(double spot, double strike, double rate, double ca
 /* QMC over Sobol + bridge for Asian (arithmetic), barrier and lookback products: the
  * reference's pieces composed (no reference counterpart for barriers and lookbacks). */
 double orc_qmc_sobol_path(const orc_product* p, uint64_t path_count);
+/* Randomized QMC (this engine's extension; no reference counterpart): replicate r uses the
+ * digital shift d -> orc_rqmc_shift(shift_seed, r, d) and the cell-midpoint mapping
+ * u = (x ^ shift + 0.5) / 2^32; returns that replicate's plain mean over indices 1..n.
+ * ORC_EUROPEAN uses dimension 0 and the terminal spot; the others the bridge path. */
+uint32_t orc_rqmc_shift(uint64_t shift_seed, uint32_t replicate, unsigned dim);
+double orc_rqmc_replicate(const orc_product* p, uint64_t path_count, uint64_t shift_seed,
+                          uint32_t replicate);
 double orc_qmc_sobol_asian(double spot, double strike, double rate, double carry, double vol,
                            double time, int type, int strike_style, int monitoring_points,
                            uint64_t path_count);

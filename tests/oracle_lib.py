@@ -81,6 +81,8 @@ def lib():
             "orc_qmc_sobol_european": (dbl, [dbl] * 6 + [i32, u64]),
             "orc_qmc_sobol_asian": (dbl, [dbl] * 6 + [i32, i32, i32, u64]),
             "orc_qmc_sobol_path": (dbl, [P(Product), u64]),
+            "orc_rqmc_shift": (C.c_uint32, [u64, C.c_uint32, C.c_uint]),
+            "orc_rqmc_replicate": (dbl, [P(Product), u64, u64, C.c_uint32]),
             "orc_qe_variance_step": (dbl, [dbl] * 6),
             "orc_heston_path_sample": (dbl, [P(Heston), dbl, i32, i32, u64, u64]),
             "orc_heston_qe_european": (Result, [P(Heston), dbl, i32, i32, u64, u64]),

@@ -124,3 +124,82 @@ def test_qmc_path_products_fp32_close_and_converge_to_mc(product, kw):
     m.seed = 42
     p, se, _ = mcdx.price(m, FP32)
     assert abs(q64 - p) <= 4 * se + 1e-3, (q64, p, se)
+
+
+# ---- Randomized QMC (digital shifts; no reference counterpart) ----
+RQMC_CASES = PATH_CASES + [(mcdx.EUROPEAN, dict()), (mcdx.EUROPEAN, dict(type=1, strike=110.0))]
+
+
+def oracle_rqmc(c, seed, r):
+    p = O.product(c.product, spot=c.spot, strike=c.strike, barrier=c.barrier, rebate=c.rebate,
+                  rate=c.rate, carry_yield=c.carry_yield, vol=c.vol, time=c.time,
+                  monitoring_points=c.monitoring_points, type=c.type, direction=c.direction,
+                  knock=c.knock, strike_style=c.strike_style)
+    return O.lib().orc_rqmc_replicate(C.byref(p), c.path_count, seed, r)
+
+
+@pytest.mark.parametrize("product,kw", RQMC_CASES)
+@pytest.mark.parametrize("m,n", [(7, 3000), (50, 1 << 12)])
+def test_rqmc_replicates_fp64_match_oracle(product, kw, m, n):
+    c = qmc_contract(product, m, n, **kw)
+    price, se, total, reps = mcdx.rqmc_path(c, FP64, 4, 77)
+    for r in range(4):
+        want = oracle_rqmc(c, 77, r)
+        assert abs(reps[r] - want) <= 1e-12 * max(1.0, abs(want)), (r, reps[r], want)
+    # the estimate and its error from the replicate means
+    assert total == 4 * n
+    assert abs(price - reps.mean()) <= 1e-14 * max(1.0, abs(price))
+    assert abs(se - reps.std(ddof=1) / 2.0) <= 1e-12 * max(1e-3, se)
+
+
+@pytest.mark.parametrize("product,kw", RQMC_CASES)
+def test_rqmc_fp32_close_to_fp64_on_the_same_points(product, kw):
+    c = qmc_contract(product, 252, 1 << 14, **kw)
+    _, _, _, r32 = mcdx.rqmc_path(c, FP32, 8, 5)
+    _, _, _, r64 = mcdx.rqmc_path(c, FP64, 8, 5)
+    assert np.all(np.abs(r32 - r64) <= 2e-4 * np.maximum(1.0, np.abs(r64))), (r32, r64)
+
+
+@pytest.mark.parametrize("product,kw", RQMC_CASES)
+def test_rqmc_unbiased_against_mc_and_tighter(product, kw):
+    # 16 replicates of 2^16 points against 2^22 Monte Carlo paths: the estimates agree within
+    # the combined error, and the randomized QMC error is well below Monte Carlo's at the
+    # same number of points (the shifts keep the net structure).
+    c = qmc_contract(product, 252, 1 << 16, **kw)
+    q, qse, total, _ = mcdx.rqmc_path(c, FP32, 16, 2024)
+    m = mcdx.GbmContract.from_buffer_copy(c)
+    m.path_count = 1 << 22
+    m.seed = 42
+    p, se, _ = mcdx.price(m, FP32)
+    assert abs(q - p) <= 4 * np.hypot(qse, se) + 1e-4, (q, qse, p, se)
+    se_mc_same_points = se * np.sqrt((1 << 22) / total)
+    assert qse < 0.5 * se_mc_same_points, (qse, se_mc_same_points)
+
+
+def test_rqmc_european_within_error_of_black_scholes():
+    import mcd
+    c = qmc_contract(mcdx.EUROPEAN, 1, 1 << 18)
+    q, qse, _, _ = mcdx.rqmc_path(c, FP64, 32, 9)
+    bs = mcd.black_scholes_merton(100.0, 100.0, 0.05, 0.0, 0.25, 1.0, mcd.OptionType.Call)
+    assert abs(q - bs) <= 4 * qse + 1e-9, (q, qse, bs)
+    assert qse < 1e-3
+
+
+def test_rqmc_deterministic_by_seed_and_rejects():
+    c = qmc_contract(mcdx.ASIAN, 50, 5000)
+    a = mcdx.rqmc_path(c, FP32, 6, 11)
+    b = mcdx.rqmc_path(c, FP32, 6, 11)
+    d = mcdx.rqmc_path(c, FP32, 6, 12)
+    assert a[:3] == b[:3] and np.array_equal(a[3], b[3])
+    assert not np.array_equal(a[3], d[3])
+    r = mcdx.Result()
+    f = mcdx.lib().mcdx_rqmc_sobol_path
+    assert f(C.byref(c), FP32, 1, 11, C.byref(r), None) == mcdx.EINVAL
+    assert f(C.byref(c), FP32, 4097, 11, C.byref(r), None) == mcdx.EINVAL
+    assert f(C.byref(c), FP32, 4, 11, None, None) == mcdx.EINVAL
+    dig = qmc_contract(mcdx.DIGITAL, 1, 5000)
+    assert f(C.byref(dig), FP32, 4, 11, C.byref(r), None) == mcdx.EINVAL
+    bb = qmc_contract(mcdx.BARRIER, 50, 5000, barrier=120.0, brownian_bridge=1)
+    assert f(C.byref(bb), FP32, 4, 11, C.byref(r), None) == mcdx.EINVAL
+    c.path_count = 0
+    assert f(C.byref(c), FP32, 4, 11, C.byref(r), None) == mcdx.OK and r.path_count == 0

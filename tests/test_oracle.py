@@ -277,6 +277,26 @@ def test_qmc_prices_match_reference_fixture():
     assert L.orc_qmc_sobol_asian(100, 95, .05, 0, .3, 1, 1, 1, 5, 5000) == fh(q["asian5_float_put"])
 
 
+def test_rqmc_shift_keeps_stratification_and_gives_an_error_estimate():
+    # A digital shift (XOR) maps each dyadic cell of width 2^-m onto another one, so the
+    # shifted first 2^m points still hit every cell once; an additive shift would not.
+    for d in (0, 1, 6, 7, 100):
+        for r in range(3):
+            sh = int(L.orc_rqmc_shift(99, r, d))
+            assert sh == int(L.orc_philox_bits(99, r, d, 3))
+            pts = np.array([L.orc_sobol_int(d, i) ^ sh for i in range(1 << 8)], dtype=np.uint64)
+            assert len(set((pts >> np.uint64(24)).tolist())) == 1 << 8, (d, r)
+    # Replicates of a European call (dimension 0, 1023 points each) scatter around
+    # Black-Scholes far less than Monte Carlo at the same number of points.
+    p = O.product(O.EUROPEAN, strike=100.0, rate=0.05, vol=0.25, monitoring_points=1)
+    reps = np.array([L.orc_rqmc_replicate(C.byref(p), 1023, 5, r) for r in range(16)])
+    bs = L.orc_bsm(100, 100, 0.05, 0, 0.25, 1, 0)
+    se = reps.std(ddof=1) / 4.0
+    assert abs(reps.mean() - bs) <= 4 * se + 1e-9, (reps.mean(), se, bs)
+    mc = O.price(p, 16 * 1023, 5)
+    assert se < 0.2 * mc[1], (se, mc[1])
+
+
 # ----------------------------------------------------------------------------- Heston
 LEWIS_CALLS = {80.0: 26.774758743998854221, 90.0: 20.933349000596710388,
                100.0: 16.070154917028834278, 110.0: 12.132211516709844867,
