@@ -216,8 +216,19 @@ def extras(torch, mcdx, L, stream, handle, ref):
     q32 = mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP32)[0]
     wall = time.perf_counter() - t0
     q64 = mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP64)[0]
+    # Randomized QMC on the same budget: 16 digitally shifted replicates of 2^16 points, which
+    # gives the error estimate plain QMC lacks.
+    asian = mcdx.contract(mcdx.ASIAN, strike=100.0, rate=0.05, vol=0.2, time=1.0,
+                          monitoring_points=STEPS, path_count=1 << 16)
+    mcdx.rqmc_path(asian, mcdx.PREC_FP32, 16, SEED)
+    t0 = time.perf_counter()
+    rq, rq_se, rq_n, _ = mcdx.rqmc_path(asian, mcdx.PREC_FP32, 16, SEED)
+    rq_wall = time.perf_counter() - t0
     res["c2_qmc_sobol_bb_asian"] = {"paths_per_s_wall": (1 << 20) / wall, "ms_wall": wall * 1e3,
-                                    "price_fp32": q32, "price_fp64": q64}
+                                    "price_fp32": q32, "price_fp64": q64,
+                                    "randomized_16x2p16": {"price": rq, "se": rq_se,
+                                                           "points": rq_n,
+                                                           "ms_wall": rq_wall * 1e3}}
     progress("C2 QMC done")
     # C3: Heston QE, 2^22 x 252, 9 strikes 80..120 from one pass (wall clock).
     par = mcdx.heston()
