@@ -312,8 +312,12 @@ def c4_entry(torch, mcdx, L, stream, handle):
         tot += 1
         ok += abs(p - cf) <= 3 * se
     out["terminal_contracts_within_3se_of_closed_form"] = [ok, tot]
+    steps = sum(c.path_count * (c.monitoring_points if c.product in
+                                (mcdx.ASIAN, mcdx.BARRIER, mcdx.LOOKBACK) else 1) for c in crn)
     out.update(seeds="one seed (42) for the book", batched_ms=b_ms, per_contract_ms=p_ms,
                speedup=p_ms / b_ms, paths=total, batched_paths_per_s=total / (b_ms / 1e3),
+               contract_path_steps=steps,
+               batched_contract_path_steps_per_s=steps / (b_ms / 1e3),
                autotuned_schedule=sched, autotune_best_ms=tuned,
                bitwise_equal_to_per_contract=bres == pres)
     dist = mcdx.c4_book()
@@ -376,7 +380,13 @@ def c5_lsm(mcdx, mcd):
         "paths": n, "exercise_dates": 50, "crr_10000": crr,
         "qr_lsm": {"price": qp, "se": qse, "ms_wall": qr_ms, "paths_per_s": n / (qr_ms / 1e3),
                    "dev_from_crr_in_se": (qp - crr) / qse,
-                   "store_bytes_algorithmic": store_bytes},
+                   "store_bytes_algorithmic": store_bytes,
+                   # the path store against the HBM roofline over the whole pricing: the
+                   # chain of 49 dependent regressions, not HBM, bounds this pass
+                   "hbm_roofline": {"achieved": store_bytes / (qr_ms / 1e3) / 1e9,
+                                    "peak": 8000.0, "unit": "GB/s",
+                                    "frac": store_bytes / (qr_ms / 1e3) / 8e12,
+                                    "bound": "latency (dependent regressions)"}},
         "learned_same_paths": {"price": lp, "se": lse, "diff_vs_qr_in_se": (lp - qp) / qse,
                                "qr_forward_replay": d["qr_forward"][0],
                                "disagreement_rate": d["disagreements"] / max(1, d["itm_decisions"]),
