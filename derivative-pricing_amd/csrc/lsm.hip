@@ -126,6 +126,8 @@ __host__ __device__ __forceinline__ double intrinsic_v(double s, double k, int t
 
 struct LsmArgs {
     double spot, strike, disc, dt;
+    double inv_strike;     // 1/K: the regression basis is taken at S * (1/K) (one multiply
+                           // per use instead of a division; every decision uses the same)
     double drift, vsd;     // FP64: per-step GBM constants (gbm.cpp:8-10 with dt)
     float mu2, sig2;       // FP32: log2-space step
     uint64_t seed, n_paths;
@@ -232,7 +234,7 @@ __device__ __forceinline__ double decide(const LsmArgs& a, double s1, double cf,
     const double ex = intrinsic_v(s1, a.strike, a.type);
     if (ex > 0.0) {
         double b[4];
-        laguerre4(s1 / a.strike, b);
+        laguerre4(s1 * a.inv_strike, b);
         double cont = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
@@ -248,7 +250,7 @@ __device__ __forceinline__ bool fold_row(const LsmArgs& a, int date, uint64_t p,
                                          double s0, double cf, double* row) {
     const bool itm = in && intrinsic_v(s0, a.strike, a.type) > 0.0;
     double b[4];
-    laguerre4(s0 / a.strike, b);
+    laguerre4(s0 * a.inv_strike, b);
 #pragma unroll
     for (int k = 0; k < 4; ++k) row[k] = itm ? b[k] : 0.0;
     if (a.cap_y && in) {
@@ -299,18 +301,9 @@ __device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t n
     QrState acc = {};
     if (b < e) acc = parts[b];
     for (uint32_t i = b + 1; i < e; ++i) acc = qr_merge(acc, parts[i]);
-    __syncthreads();
-    lds[g] = acc;
-    __syncthreads();
-    for (uint32_t s = 1; s < 256; s <<= 1) {
-        QrState m;
-        const bool act = (g & (2 * s - 1)) == 0;
-        if (act) m = qr_merge(lds[g], lds[g + s]);
-        __syncthreads();
-        if (act) lds[g] = m;
-        __syncthreads();
-    }
-    const QrState t = lds[0];
+    // The pairwise tree over the 256 groups: shuffles inside each wave, then the four wave
+    // totals (the same merge order as a tree over an LDS array, without its barriers).
+    const QrState t = block_tree_all(acc, lds);
     *has = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) beta[i] = 0.0;
@@ -496,7 +489,7 @@ __device__ __forceinline__ void policy_visit(const LsmArgs& a, const PolicyArgs&
         double b0, b1, b2, b3;
         {
             double b[4];
-            laguerre4(s / a.strike, b);
+            laguerre4(s * a.inv_strike, b);
             b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
         }
         const double* beta = a.coeffs + 4 * date;
@@ -682,7 +675,7 @@ __device__ __forceinline__ void frozen_visit(const LsmArgs& a, Stop& st, int dat
     }
     if (ex > 0.0 && a.has[date]) {
         double b[4];
-        laguerre4(s / a.strike, b);
+        laguerre4(s * a.inv_strike, b);
         const double* beta = a.coeffs + 4 * date;
         double cont = 0.0;
 #pragma unroll
@@ -818,6 +811,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     const double dt = time / (double)m;
     a.spot = spot;
     a.strike = strike;
+    a.inv_strike = 1.0 / strike;
     a.dt = dt;
     a.disc = exp(-rate * dt);
     a.drift = (rate - carry - 0.5 * vol * vol) * dt;
