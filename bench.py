@@ -224,7 +224,25 @@ def extras(torch, mcdx, L, stream, handle, ref):
     t0 = time.perf_counter()
     rq, rq_se, rq_n, _ = mcdx.rqmc_path(asian, mcdx.PREC_FP32, 16, SEED)
     rq_wall = time.perf_counter() - t0
+    # QMC over the bridge for the other two headline payoffs (the reference prices only the
+    # Asian by QMC), against the reference's own FP64 Monte Carlo price at 2^24 paths.
+    qpaths = {}
+    for name, kw, key in (("barrier_uo", dict(barrier=120.0, direction=0, knock=1), "c2_barrier_uo_call"),
+                          ("lookback_float", dict(strike_style=1), "c2_lookback_float_call")):
+        prod = mcdx.BARRIER if name == "barrier_uo" else mcdx.LOOKBACK
+        qc = mcdx.contract(prod, strike=100.0, rate=0.05, vol=0.2, time=1.0,
+                           monitoring_points=STEPS, path_count=1 << 20, **kw)
+        mcdx.qmc_path(qc, mcdx.PREC_FP32)
+        t0 = time.perf_counter()
+        qv = mcdx.qmc_path(qc, mcdx.PREC_FP32)
+        qms = (time.perf_counter() - t0) * 1e3
+        entry = {"ms_wall": qms, "price_fp32": qv}
+        if ref and key in ref:
+            rp, rse = float.fromhex(ref[key]["price"]), float.fromhex(ref[key]["se"])
+            entry.update(reference_mc_fp64=rp, reference_mc_se=rse, dev_in_reference_se=(qv - rp) / rse)
+        qpaths[name] = entry
     res["c2_qmc_sobol_bb_asian"] = {"paths_per_s_wall": (1 << 20) / wall, "ms_wall": wall * 1e3,
+                                    "other_payoffs_2p20x252": qpaths,
                                     "price_fp32": q32, "price_fp64": q64,
                                     "randomized_16x2p16": {"price": rq, "se": rq_se,
                                                            "points": rq_n,
@@ -372,6 +390,17 @@ def c5_lsm(mcdx, mcd):
     judged = mcdx.lsm_learned_price(*put, n, SEED + 1, mcdx.PREC_FP32, w, 1024, d["table_lo"],
                                     d["table_hi"], co, has)
     crr = mcd.crr_binomial_american(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 10_000, mcd.OptionType.Put)
+    # Frozen-policy American Greeks (one fused pass of 5 bumped scenarios over the QR policy),
+    # against central differences of CRR-2000 with the same bumps (1% spot, 1% vol).
+    gk_ms, gk = med(lambda: mcdx.lsm_greeks(*put, n, SEED, mcdx.PREC_FP32))
+
+    def crr_at(s=100.0, v=0.2):
+        return mcd.crr_binomial_american(s, 100.0, 0.06, 0.0, v, 1.0, 2000, mcd.OptionType.Put)
+    hs, hv = 1.0, 0.002
+    c0 = crr_at()
+    crr_greeks = {"delta": (crr_at(s=100 + hs) - crr_at(s=100 - hs)) / (2 * hs),
+                  "gamma": (crr_at(s=100 + hs) - 2 * c0 + crr_at(s=100 - hs)) / hs ** 2,
+                  "vega": (crr_at(v=0.2 + hv) - crr_at(v=0.2 - hv)) / (2 * hv)}
     qp, qse = q[0], q[1]
     lp, lse, _ = d["learned"]
     sp, sse, _ = sv["learned"]
@@ -397,10 +426,19 @@ def c5_lsm(mcdx, mcd):
                                         "dev_from_crr_in_se": (sp - crr) / sse,
                                         "diff_vs_qr_lsm_in_se": (sp - qp) / qse,
                                         "qr_policy_same_paths": judged["qr_forward"][0],
+                                        "diff_vs_qr_policy_same_paths_in_se":
+                                            (sp - judged["qr_forward"][0]) / sse,
+                                        "note": "out-of-sample exercise policy on fresh paths: "
+                                                "a low-biased estimator, like the QR policy on "
+                                                "the same paths (qr_policy_same_paths)",
                                         "disagreement_rate": judged["disagreements"] /
                                         max(1, judged["itm_decisions"]),
                                         "fallbacks": sv["fallbacks"]},
         "serving_speedup_vs_qr_lsm": qr_ms / serve_ms,
+        "american_greeks_frozen_policy": {
+            "ms_wall": gk_ms,
+            "price": gk[0][:2], "delta": gk[1][:2], "gamma": gk[2][:2], "vega": gk[3][:2],
+            "crr_2000_central_differences": crr_greeks},
     }
 
 
