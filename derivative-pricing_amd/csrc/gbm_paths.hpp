@@ -21,6 +21,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #include "runtime.hpp"
 
 namespace mcdx {
@@ -87,16 +89,18 @@ struct StateF32 {
     bool breached;
 };
 
-template <int KIND>
-__device__ __forceinline__ void observe_f32(const GbmArgs& a, StateF32& s, bool need_arith,
-                                            bool need_geo) {
+// NEED_ARITH / NEED_GEO: which Asian sums the contract needs, as template arguments so the
+// step loop carries no per-step test (a runtime flag here is if-converted into FP64 work on
+// every step).
+template <int KIND, bool NEED_ARITH, bool NEED_GEO>
+__device__ __forceinline__ void observe_f32(const GbmArgs& a, StateF32& s) {
     if constexpr (KIND == kAsian) {
-        if (need_arith) {
+        if constexpr (NEED_ARITH) {
             const float u = fast_exp2(s.x);
             s.sum_u += (double)u;
             s.last_u = u;
         }
-        if (need_geo) s.sum_x += (double)s.x;
+        if constexpr (NEED_GEO) s.sum_x += (double)s.x;
     } else if constexpr (KIND == kBarrier) {
         const bool crossed = a.direction == 0 ? s.x >= a.xb : s.x <= a.xb;
         s.breached = s.breached || crossed;
@@ -148,6 +152,9 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
         const bool need_geo = a.average_style == 1 || a.cv;
         StateF32 s = {0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
         StateF32 t = s;
+        // The step loop, instantiated per combination of the Asian sums it needs.
+        auto steps = [&](auto arith_tag, auto geo_tag) {
+        constexpr bool NA = decltype(arith_tag)::value, NG = decltype(geo_tag)::value;
         // One monitoring step given its draw (draw index = step, monte_carlo.cpp:166-167).
         auto apply = [&](uint32_t draw, float z) {
             if constexpr (KIND == kBarrierBB) {
@@ -178,10 +185,10 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
                 if (ANTI) t.x = xt;
             } else {
                 s.x = s.x + fmaf(a.sig2, z, a.mu2);
-                observe_f32<KIND>(a, s, need_arith, need_geo);
+                observe_f32<KIND, NA, NG>(a, s);
                 if (ANTI) {
                     t.x = t.x + fmaf(-a.sig2, z, a.mu2);
-                    observe_f32<KIND>(a, t, need_arith, need_geo);
+                    observe_f32<KIND, NA, NG>(a, t);
                 }
             }
         };
@@ -195,6 +202,16 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
         }
         for (; step < a.m; ++step)
             apply((uint32_t)step, inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        if constexpr (KIND == kAsian) {
+            if (need_arith && need_geo) steps(T{}, T{});
+            else if (need_arith) steps(T{}, F{});
+            else steps(F{}, T{});
+        } else {
+            steps(F{}, F{});
+        }
         double r = settle_f32<KIND>(a, s);
         if (ANTI) r = 0.5 * (r + settle_f32<KIND>(a, t));
         return r;
