@@ -35,8 +35,8 @@ SEED = 42
 # 10 x (2 mul_hi + 2 mul_lo + 4 xor) = 80; inverse normal (central branch, 99.9% of draws)
 # = fold u to min(u, 1-u) 2 + convert 1 + 4m 2 + 4m(1-m) 3 + log2 1 + degree-9 Horner 18 +
 # |q| 2 + product 1 + sign 2 + tail test 1 = 33; log-space GBM step = 1 FMA + 1 add = 3;
-# payoff observation: arithmetic average exp2 + cvt + f64 add = 3, barrier compare + or = 2,
-# running max + min = 2.
+# payoff observation: arithmetic average exp2 + cvt + f64 add = 3, discrete barrier and
+# lookback running max + min = 2 (the barrier is decided from the extrema once per path).
 OPS_DRAW = 80 + 33
 OPS_GBM = 3
 OPS_OBSERVE = {"asian": 3, "barrier_uo": 2, "lookback_float": 2}

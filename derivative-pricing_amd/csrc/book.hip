@@ -132,17 +132,19 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             // Step constants, read once per item for every member (A[i] is always a valid
             // contract), so the step loop runs on registers: observe_f32's arithmetic with
             // the member's own constants.
-            bool na[kFuseMax], ng[kFuseMax], up[kFuseMax];
-            float mu2[kFuseMax], sig2[kFuseMax], xb[kFuseMax];
+            bool na[kFuseMax], ng[kFuseMax];
+            float mu2[kFuseMax], sig2[kFuseMax];
 #pragma unroll
             for (int i = 0; i < kFuseMax; ++i) {
                 s[i] = StateF32{0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
+                if constexpr (KIND == kBarrier) { // extrema over the monitoring dates only
+                    s[i].xmax = -__builtin_inff();
+                    s[i].xmin = __builtin_inff();
+                }
                 na[i] = A[i]->average_style == 0 || A[i]->cv;
                 ng[i] = A[i]->average_style == 1 || A[i]->cv;
-                up[i] = A[i]->direction == 0;
                 mu2[i] = A[i]->mu2;
                 sig2[i] = A[i]->sig2;
-                xb[i] = A[i]->xb;
             }
             auto apply = [&](float z) {
 #pragma unroll
@@ -157,10 +159,7 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                                 t.last_u = u;
                             }
                             if (ng[i]) t.sum_x += (double)t.x;
-                        } else if constexpr (KIND == kBarrier) {
-                            const bool crossed = up[i] ? t.x >= xb[i] : t.x <= xb[i];
-                            t.breached = t.breached || crossed;
-                        } else if constexpr (KIND == kLookback) {
+                        } else if constexpr (KIND == kBarrier || KIND == kLookback) {
                             t.xmax = fmaxf(t.xmax, t.x);
                             t.xmin = fminf(t.xmin, t.x);
                         }

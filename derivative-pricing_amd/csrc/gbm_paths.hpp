@@ -82,7 +82,8 @@ __device__ __forceinline__ double lookback_payoff(const GbmArgs& a, double last,
 // ---------------------------------------------------------------------------
 struct StateF32 {
     float x;          // log2(S/S0)
-    float xmax, xmin; // lookback extrema in log2 space
+    float xmax, xmin; // extrema in log2 space: lookback (seeded at S0), discrete barrier
+                      // (monitoring dates only)
     float last_u;     // S/S0 at the last step (arithmetic Asian)
     double sum_u;     // sum of S/S0 over monitoring dates
     double sum_x;     // sum of log2(S/S0) (geometric average)
@@ -101,10 +102,10 @@ __device__ __forceinline__ void observe_f32(const GbmArgs& a, StateF32& s) {
             s.last_u = u;
         }
         if constexpr (NEED_GEO) s.sum_x += (double)s.x;
-    } else if constexpr (KIND == kBarrier) {
-        const bool crossed = a.direction == 0 ? s.x >= a.xb : s.x <= a.xb;
-        s.breached = s.breached || crossed;
-    } else if constexpr (KIND == kLookback) {
+    } else if constexpr (KIND == kBarrier || KIND == kLookback) {
+        // A discrete barrier is breached iff some monitoring date's x crosses xb, i.e. iff
+        // the largest (up) or smallest (down) x over the dates does: the same comparisons on
+        // the same values, taken once in settle_f32 instead of on every step.
         s.xmax = fmaxf(s.xmax, s.x);
         s.xmin = fminf(s.xmin, s.x);
     }
@@ -123,7 +124,10 @@ __device__ __forceinline__ double settle_f32(const GbmArgs& a, const StateF32& s
             return ap - (gp - a.kv);
         }
         return a.discount * asian_payoff(a, a.average_style == 0 ? arith : geo, last);
-    } else if constexpr (KIND == kBarrier || KIND == kBarrierBB) {
+    } else if constexpr (KIND == kBarrier) {
+        const bool breached = a.direction == 0 ? s.xmax >= a.xb : s.xmin <= a.xb;
+        return a.discount * barrier_payoff(a, a.spot * (double)fast_exp2(s.x), breached);
+    } else if constexpr (KIND == kBarrierBB) {
         return a.discount * barrier_payoff(a, a.spot * (double)fast_exp2(s.x), s.breached);
     } else {
         const double last = a.spot * (double)fast_exp2(s.x);
@@ -151,6 +155,10 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path) {
         const bool need_arith = a.average_style == 0 || a.cv;
         const bool need_geo = a.average_style == 1 || a.cv;
         StateF32 s = {0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
+        if constexpr (KIND == kBarrier) { // the barrier is watched on monitoring dates only
+            s.xmax = -__builtin_inff();
+            s.xmin = __builtin_inff();
+        }
         StateF32 t = s;
         // The step loop, instantiated per combination of the Asian sums it needs.
         auto steps = [&](auto arith_tag, auto geo_tag) {
