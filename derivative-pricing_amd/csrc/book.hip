@@ -155,7 +155,7 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                         if constexpr (KIND == kAsian) {
                             if (na[i]) {
                                 const float u = fast_exp2(t.x);
-                                t.sum_u += (double)u;
+                                t.blk_u += u;
                                 t.last_u = u;
                             }
                             if (ng[i]) t.sum_x += (double)t.x;
@@ -169,13 +169,23 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             const int m = (int)g.m;
             constexpr int B = 8;
             int step = 0;
+            // flush_f32's block sums, at the same dates as path_f32's
+            auto flush = [&]() {
+                if constexpr (KIND == kAsian) {
+#pragma unroll
+                    for (int i = 0; i < kFuseMax; ++i)
+                        if (i < kact && na[i]) flush_f32<kAsian, true>(s[i]);
+                }
+            };
             for (; step + B <= m; step += B) {
                 float z[B];
                 normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
 #pragma unroll
                 for (int j = 0; j < B; ++j) apply(z[j]);
+                flush();
             }
             for (; step < m; ++step) apply(inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+            flush();
 #pragma unroll
             for (int i = 0; i < kFuseMax; ++i) v[i] = i < kact ? settle_f32<KIND>(*A[i], s[i]) : 0.0;
         }
