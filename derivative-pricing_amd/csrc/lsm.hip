@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "mlp.hpp"
+#include "qmc_store.hpp"
 #include "runtime.hpp"
 
 namespace mcdx {
@@ -856,15 +857,32 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     return MCDX_OK;
 }
 
+// Cash flows at maturity and a cleared policy, once the store holds the paths.
 template <bool FP64>
-hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
+hipError_t enqueue_init(const LsmRun& r, hipStream_t s) {
     const unsigned g = (unsigned)r.blocks;
-    hipLaunchKernelGGL(lsm_paths_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
     hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
     hipLaunchKernelGGL(lsm_clear_policy_kernel, dim3((r.a.m + 1 + 255) / 256), dim3(256), 0, s,
                        r.a.has, r.a.coeffs, r.a.m);
     return hipGetLastError();
 }
+
+template <bool FP64>
+hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
+    const unsigned g = (unsigned)r.blocks;
+    hipLaunchKernelGGL(lsm_paths_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? enqueue_init<FP64>(r, s) : e;
+}
+
+// Where the store's paths come from: Philox (the reference's, lsm.cpp:35-58) or Sobol +
+// Brownian bridge, optionally digitally shifted (replicate `replicate` of key shift_seed).
+struct PathSource {
+    bool qmc = false;
+    bool shifted = false;
+    uint64_t shift_seed = 0;
+    uint32_t replicate = 0;
+};
 
 // Backward induction with regression (fit) or with a given policy already in coeffs/has.
 template <bool FP64>
@@ -884,20 +902,31 @@ hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
 
 } // namespace
 
-int lsm_price(double spot, double strike, double rate, double carry, double vol, double time,
-              uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, mcdx_result* out,
-              double* coeffs_out, uint8_t* has_out, uint8_t* inception_out) {
+int lsm_price_src(double spot, double strike, double rate, double carry, double vol, double time,
+                  uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision,
+                  const PathSource& src, mcdx_result* out, double* coeffs_out, uint8_t* has_out,
+                  uint8_t* inception_out) {
     if (!out) return MCDX_EINVAL;
     ThreadCtx* ctx;
     int st = thread_ctx(&ctx);
     if (st) return st;
     LsmRun r;
-    if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r)))
+    char* scratch = nullptr;
+    if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r,
+                        src.qmc ? qmc_store_scratch_bytes(m) : 0, src.qmc ? &scratch : nullptr)))
         return st;
     const size_t pol_bytes = (size_t)(m + 1) * 4 * 8 + (size_t)(m + 1);
     if ((st = ensure_pinned(ctx, sizeof(Welford) + pol_bytes + 64))) return st;
     const bool fp64 = precision == MCDX_PREC_FP64;
-    hipError_t e = fp64 ? enqueue_paths<true>(r, ctx->stream) : enqueue_paths<false>(r, ctx->stream);
+    hipError_t e;
+    if (src.qmc) {
+        if ((st = qmc_fill_store(spot, rate, carry, vol, time, m, n, fp64, src.shifted,
+                                 src.shift_seed, src.replicate, r.a.store, scratch, ctx->stream)))
+            return st;
+        e = fp64 ? enqueue_init<true>(r, ctx->stream) : enqueue_init<false>(r, ctx->stream);
+    } else {
+        e = fp64 ? enqueue_paths<true>(r, ctx->stream) : enqueue_paths<false>(r, ctx->stream);
+    }
     if (e == hipSuccess)
         e = fp64 ? enqueue_backward<true>(r, true, ctx->stream) : enqueue_backward<false>(r, true, ctx->stream);
     char* h = (char*)ctx->pinned;
@@ -926,6 +955,42 @@ int lsm_price(double spot, double strike, double rate, double carry, double vol,
         if (inception_out) *inception_out = 0;
         *out = res;
     }
+    return MCDX_OK;
+}
+
+int lsm_price(double spot, double strike, double rate, double carry, double vol, double time,
+              uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, mcdx_result* out,
+              double* coeffs_out, uint8_t* has_out, uint8_t* inception_out) {
+    return lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, seed, precision,
+                         PathSource{}, out, coeffs_out, has_out, inception_out);
+}
+
+// QR-LSM on Sobol + bridge paths: one plain run (replicates == 0) or `replicates` digitally
+// shifted runs whose prices give the estimate and its standard error.
+int lsm_qmc(double spot, double strike, double rate, double carry, double vol, double time,
+            uint8_t type, int32_t m, uint64_t n, uint32_t replicates, uint64_t shift_seed,
+            int precision, mcdx_result* out, double* replicate_prices) {
+    if (!out || replicates == 1 || replicates > 4096 || n > (1ull << 32) - 2) return MCDX_EINVAL;
+    if (replicates == 0)
+        return lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, 0, precision,
+                             PathSource{true, false, 0, 0}, out, nullptr, nullptr, nullptr);
+    double sum = 0.0, ss = 0.0;
+    std::vector<double> prices(replicates);
+    for (uint32_t r = 0; r < replicates; ++r) {
+        mcdx_result one;
+        const int st = lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, 0, precision,
+                                     PathSource{true, true, shift_seed, r}, &one, nullptr,
+                                     nullptr, nullptr);
+        if (st) return st;
+        prices[r] = one.price;
+        sum += one.price;
+    }
+    const double mean = sum / (double)replicates;
+    for (uint32_t r = 0; r < replicates; ++r) ss += (prices[r] - mean) * (prices[r] - mean);
+    if (replicate_prices)
+        for (uint32_t r = 0; r < replicates; ++r) replicate_prices[r] = prices[r];
+    *out = mcdx_result{mean, sqrt(ss / (double)(replicates - 1) / (double)replicates),
+                       n * replicates};
     return MCDX_OK;
 }
 
@@ -1326,6 +1391,14 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
 using namespace mcdx;
 
 extern "C" {
+
+int mcdx_lsm_american_qmc(double spot, double strike, double rate, double carry_yield, double vol,
+                          double time, uint8_t type, int32_t monitoring_points,
+                          uint64_t path_count, uint32_t replicates, uint64_t shift_seed,
+                          int precision, mcdx_result* out, double* replicate_prices) {
+    return lsm_qmc(spot, strike, rate, carry_yield, vol, time, type, monitoring_points, path_count,
+                   replicates, shift_seed, precision, out, replicate_prices);
+}
 
 int mcdx_lsm_american(double spot, double strike, double rate, double carry_yield, double vol,
                       double time, uint8_t type, int32_t monitoring_points, uint64_t path_count,

@@ -29,6 +29,7 @@
 #include <mutex>
 #include <vector>
 
+#include "qmc_store.hpp"
 #include "runtime.hpp"
 
 namespace mcdx {
@@ -143,7 +144,8 @@ struct BridgeOp {
     double frac;     // (t_mid - t_left) / (t_right - t_left)
     double sd;       // sqrt((t_mid - t_left)(t_right - t_mid) / (t_right - t_left))
     double drift_t;  // (r - q - vol^2/2) * t_{point+1}
-    float frac_f, sd_f, drift2_f, pad;
+    float frac_f, sd_f, drift2_f;
+    uint32_t point;  // the date index (0..n-1) whose bridge value this op produces
     // Whole 32-bit words, so the device reads them through the scalar unit (it has no
     // sub-word loads): the Sobol dimension (= the reference's draw index for this split) and
     // the LDS slots packed as left | right << 8 | output << 16.
@@ -208,6 +210,7 @@ int build_bridge(int n, double spot_unused, double rate, double carry, double vo
         op.drift2_f = (float)(op.drift_t * log2e);
         op.dim = (uint32_t)dim;
         op.slots = (uint32_t)sl | (uint32_t)sr << 8 | (uint32_t)so << 16;
+        op.point = (uint32_t)point;
         ops.push_back(op);
     };
     // Terminal point: w[n-1] = sqrt(t[n]) * z_0 (bridge.cpp:30).
@@ -702,6 +705,55 @@ int qmc_path(const mcdx_gbm_contract* c, int precision, mcdx_result* out) {
     return qmc_run(a, precision, ops, slots, out);
 }
 
+// One Sobol index per thread (index p + 1 for path p), 256 threads, bridge values in LDS
+// slots [slot][thread]; each op's spot goes to its date's store column (coalesced across the
+// block). Any index layout works here: the integer is folded from its own Gray code.
+template <bool FP64, bool SHIFTED>
+__global__ __launch_bounds__(256) void qmc_store_kernel(const uint32_t* __restrict__ v,
+                                                        const BridgeOp* __restrict__ ops,
+                                                        int32_t n_ops,
+                                                        const uint32_t* __restrict__ shift,
+                                                        double spot, double vol, float vol2,
+                                                        uint64_t n, void* store) {
+    extern __shared__ unsigned char smem[];
+    const uint32_t t = threadIdx.x;
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + t;
+    if (p >= n) return; // no barrier follows
+    const uint64_t i = p + 1;
+    const uint32_t g = (uint32_t)(i ^ (i >> 1));
+    if constexpr (FP64) {
+        double* w_lds = (double*)smem;
+        w_lds[t] = 0.0;
+        for (int k = 0; k < n_ops; ++k) {
+            const BridgeOp& op = ops[k];
+            uint32_t x = sobol_fold(v + 32 * op.dim, g);
+            if (SHIFTED) x ^= shift[op.dim];
+            const double z = qmc_normal_f64<true, SHIFTED>(x);
+            const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
+            const double wl = w_lds[sl * 256 + t], wr = w_lds[sr * 256 + t];
+            const double w = (wl + op.frac * (wr - wl)) + op.sd * z;
+            w_lds[so * 256 + t] = w;
+            ((double*)store)[(uint64_t)op.point * n + p] = spot * exp(op.drift_t + vol * w);
+        }
+    } else {
+        float* w_lds = (float*)smem;
+        w_lds[t] = 0.f;
+        const float s0 = (float)spot;
+        for (int k = 0; k < n_ops; ++k) {
+            const BridgeOp& op = ops[k];
+            uint32_t x = sobol_fold(v + 32 * op.dim, g);
+            if (SHIFTED) x ^= shift[op.dim];
+            const float z = qmc_normal_f32<SHIFTED>(x);
+            const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
+            const float wl = w_lds[sl * 256 + t], wr = w_lds[sr * 256 + t];
+            const float w = __builtin_fmaf(op.sd_f, z, __builtin_fmaf(op.frac_f, wr - wl, wl));
+            w_lds[so * 256 + t] = w;
+            ((float*)store)[(uint64_t)op.point * n + p] =
+                s0 * fast_exp2(__builtin_fmaf(vol2, w, op.drift2_f));
+        }
+    }
+}
+
 constexpr uint32_t kMaxReplicates = 4096;
 
 // Randomized QMC: `replicates` independent digital shifts of the same point set; the price
@@ -764,6 +816,57 @@ int rqmc_path(const mcdx_gbm_contract* c, int precision, uint32_t replicates,
         for (uint32_t r = 0; r < replicates; ++r) replicate_prices[r] = reps[r].price;
     *out = mcdx_result{mean, se, c->path_count * replicates};
     return MCDX_OK;
+}
+
+size_t qmc_store_scratch_bytes(int32_t m) {
+    const size_t ops = ((size_t)(m > 0 ? m : 1) * sizeof(BridgeOp) + 255) & ~(size_t)255;
+    const size_t sh = ((size_t)(m > 0 ? m : 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    return ops + sh;
+}
+
+int qmc_fill_store(double spot, double rate, double carry, double vol, double time, int32_t m,
+                   uint64_t n, bool fp64, bool shifted, uint64_t shift_seed, uint32_t replicate,
+                   void* store, void* scratch, hipStream_t s) {
+    if (m < 1 || m > (int32_t)kSobolDims || n == 0 || n > (1ull << 32) - 2) return MCDX_EINVAL;
+    const uint32_t* v;
+    int st = device_tables(&v);
+    if (st) return st;
+    std::vector<BridgeOp> ops;
+    const int slots = build_bridge(m, spot, rate, carry, vol, time, ops);
+    if (slots < 0) return MCDX_EINVAL;
+    const size_t shmem = (size_t)slots * 256 * (fp64 ? sizeof(double) : sizeof(float));
+    if (shmem > 64 * 1024) return MCDX_EINVAL;
+    std::vector<uint32_t> shift((size_t)m, 0u);
+    if (shifted)
+        for (int32_t d = 0; d < m; ++d) shift[(size_t)d] = philox_bits(shift_seed, replicate, (uint32_t)d, 3u);
+    BridgeOp* dops = (BridgeOp*)scratch;
+    uint32_t* dshift = (uint32_t*)((char*)scratch + (((size_t)m * sizeof(BridgeOp) + 255) & ~(size_t)255));
+    hipError_t e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(BridgeOp),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dshift, shift.data(), shift.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s); // the host vectors end with this call
+    if (e != hipSuccess) return status_of(e);
+    const double log2e = 1.44269504088896340735992468100189214;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    const float vol2 = (float)(vol * log2e);
+    if (fp64) {
+        if (shifted)
+            hipLaunchKernelGGL((qmc_store_kernel<true, true>), dim3(grid), dim3(256), shmem, s, v, dops,
+                               (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
+        else
+            hipLaunchKernelGGL((qmc_store_kernel<true, false>), dim3(grid), dim3(256), shmem, s, v, dops,
+                               (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
+    } else {
+        if (shifted)
+            hipLaunchKernelGGL((qmc_store_kernel<false, true>), dim3(grid), dim3(256), shmem, s, v, dops,
+                               (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
+        else
+            hipLaunchKernelGGL((qmc_store_kernel<false, false>), dim3(grid), dim3(256), shmem, s, v, dops,
+                               (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
+    }
+    return status_of(hipGetLastError());
 }
 
 } // namespace mcdx

@@ -116,6 +116,8 @@ SIGNATURES = {
                                            P(dbl)]),
     "mcdx_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, C.c_int, P(Result), P(dbl),
                                                 P(u8), P(u8)]),
+    "mcdx_lsm_american_qmc": (C.c_int, [dbl] * 6 + [u8, i32, u64, u32, u64, C.c_int, P(Result),
+                                                P(dbl)]),
     "mcdx_monte_carlo_lsm_american": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(Result)]),
     "mcdx_lsm_reprice_frozen": (C.c_int, [dbl] * 6 + [u8, i32, u64, u64, P(dbl), P(u8), u8,
                                                       C.c_int, P(Result)]),
@@ -279,6 +281,17 @@ def lsm_reprice(spot, strike, rate, carry, vol, time, type, m, n, seed, coeffs, 
                                         co.ctypes.data_as(P(dbl)), hs.ctypes.data_as(P(u8)),
                                         inception, precision, C.byref(r)), "mcdx_lsm_reprice_frozen")
     return r.price
+
+
+def lsm_qmc(spot, strike, rate, carry, vol, time, type, m, n, replicates, shift_seed, precision):
+    """QR-LSM on Sobol + bridge paths: (price, se, paths, run prices)."""
+    r = Result()
+    reps = np.zeros(max(replicates, 1), dtype=np.float64)
+    check(lib().mcdx_lsm_american_qmc(spot, strike, rate, carry, vol, time, type, m, n, replicates,
+                                      shift_seed, precision, C.byref(r),
+                                      reps.ctypes.data_as(P(dbl)) if replicates else None),
+          "mcdx_lsm_american_qmc")
+    return r.price, r.standard_error, r.path_count, reps[:replicates]
 
 
 def lsm_greeks(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, hs=0.0, hv=0.0):

@@ -188,6 +188,17 @@ int mcdx_heston_path_samples(const mcdx_heston_params* params, double strike, ui
                              uint64_t path0, uint64_t n, double* out);
 
 /* ---- Longstaff-Schwartz (src/pricers/lsm.cpp) ---- */
+/* QR-LSM on Sobol + Brownian-bridge paths (no reference counterpart: the reference's LSM
+ * simulates Philox paths, lsm.cpp:35-58). Path p is Sobol index p + 1 through the bridge, the
+ * regression and decisions are mcdx_lsm_american's. replicates == 0: one run (the standard
+ * error is the spread of the path values, not an error of the estimate); replicates in
+ * 2..4096: that many runs with digitally shifted points (shift words and cell-midpoint mapping
+ * as mcdx_rqmc_sobol_path), out = the mean of the run prices with their standard error and
+ * path_count * replicates; replicate_prices (optional, [replicates]) gets each run's price. */
+int mcdx_lsm_american_qmc(double spot, double strike, double rate, double carry_yield, double vol,
+                          double time, uint8_t type, int32_t monitoring_points,
+                          uint64_t path_count, uint32_t replicates, uint64_t shift_seed,
+                          int precision, mcdx_result* out, double* replicate_prices);
 /* monte_carlo_lsm_american (src/pricers/lsm.cpp:62-149) at `precision`. The policy
  * (LsmPolicy, include/mcd/pricers/lsm.hpp:16-24) is returned as coefficients[(M+1)*4]
  * (row d = date d), has_coefficients[M+1] and exercise_at_inception; any may be NULL. */

@@ -910,14 +910,68 @@ static double* lsm_paths(double spot, double rate, double carry, double vol, dou
     return paths;
 }
 
+/* Row-major n x (steps+1) store of Sobol + Brownian-bridge paths: path i is Sobol index i + 1
+ * through brownian_bridge_path.cpp:20-77; shift == NULL gives u = x / 2^32, else the
+ * digitally shifted point at its cell midpoint, u = (x ^ shift[d] + 0.5) / 2^32. */
+static double* lsm_qmc_paths(double spot, double rate, double carry, double vol, double time,
+                             int steps, uint64_t n, const uint32_t* shift) {
+    const size_t stride = (size_t)steps + 1;
+    double* paths = (double*)malloc(sizeof(double) * (size_t)n * stride);
+    double* z = (double*)malloc(sizeof(double) * (size_t)steps);
+    double* sp = (double*)malloc(sizeof(double) * (size_t)steps);
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < steps; ++d) {
+            const uint32_t x = orc_sobol_int((unsigned)d, i + 1);
+            const double u = shift ? ((double)(x ^ shift[d]) + 0.5) / 4294967296.0
+                                   : (double)x / 4294967296.0;
+            z[d] = orc_inverse_normal_cdf(u);
+        }
+        orc_brownian_bridge_path(spot, rate, carry, vol, time, steps, z, sp);
+        const size_t row = (size_t)i * stride;
+        paths[row] = spot;
+        for (int d = 0; d < steps; ++d) paths[row + (size_t)d + 1] = sp[d];
+    }
+    free(z);
+    free(sp);
+    return paths;
+}
+
+static orc_result lsm_on_paths(double* paths, double spot, double strike, double rate,
+                               double time, int type, int steps, uint64_t n, double* coeffs,
+                               uint8_t* has_coeffs, uint8_t* exercise_at_inception);
+
 orc_result orc_lsm_american(double spot, double strike, double rate, double carry, double vol,
                             double time, int type, int steps, uint64_t n, uint64_t seed,
                             double* coeffs, uint8_t* has_coeffs,
                             uint8_t* exercise_at_inception) { /* lsm.cpp:62-149 */
+    double* paths = lsm_paths(spot, rate, carry, vol, time, steps, n, seed);
+    return lsm_on_paths(paths, spot, strike, rate, time, type, steps, n, coeffs, has_coeffs,
+                        exercise_at_inception);
+}
+
+orc_result orc_lsm_american_qmc(double spot, double strike, double rate, double carry,
+                                double vol, double time, int type, int steps, uint64_t n,
+                                int shifted, uint64_t shift_seed, uint32_t replicate,
+                                double* coeffs, uint8_t* has_coeffs,
+                                uint8_t* exercise_at_inception) {
+    uint32_t* shift = NULL;
+    if (shifted) {
+        shift = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)steps);
+        for (int d = 0; d < steps; ++d) shift[d] = orc_rqmc_shift(shift_seed, replicate, (unsigned)d);
+    }
+    double* paths = lsm_qmc_paths(spot, rate, carry, vol, time, steps, n, shift);
+    free(shift);
+    return lsm_on_paths(paths, spot, strike, rate, time, type, steps, n, coeffs, has_coeffs,
+                        exercise_at_inception);
+}
+
+/* Backward induction on a path store (lsm.cpp:62-149); frees the store. */
+static orc_result lsm_on_paths(double* paths, double spot, double strike, double rate,
+                               double time, int type, int steps, uint64_t n, double* coeffs,
+                               uint8_t* has_coeffs, uint8_t* exercise_at_inception) {
     const double dt = time / (double)steps;
     const double disc = exp(-rate * dt);
     const size_t stride = (size_t)steps + 1;
-    double* paths = lsm_paths(spot, rate, carry, vol, time, steps, n, seed);
     double* cf = (double*)malloc(sizeof(double) * (size_t)n);
     double* design = (double*)malloc(sizeof(double) * (size_t)n * 4);
     double* targets = (double*)malloc(sizeof(double) * (size_t)n);

@@ -128,6 +128,14 @@ orc_result orc_lsm_american(double spot, double strike, double rate, double carr
                             double time, int type, int steps, uint64_t path_count,
                             uint64_t seed, double* coeffs, uint8_t* has_coeffs,
                             uint8_t* exercise_at_inception);
+/* LSM on Sobol + Brownian-bridge paths (this engine's extension): path i = Sobol index i + 1;
+ * shifted != 0 applies replicate `replicate`'s digital shift (orc_rqmc_shift) at the cell
+ * midpoint. The regression is orc_lsm_american's. */
+orc_result orc_lsm_american_qmc(double spot, double strike, double rate, double carry,
+                                double vol, double time, int type, int steps,
+                                uint64_t path_count, int shifted, uint64_t shift_seed,
+                                uint32_t replicate, double* coeffs, uint8_t* has_coeffs,
+                                uint8_t* exercise_at_inception);
 double orc_lsm_reprice_frozen(double spot, double strike, double rate, double carry, double vol,
                               double time, int type, int steps, uint64_t path_count,
                               uint64_t seed, const double* coeffs, const uint8_t* has_coeffs,

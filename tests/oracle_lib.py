@@ -92,6 +92,8 @@ def lib():
             "orc_gauss_legendre_rule": (None, [i32, P(dbl), P(dbl)]),
             "orc_householder_ls": (None, [P(dbl), i32, i32, P(dbl), P(dbl)]),
             "orc_lsm_american": (Result, [dbl] * 6 + [i32, i32, u64, u64, P(dbl), P(u8), P(u8)]),
+            "orc_lsm_american_qmc": (Result, [dbl] * 6 + [i32, i32, u64, i32, u64, C.c_uint32,
+                                                         P(dbl), P(u8), P(u8)]),
             "orc_lsm_reprice_frozen": (dbl, [dbl] * 6 + [i32, i32, u64, u64, P(dbl), P(u8), u8]),
             "orc_bsm": (dbl, [dbl] * 6 + [i32]),
             "orc_black76": (dbl, [dbl] * 5 + [i32]),

@@ -127,3 +127,55 @@ def test_frozen_policy_delta_is_less_noisy_than_refitting():
         dn = mcdx.lsm(100.0 - h, *args[1:], n, seed, FP64)[0]
         naive.append((up - dn) / (2 * h))
     assert np.std(frozen, ddof=1) < np.std(naive, ddof=1)
+
+
+# ---- QR-LSM on Sobol + Brownian-bridge paths (plain and randomized) ----
+def oracle_lsm_qmc(args, shifted=0, shift_seed=0, replicate=0):
+    import ctypes as C
+    s, k, r, q, v, t, typ, m, n = args
+    co = np.zeros((m + 1) * 4)
+    has = np.zeros(m + 1, np.uint8)
+    inc = C.c_uint8()
+    res = L.orc_lsm_american_qmc(s, k, r, q, v, t, typ, m, n, shifted, shift_seed, replicate,
+                                 co.ctypes.data_as(C.POINTER(C.c_double)),
+                                 has.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(inc))
+    return res.price, res.standard_error
+
+
+@pytest.mark.parametrize("typ,s0,vol,m,n", [(1, 100.0, 0.2, 50, 20_000), (0, 110.0, 0.25, 12, 16_384)])
+def test_qmc_lsm_fp64_matches_oracle(typ, s0, vol, m, n):
+    args = (s0, 100.0, 0.06, 0.01, vol, 1.0, typ, m, n)
+    p, se, total, _ = mcdx.lsm_qmc(*args, 0, 0, FP64)
+    op, ose = oracle_lsm_qmc(args)
+    assert total == n
+    assert abs(p - op) <= 0.02 * ose, (p, op, ose)
+    # randomized runs: every run equals the oracle's run with the same shift
+    p, se, total, reps = mcdx.lsm_qmc(*args, 3, 17, FP64)
+    assert total == 3 * n
+    for r in range(3):
+        op, ose = oracle_lsm_qmc(args, 1, 17, r)
+        assert abs(reps[r] - op) <= 0.02 * ose, (r, reps[r], op)
+    assert abs(p - reps.mean()) <= 1e-14 * p and abs(se - reps.std(ddof=1) / np.sqrt(3)) <= 1e-12
+
+
+def test_qmc_lsm_against_crr_and_monte_carlo():
+    # 16 shifted runs of 2^16 Sobol paths: the estimate sits where the Philox LSM does (both
+    # low-biased by the same regression), with a much smaller error than Monte Carlo's.
+    args = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50, 1 << 16)
+    q, qse, _, reps = mcdx.lsm_qmc(*args, 16, 2024, FP32)
+    p, se, _, _, _ = mcdx.lsm(*args, 42, FP32)
+    crr = L.orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 2000, 1)
+    assert abs(q - p) <= 4 * np.hypot(qse, se), (q, qse, p, se)
+    assert abs(q - crr) <= 4 * qse + 0.05, (q, qse, crr)
+    assert qse < 0.5 * se / 4.0, (qse, se)   # Monte Carlo at 16x the paths would give se / 4
+    assert len(reps) == 16 and np.all(reps > 4.0)
+
+
+def test_qmc_lsm_rejects():
+    r = mcdx.Result()
+    f = mcdx.lib().mcdx_lsm_american_qmc
+    args = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50, 1000)
+    assert f(*args, 1, 5, FP32, mcdx.C.byref(r), None) == mcdx.EINVAL       # one replicate
+    assert f(*args, 4097, 5, FP32, mcdx.C.byref(r), None) == mcdx.EINVAL
+    assert f(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 2000, 1000, 0, 5, FP32, mcdx.C.byref(r),
+             None) == mcdx.EINVAL                                           # beyond 1024 dates
