@@ -394,6 +394,10 @@ def c5_lsm(mcdx, mcd):
     # against central differences of CRR-2000 with the same bumps (1% spot, 1% vol).
     gk_ms, gk = med(lambda: mcdx.lsm_greeks(*put, n, SEED, mcdx.PREC_FP32))
 
+    # QR-LSM on Sobol + bridge paths: 16 digitally shifted runs of 2^16 paths (2^20 in all),
+    # whose spread gives the error the in-sample Philox estimate does not state.
+    qmc_ms, ql = med(lambda: mcdx.lsm_qmc(*put, 1 << 16, 16, SEED, mcdx.PREC_FP32), reps=1)
+
     def crr_at(s=100.0, v=0.2):
         return mcd.crr_binomial_american(s, 100.0, 0.06, 0.0, v, 1.0, 2000, mcd.OptionType.Put)
     hs, hv = 1.0, 0.002
@@ -435,6 +439,9 @@ def c5_lsm(mcdx, mcd):
                                         max(1, judged["itm_decisions"]),
                                         "fallbacks": sv["fallbacks"]},
         "serving_speedup_vs_qr_lsm": qr_ms / serve_ms,
+        "qr_lsm_randomized_qmc_16x2p16": {"price": ql[0], "se": ql[1], "paths": ql[2],
+                                          "ms_wall": qmc_ms,
+                                          "diff_vs_qr_lsm_in_se": (ql[0] - qp) / math.hypot(ql[1], qse)},
         "american_greeks_frozen_policy": {
             "ms_wall": gk_ms,
             "price": gk[0][:2], "delta": gk[1][:2], "gamma": gk[2][:2], "vega": gk[3][:2],
