@@ -96,6 +96,64 @@ __host__ __device__ __forceinline__ void givens_fold_rows(QrState& st, double (&
     }
 }
 
+// Fold NR rows into (R, z) by Householder reflections of the stacked (4 + NR) x 5 matrix
+// [R z; rows ys] (linalg.cpp:7-57 applied to the lane's block): per column, the reflection
+// v = x - alpha e1 with alpha = -sign(x1) |x| and tau = 1 / (alpha (alpha - x1)) maps the
+// column below the diagonal to zero and is applied to the columns on its right. About a third
+// of the FP64 operations of NR Givens row folds. A column that is zero from the diagonal down
+// is left as it is (tau = 0), as is every all-zero row (out-of-the-money paths).
+template <int NR>
+__device__ __forceinline__ void householder_fold_rows(QrState& st, const double (&rows)[NR][4],
+                                                      const double (&ys)[NR]) {
+    constexpr int M = kLsmBasis + NR;
+    double A[M][kLsmBasis + 1];
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+        for (int j = 0; j < kLsmBasis; ++j) A[i][j] = j >= i ? st.r[ridx(i, j)] : 0.0;
+        A[i][kLsmBasis] = st.z[i];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+#pragma unroll
+        for (int j = 0; j < kLsmBasis; ++j) A[kLsmBasis + r][j] = rows[r][j];
+        A[kLsmBasis + r][kLsmBasis] = ys[r];
+    }
+#pragma unroll
+    for (int j = 0; j < kLsmBasis; ++j) {
+        // Rows j+1 .. kLsmBasis-1 are R's rows, zero in column j (below R's diagonal), so
+        // the reflection runs over row j and the NR streamed rows only.
+        double norm2 = A[j][j] * A[j][j];
+#pragma unroll
+        for (int i = kLsmBasis; i < M; ++i) norm2 = __builtin_fma(A[i][j], A[i][j], norm2);
+        const double x1 = A[j][j];
+        const double nrm = __builtin_sqrt(norm2);
+        const double alpha = x1 > 0.0 ? -nrm : nrm;
+        const bool live = norm2 > 0.0;
+        const double tau = live ? 1.0 / (alpha * (alpha - x1)) : 0.0;
+        const double v0 = x1 - alpha; // v = (v0, A[kLsmBasis.., j])
+#pragma unroll
+        for (int k = j + 1; k <= kLsmBasis; ++k) {
+            double d = v0 * A[j][k];
+#pragma unroll
+            for (int i = kLsmBasis; i < M; ++i) d = __builtin_fma(A[i][j], A[i][k], d);
+            const double f = tau * d;
+            A[j][k] = __builtin_fma(-f, v0, A[j][k]);
+#pragma unroll
+            for (int i = kLsmBasis; i < M; ++i) A[i][k] = __builtin_fma(-f, A[i][j], A[i][k]);
+        }
+        A[j][j] = live ? alpha : x1;
+#pragma unroll
+        for (int i = kLsmBasis; i < M; ++i) A[i][j] = live ? 0.0 : A[i][j];
+    }
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+        for (int j = i; j < kLsmBasis; ++j) st.r[ridx(i, j)] = A[i][j];
+        st.z[i] = A[i][kLsmBasis];
+    }
+}
+
 // Stream b's rows [R_b | z_b] into a (row i is zero before column i).
 __host__ __device__ __forceinline__ QrState qr_merge(QrState a, const QrState& b) {
     static_assert(kLsmBasis == 4, "the merge streams four rows");
@@ -353,23 +411,18 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
             cf[u] = c;
         }
         if (!regress) continue; // uniform across the grid
-        // The chunk's rows in ascending path order, in groups of four folded in
-        // anti-diagonal order; out-of-the-money and past-the-end paths fold as zero rows,
-        // which leave R and z unchanged.
+        // The chunk's rows (out-of-the-money and past-the-end paths as zero rows) folded into
+        // the lane's factor in one blocked Householder step.
+        double rows[kLsmChunk][4], ys[kLsmChunk];
 #pragma unroll
-        for (uint32_t g = 0; g < kLsmChunk; g += 4) {
-            double rows[4][4], ys[4];
-#pragma unroll
-            for (uint32_t r = 0; r < 4; ++r) {
-                const uint32_t u = g + r;
-                const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
-                const bool in = j0 + u < a.per_lane && p < a.n_paths;
-                const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[r]);
-                ys[r] = itm ? cf[u] : 0.0;
-                st.rows += itm ? 1u : 0u;
-            }
-            givens_fold_rows<4, false>(st, rows, ys);
+        for (uint32_t u = 0; u < kLsmChunk; ++u) {
+            const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
+            const bool in = j0 + u < a.per_lane && p < a.n_paths;
+            const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[u]);
+            ys[u] = itm ? cf[u] : 0.0;
+            st.rows += itm ? 1u : 0u;
         }
+        householder_fold_rows<kLsmChunk>(st, rows, ys);
     }
     if (!regress) return; // uniform across the grid
     const QrState t = block_tree_all(st, lds4);
