@@ -1,8 +1,8 @@
 """GPU Longstaff-Schwartz against the oracle (mcdx C ABI).
 
-The device regression is a Givens tall-skinny QR in FP64: the same least-squares solution as
-the reference's sequential Householder sweep (src/core/linalg.cpp:7-71) with different
-rounding, so coefficients are compared at 1e-7 relative and prices at 0.02 SE (a decision
+The device regression is a tall-skinny QR in FP64 (blocked Householder steps per lane, Givens
+merges of the lanes' factors along a fixed tree): the same least-squares solution as the
+reference's sequential Householder sweep (src/core/linalg.cpp:7-71) with different rounding, so coefficients are compared at 1e-7 relative and prices at 0.02 SE (a decision
 within rounding of the exercise boundary may flip). FP32-generation prices are compared to
 FP64 at 0.5 SE; acceptance against CRR follows the reference's tests/lsm_test.cpp:76-98.
 """
