@@ -198,7 +198,7 @@ struct LsmArgs {
     double* cf;            // [path]
     double* coeffs;        // [(M+1) x 4] device
     uint8_t* has;          // [M+1] device
-    QrState* qr_partials;  // [blocks]
+    QrState* qr_partials;  // [2][blocks]: the block factors of even and odd dates
     Welford* partials;     // [tiles]
     // Optional capture of the regression samples for the network, [date 1..M-1][path]:
     // features (4 (S/K - 1), 2 date/M - 1) and target 10 cf/K (NaN when out of the money).
@@ -375,14 +375,35 @@ __device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t n
 }
 
 // Multi-kernel driver, one date: apply the later date's decision, discount, fold.
+// One date of the backward induction. With fit, the later date's regression is solved here:
+// every block merges the later date's block factors (written by the previous launch, in the
+// other half of the double-buffered partials) along the same fixed tree, so every block holds
+// the same coefficients, and block 0 records them; no separate solve launch per date. Without
+// fit, the policy in coeffs/has is applied.
 template <bool FP64>
-__global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress) {
+__global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress,
+                                                             int fit) {
     __shared__ QrState lds4[4];
     const int later = date + 1;
-    const bool use = later <= a.m - 1 && a.has[later];
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
-    if (use)
-        for (int j = 0; j < 4; ++j) beta[j] = a.coeffs[4 * later + j];
+    bool use = false;
+    if (later <= a.m - 1) { // uniform
+        if (fit) {
+            int has = 0;
+            merge_solve_all(a.qr_partials + (size_t)(later & 1) * a.blocks, a.blocks, lds4, beta,
+                            &has);
+            use = has != 0;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.has[later] = (uint8_t)has;
+                if (has)
+                    for (int j = 0; j < 4; ++j) a.coeffs[4 * later + j] = beta[j];
+            }
+        } else {
+            use = a.has[later] != 0;
+            if (use)
+                for (int j = 0; j < 4; ++j) beta[j] = a.coeffs[4 * later + j];
+        }
+    }
     QrState st = {};
     // Chunks of kLsmChunk paths: every load of a chunk is issued before the first use, so a
     // lane waits for memory once per chunk instead of twice per path. Rows are folded in
@@ -426,19 +447,7 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
     }
     if (!regress) return; // uniform across the grid
     const QrState t = block_tree_all(st, lds4);
-    if (threadIdx.x == 0) a.qr_partials[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(kLsmBlock) void lsm_solve_kernel(LsmArgs a, int date) {
-    __shared__ QrState lds[256];
-    double beta[4];
-    int has;
-    merge_solve_all(a.qr_partials, a.blocks, lds, beta, &has);
-    if (threadIdx.x == 0) {
-        a.has[date] = (uint8_t)has;
-        if (has)
-            for (int j = 0; j < 4; ++j) a.coeffs[4 * date + j] = beta[j];
-    }
+    if (threadIdx.x == 0) a.qr_partials[(size_t)(date & 1) * a.blocks + blockIdx.x] = t;
 }
 
 // Welford over the final cash flows in the canonical tile order.
@@ -941,12 +950,12 @@ struct PathSource {
 template <bool FP64>
 hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
     const unsigned g = r.a.blocks;
-    for (int date = r.a.m - 1; date >= 1; --date) {
+    for (int date = r.a.m - 1; date >= 1; --date)
         hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, date,
-                           fit ? 1 : 0);
-        if (fit) hipLaunchKernelGGL(lsm_solve_kernel, dim3(1), dim3(kLsmBlock), 0, s, r.a, date);
-    }
-    hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, 0, 0);
+                           fit ? 1 : 0, fit ? 1 : 0);
+    // date 0: the decisions at date 1 (fitting date 1 first when fitting), no regression
+    hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, 0, 0,
+                       fit ? 1 : 0);
     hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
