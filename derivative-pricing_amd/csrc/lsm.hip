@@ -12,12 +12,13 @@
 //
 // Layout: the store is SoA [date][path] (dates 1..M; the t=0 column is the constant S0),
 // FP32 in FP32 mode (2^20 paths x 50 dates = 210 MB), FP64 in FP64 mode; the cash flows are
-// one FP64 vector. Each date runs one fused kernel: apply the later date's exercise decision,
-// discount, and fold the ITM rows [basis(S/K) | cf] of each lane into an FP64 Givens QR
-// state (R upper-triangular 4x4 plus Q^T y); lane states merge along the block's pairwise
-// tree, block states along a fixed tree in the solve kernel (tall-skinny QR), which then
-// back-substitutes the coefficients on the device. The least-squares solution is the
-// reference's; only the rounding differs from its sequential Householder sweep.
+// one FP64 vector. Each date runs one fused kernel: solve the later date's regression from
+// its block factors, apply its exercise decision, discount, and fold the ITM rows
+// [basis(S/K) | cf] of each lane into an FP64 QR state (R upper-triangular 4x4 plus Q^T y;
+// blocked Householder steps per lane), merge the lane states along the block's pairwise tree
+// by Givens rotations (tall-skinny QR) and write the block's factor. The least-squares
+// solution is the reference's; only the rounding differs from its sequential Householder
+// sweep.
 #include <math.h>
 
 #include <vector>
@@ -275,10 +276,10 @@ __device__ __forceinline__ void qr_shfl_down(const QrState& s, QrState& o, int o
 // ---- Backward induction -------------------------------------------------------------------
 // Path layout: B = min(256, ceil(n / 4096)) blocks of 256 lanes; lane t of block b owns
 // paths ((j * B + b) * 256 + t) for j < P = ceil(n / (256 B)), coalesced per j. Each date
-// runs one kernel (decision at the later date, discount, fold the ITM rows into the lane's
-// QR state, block tree) and one single-block solve kernel (merge the B partials along a
-// fixed tree, back-substitute on the device). The layout depends on n only, so the
-// coefficients are bitwise reproducible.
+// runs one kernel (merge the later date's B block factors along a fixed tree and
+// back-substitute, decision at the later date, discount, fold the ITM rows into the lane's
+// QR state, block tree). The layout depends on n only, so the coefficients are bitwise
+// reproducible.
 constexpr uint32_t kLsmMaxBlocks = 256;
 constexpr uint32_t kLsmChunk = 8; // paths per lane whose loads are issued together
 
