@@ -297,6 +297,32 @@ def test_rqmc_shift_keeps_stratification_and_gives_an_error_estimate():
     assert se < 0.2 * mc[1], (se, mc[1])
 
 
+def _lsm_qmc(typ, n, m, shifted=0, seed=0, rep=0):
+    co = np.zeros((m + 1) * 4)
+    has = np.zeros(m + 1, np.uint8)
+    inc = C.c_uint8()
+    return L.orc_lsm_american_qmc(100, 100, .05, 0, .2, 1, typ, m, n, shifted, seed, rep,
+                                  co.ctypes.data_as(C.POINTER(C.c_double)),
+                                  has.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(inc))
+
+
+def test_lsm_on_qmc_paths_pinned_by_european_and_monte_carlo():
+    # An American call without dividends is never exercised early (the reference's
+    # tests/lsm_test.cpp:26-37): on Sobol + bridge paths its LSM price is the QMC European
+    # price of the same points (dimension 0 is the terminal draw, sqrt(T) z0 the terminal
+    # bridge value) up to in-sample decisions, and both are Black-Scholes.
+    n = 1 << 14
+    call = _lsm_qmc(0, n, 20).price
+    eu = L.orc_qmc_sobol_european(100, 100, .05, 0, .2, 1, 0, n)
+    bs = L.orc_bsm(100, 100, .05, 0, .2, 1, 0)
+    assert abs(call - eu) <= 2e-3 * eu and abs(call - bs) <= 3e-3 * bs, (call, eu, bs)
+    # The put on QMC paths agrees with the Philox LSM within its Monte Carlo error.
+    put = _lsm_qmc(1, n, 20).price
+    mc = L.orc_lsm_american(100, 100, .05, 0, .2, 1, 1, 20, n, 42, (C.c_double * 84)(),
+                            (C.c_uint8 * 21)(), C.byref(C.c_uint8()))
+    assert abs(put - mc.price) <= 3 * mc.standard_error, (put, mc.price, mc.standard_error)
+
+
 # ----------------------------------------------------------------------------- Heston
 LEWIS_CALLS = {80.0: 26.774758743998854221, 90.0: 20.933349000596710388,
                100.0: 16.070154917028834278, 110.0: 12.132211516709844867,
