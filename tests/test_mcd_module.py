@@ -36,6 +36,34 @@ def test_module_exposes_reference_api():
     assert int(mcd.OptionType.Put) == 1 and int(mcd.BarrierKnock.Out) == 1
 
 
+# Every attribute the reference's Python callers read from the module's results
+# (infra/lambda/request.py:166-222, 253-310 and tools/generate_report.py): with these the
+# callers run unchanged on this module.
+CALLER_FIELDS = {
+    "McResult": ["price", "standard_error", "path_count"],
+    "BinomialResult": ["price", "up_factor", "down_factor", "risk_neutral_probability"],
+    "EuropeanGreeks": ["delta", "gamma", "vega", "rho", "theta"],
+    "LrGreeks": ["delta", "gamma", "vega", "rho", "theta"],
+    "LrGreeksResult": ["value", "standard_error"],
+    "PathwiseGreeks": ["delta", "vega", "rho"],
+    "PathwiseGreeksResult": ["value", "standard_error"],
+    "QmcResult": ["price"],
+    "LsmResult": ["price", "standard_error"],
+    "HestonMcResult": ["price", "standard_error", "path_count"],
+}
+
+
+def test_reference_callers_find_every_result_field():
+    for cls, fields in CALLER_FIELDS.items():
+        missing = [f for f in fields if not hasattr(getattr(mcd, cls), f)]
+        assert not missing, (cls, missing)
+    # PathwiseGreeks has no gamma or theta at all (pathwise.hpp:19-25), as the callers expect
+    assert not hasattr(mcd.PathwiseGreeks, "gamma") and not hasattr(mcd.PathwiseGreeks, "theta")
+    o = mcd.McOptions()  # request.py:215-222 sets the variance-reduction toggles
+    o.antithetic, o.control_variate = True, True
+    assert o.antithetic and o.control_variate and not o.brownian_bridge
+
+
 def test_closed_forms_match_reference():
     C, P = mcd.OptionType.Call, mcd.OptionType.Put
     assert abs(mcd.black_scholes_merton(100, 100, 0.05, 0.0, 0.2, 1.0, C) - float.fromhex(GOLD["bsm_c1"])) < 1e-13
