@@ -536,8 +536,11 @@ struct PolicyState {
     uint32_t dec, dis, fb;
 };
 
+// has: the coefficient flags [date] in LDS (a byte load from global memory would be a
+// vector load and a wait on every visit).
 __device__ __forceinline__ void policy_visit(const LsmArgs& a, const PolicyArgs& q,
-                                             PolicyState& ps, int date, double s) {
+                                             const uint8_t* has, PolicyState& ps, int date,
+                                             double s) {
     const double ex = intrinsic_v(s, a.strike, a.type);
     if (date == a.m) { // lsm.cpp:76-81
         if (!ps.n_done) { ps.n_done = true; ps.n_date = date; ps.n_val = ex; }
@@ -549,7 +552,7 @@ __device__ __forceinline__ void policy_visit(const LsmArgs& a, const PolicyArgs&
     const bool in_range = x >= q.xlo && x <= q.xhi;
     const bool both = !ps.n_done && !ps.q_done;
     bool dq = false;
-    if ((!ps.q_done || both || !in_range) && a.has[date]) { // lsm.cpp:110-122
+    if ((!ps.q_done || both || !in_range) && has[date]) { // lsm.cpp:110-122
         double b0, b1, b2, b3;
         {
             double b[4];
@@ -599,8 +602,11 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
     __shared__ double sh_n[kTilePaths], sh_q[kTilePaths];
     __shared__ Welford lds_n[4], lds_q[4];
     __shared__ uint32_t sh_cnt[3][16];
+    extern __shared__ uint8_t sh_has[]; // [m + 1], launched with that much dynamic LDS
     const uint64_t tile = blockIdx.x;
     const uint32_t t = threadIdx.x;
+    for (int i = (int)t; i <= a.m; i += (int)blockDim.x) sh_has[i] = a.has[i];
+    __syncthreads();
     const uint64_t p = tile * kTilePaths + t;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     // Without the judge the QR policy starts stopped: it then only decides for the network
@@ -613,7 +619,7 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
             for (int step = 0; step < a.m && !(ps.n_done && ps.q_done); ++step) {
                 const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
                 sp = sp * exp(a.drift + a.vsd * z);
-                policy_visit(a, q, ps, step + 1, sp);
+                policy_visit(a, q, sh_has, ps, step + 1, sp);
             }
         } else {
             float x = 0.f;
@@ -626,13 +632,13 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
                     x = x + __builtin_fmaf(a.sig2, z[j], a.mu2);
-                    policy_visit(a, q, ps, step + j + 1, (double)(s0 * fast_exp2(x)));
+                    policy_visit(a, q, sh_has, ps, step + j + 1, (double)(s0 * fast_exp2(x)));
                 }
             }
             for (; step < a.m && !(ps.n_done && ps.q_done); ++step) {
                 const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
                 x = x + __builtin_fmaf(a.sig2, z, a.mu2);
-                policy_visit(a, q, ps, step + 1, (double)(s0 * fast_exp2(x)));
+                policy_visit(a, q, sh_has, ps, step + 1, (double)(s0 * fast_exp2(x)));
             }
         }
     }
@@ -730,14 +736,15 @@ struct Stop {
 };
 
 // One scenario's exercise by the frozen policy at `date` (lsm.cpp:110-122).
-__device__ __forceinline__ void frozen_visit(const LsmArgs& a, Stop& st, int date, double s) {
+__device__ __forceinline__ void frozen_visit(const LsmArgs& a, const uint8_t* has, Stop& st,
+                                             int date, double s) {
     if (st.done) return;
     const double ex = intrinsic_v(s, a.strike, a.type);
     if (date == a.m) { // lsm.cpp:76-81
         st = Stop{true, date, ex};
         return;
     }
-    if (ex > 0.0 && a.has[date]) {
+    if (ex > 0.0 && has[date]) {
         double b[4];
         laguerre4(s * a.inv_strike, b);
         const double* beta = a.coeffs + 4 * date;
@@ -762,8 +769,11 @@ template <bool FP64>
 __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a, AmGreekArgs g) {
     __shared__ double sh[4][kTilePaths];
     __shared__ Welford lds[4][4];
+    extern __shared__ uint8_t sh_has[]; // [m + 1], launched with that much dynamic LDS
     const uint64_t tile = blockIdx.x;
     const uint32_t t = threadIdx.x;
+    for (int i = (int)t; i <= a.m; i += (int)blockDim.x) sh_has[i] = a.has[i];
+    __syncthreads();
     const uint64_t p = tile * kTilePaths + t;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     Stop c0 = {false, 0, 0.0}, c1 = c0, c2 = c0, c3 = c0, c4 = c0;
@@ -781,11 +791,11 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
                 s3 = s3 * exp(g.drift_u + g.vsd_u * z);
                 s4 = s4 * exp(g.drift_d + g.vsd_d * z);
                 const int d = step + 1;
-                frozen_visit(a, c0, d, s0);
-                frozen_visit(a, c1, d, s1);
-                frozen_visit(a, c2, d, s2);
-                frozen_visit(a, c3, d, s3);
-                frozen_visit(a, c4, d, s4);
+                frozen_visit(a, sh_has, c0, d, s0);
+                frozen_visit(a, sh_has, c1, d, s1);
+                frozen_visit(a, sh_has, c2, d, s2);
+                frozen_visit(a, sh_has, c3, d, s3);
+                frozen_visit(a, sh_has, c4, d, s4);
             }
         } else {
             float x = 0.f, xu = 0.f, xd = 0.f;
@@ -797,11 +807,11 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
                 xd = xd + __builtin_fmaf(g.sig2_d, z, g.mu2_d);
                 const float e = fast_exp2(x);
                 const int d = step + 1;
-                frozen_visit(a, c0, d, (double)(s0 * e));
-                frozen_visit(a, c1, d, (double)(g.s0_up * e));
-                frozen_visit(a, c2, d, (double)(g.s0_dn * e));
-                frozen_visit(a, c3, d, (double)(s0 * fast_exp2(xu)));
-                frozen_visit(a, c4, d, (double)(s0 * fast_exp2(xd)));
+                frozen_visit(a, sh_has, c0, d, (double)(s0 * e));
+                frozen_visit(a, sh_has, c1, d, (double)(g.s0_up * e));
+                frozen_visit(a, sh_has, c2, d, (double)(g.s0_dn * e));
+                frozen_visit(a, sh_has, c3, d, (double)(s0 * fast_exp2(xu)));
+                frozen_visit(a, sh_has, c4, d, (double)(s0 * fast_exp2(xd)));
             }
         }
     }
@@ -1115,6 +1125,10 @@ void table_range(double spot, double strike, double rate, double carry, double v
     }
 }
 
+// The replay kernels keep the coefficient flags [0..m] in LDS.
+constexpr int kMaxLdsDates = 32768;
+size_t has_lds_bytes(int m) { return ((size_t)m + 1 + 15) & ~(size_t)15; }
+
 struct PolicyBufs {
     float *table, *gx, *gt;
     Welford *pn, *pq;
@@ -1145,6 +1159,7 @@ PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
 int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_t grid,
                    double lo, double hi, bool fp64, bool judge, hipStream_t s) {
     const int m = r.a.m;
+    if (m + 1 > kMaxLdsDates) return MCDX_EINVAL;
     const uint64_t pts = (uint64_t)(m - 1) * grid;
     const double h = (hi - lo) / (double)(grid - 1);
     hipLaunchKernelGGL(lsm_grid_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, b.gx,
@@ -1154,9 +1169,11 @@ int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_
     PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, 1.0 / r.a.strike, r.a.strike / kTargetScale,
                  b.pn, b.pq, b.tile_counts, judge ? 1 : 0};
     if (fp64)
-        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTilePaths), 0, s, r.a, q);
+        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTilePaths),
+                           has_lds_bytes(m), s, r.a, q);
     else
-        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTilePaths), 0, s, r.a, q);
+        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTilePaths),
+                           has_lds_bytes(m), s, r.a, q);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(lsm_count_kernel, dim3(1), dim3(256), 0, s, b.tile_counts, r.tiles, b.ctr);
@@ -1226,7 +1243,7 @@ int lsm_am_greeks(double spot, double strike, double rate, double carry, double 
     if (!out4 || !(spot > 0.0) || !(vol > 0.0)) return MCDX_EINVAL;
     if (!(hs > 0.0)) hs = fmax(spot * 0.01, 1e-4); // default_bump_sizes, finite_difference.cpp:31-38
     if (!(hv > 0.0)) hv = fmax(vol * 0.01, 1e-4);
-    if (!(hs < spot) || !(hv < vol)) return MCDX_EINVAL;
+    if (!(hs < spot) || !(hv < vol) || m + 1 > kMaxLdsDates) return MCDX_EINVAL;
     ThreadCtx* ctx;
     int st = thread_ctx(&ctx);
     if (st) return st;
@@ -1264,9 +1281,11 @@ int lsm_am_greeks(double spot, double strike, double rate, double carry, double 
     g.parts = (Welford*)ex;
     g.tiles = tiles;
     if (fp64)
-        hipLaunchKernelGGL(lsm_am_greeks_kernel<true>, dim3((unsigned)tiles), dim3(kTilePaths), 0, s, r.a, g);
+        hipLaunchKernelGGL(lsm_am_greeks_kernel<true>, dim3((unsigned)tiles), dim3(kTilePaths),
+                           has_lds_bytes(m), s, r.a, g);
     else
-        hipLaunchKernelGGL(lsm_am_greeks_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths), 0, s, r.a, g);
+        hipLaunchKernelGGL(lsm_am_greeks_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths),
+                           has_lds_bytes(m), s, r.a, g);
     e = hipGetLastError();
     for (int q = 0; q < 4 && e == hipSuccess; ++q) {
         Welford* part = g.parts + (uint64_t)q * (tiles + 1);
