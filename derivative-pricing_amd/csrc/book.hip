@@ -76,13 +76,14 @@ struct mcdx_book {
 namespace mcdx {
 
 template <bool FP64>
-__device__ __forceinline__ double book_sample(const GbmArgs& a, int kind, bool anti, uint64_t path) {
+__device__ __forceinline__ double book_sample(const GbmArgs& a, int kind, bool anti, uint64_t path,
+                                              uint32_t phh) {
     switch (kind) {
-    case kTerminal: return anti ? path_sample<kTerminal, FP64, true>(a, path) : path_sample<kTerminal, FP64, false>(a, path);
-    case kAsian: return anti ? path_sample<kAsian, FP64, true>(a, path) : path_sample<kAsian, FP64, false>(a, path);
-    case kBarrier: return anti ? path_sample<kBarrier, FP64, true>(a, path) : path_sample<kBarrier, FP64, false>(a, path);
-    case kBarrierBB: return anti ? path_sample<kBarrierBB, FP64, true>(a, path) : path_sample<kBarrierBB, FP64, false>(a, path);
-    default: return anti ? path_sample<kLookback, FP64, true>(a, path) : path_sample<kLookback, FP64, false>(a, path);
+    case kTerminal: return anti ? path_sample<kTerminal, FP64, true>(a, path, phh) : path_sample<kTerminal, FP64, false>(a, path, phh);
+    case kAsian: return anti ? path_sample<kAsian, FP64, true>(a, path, phh) : path_sample<kAsian, FP64, false>(a, path, phh);
+    case kBarrier: return anti ? path_sample<kBarrier, FP64, true>(a, path, phh) : path_sample<kBarrier, FP64, false>(a, path, phh);
+    case kBarrierBB: return anti ? path_sample<kBarrierBB, FP64, true>(a, path, phh) : path_sample<kBarrierBB, FP64, false>(a, path, phh);
+    default: return anti ? path_sample<kLookback, FP64, true>(a, path, phh) : path_sample<kLookback, FP64, false>(a, path, phh);
     }
 }
 
@@ -116,8 +117,10 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
     for (int j = 0; j < 3; ++j) gw[j] = Welford{0, 0.0, 0.0};
 #pragma unroll 1
     for (int k = 0; k < kPathsPerLane; ++k) {
-        const uint64_t path = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
-        const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+        // base is a multiple of 256 and threadIdx.x < 256: every lane's path >> 32 is base >> 32.
+        const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
+        const uint64_t path = base + threadIdx.x;
+        const uint32_t plo = (uint32_t)path, phh = (uint32_t)(base >> 32);
         double v[kFuseMax];
         if constexpr (KIND == kTerminal) {
             const float z = inv_normal_f32(philox_w0(plo, phh, 0u, 0u, k0, k1));
@@ -257,9 +260,10 @@ __global__ __launch_bounds__(kTileLanes) void book_kernel(const GbmArgs* __restr
         Welford w = {0, 0.0, 0.0};
 #pragma unroll 1
         for (int k = 0; k < kPathsPerLane; ++k) {
-            const uint64_t path = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
+            const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
+            const uint64_t path = base + threadIdx.x;
             if (path < a.n_paths) {
-                const double x = book_sample<FP64>(a, kind, anti, path);
+                const double x = book_sample<FP64>(a, kind, anti, path, (uint32_t)(base >> 32));
                 w.n += 1;
                 const double d = x - w.mean;
                 w.mean += d * (1.0 / (double)(k + 1));

@@ -49,7 +49,10 @@ __host__ __device__ __forceinline__ uint32_t philox_w0(uint32_t c0, uint32_t c1,
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        // Round 1's left word comes from c1, c2 and the key only: plain XORs, so that when
+        // those are wave-uniform (a path high word shared by the wave, the draw index, the
+        // seed) it and the next round's M0 product stay on the scalar unit.
+        const uint32_t n0 = r == 0 ? (uint32_t)(p1 >> 32) ^ c1 ^ k0 : xor3((uint32_t)(p1 >> 32), c1, k0);
         const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c0 = n0;
         c1 = (uint32_t)p1;

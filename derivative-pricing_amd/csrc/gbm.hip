@@ -13,9 +13,11 @@ __global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
     Welford w = {0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kPathsPerLane; ++k) {
-        const uint64_t path = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
+        // base is a multiple of 256 and threadIdx.x < 256: every lane's path >> 32 is base >> 32.
+        const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
+        const uint64_t path = base + threadIdx.x;
         if (path < a.n_paths) {
-            const double x = path_sample<KIND, FP64, ANTI>(a, path);
+            const double x = path_sample<KIND, FP64, ANTI>(a, path, (uint32_t)(base >> 32));
             w.n += 1;
             const double d = x - w.mean;
             w.mean += d * (1.0 / (double)(k + 1));
@@ -29,7 +31,7 @@ __global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
 template <int KIND, bool FP64, bool ANTI>
 __global__ __launch_bounds__(256) void gbm_samples_kernel(GbmArgs a, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a.samples[i] = path_sample<KIND, FP64, ANTI>(a, a.path0 + i);
+    if (i < n) a.samples[i] = path_sample<KIND, FP64, ANTI>(a, a.path0 + i, (uint32_t)((a.path0 + i) >> 32));
 }
 
 namespace {

@@ -149,10 +149,13 @@ __device__ __forceinline__ double settle_f32(const GbmArgs& a, const StateF32& s
     }
 }
 
+// `phh` = path >> 32, passed separately so that a caller whose lanes share it (a block of 256
+// aligned paths) can hand it over in a scalar register: the first Philox round's left word and
+// the second round's M0 product then stay on the scalar unit.
 template <int KIND, bool ANTI>
-__device__ double path_f32(const GbmArgs& a, uint64_t path) {
+__device__ double path_f32(const GbmArgs& a, uint64_t path, uint32_t phh) {
 #pragma clang fp contract(fast)
-    const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+    const uint32_t plo = (uint32_t)path;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     if constexpr (KIND == kTerminal) {
         const float z = inv_normal_f32(philox_w0(plo, phh, 0u, 0u, k0, k1));
@@ -317,8 +320,8 @@ __device__ __forceinline__ void bb_step_f64(const GbmArgs& a, StateF64& p, doubl
 }
 
 template <int KIND, bool ANTI>
-__device__ double path_f64(const GbmArgs& a, uint64_t path) {
-    const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+__device__ double path_f64(const GbmArgs& a, uint64_t path, uint32_t phh) {
+    const uint32_t plo = (uint32_t)path;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     if constexpr (KIND == kTerminal) {
         const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, 0u, 0u, k0, k1)));
@@ -357,10 +360,11 @@ __device__ double path_f64(const GbmArgs& a, uint64_t path) {
     }
 }
 
+// One path's sample; phh must equal path >> 32.
 template <int KIND, bool FP64, bool ANTI>
-__device__ __forceinline__ double path_sample(const GbmArgs& a, uint64_t path) {
-    if constexpr (FP64) return path_f64<KIND, ANTI>(a, path);
-    else return path_f32<KIND, ANTI>(a, path);
+__device__ __forceinline__ double path_sample(const GbmArgs& a, uint64_t path, uint32_t phh) {
+    if constexpr (FP64) return path_f64<KIND, ANTI>(a, path, phh);
+    else return path_f32<KIND, ANTI>(a, path, phh);
 }
 
 

@@ -39,8 +39,9 @@ struct HestonArgs {
 __device__ __forceinline__ double smax_h(double a, double b) { return (a < b) ? b : a; }
 
 // Terminal spot of one path, FP32 generation with FP64 moment matching.
-__device__ double heston_path_f32(const HestonArgs& a, uint64_t path) {
-    const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+// phh = path >> 32 (wave-uniform in the tile kernel, where it then stays in scalar registers).
+__device__ double heston_path_f32(const HestonArgs& a, uint64_t path, uint32_t phh) {
+    const uint32_t plo = (uint32_t)path;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     float v = (float)a.v0;
     float x = 0.f; // log(S/S0)
@@ -80,8 +81,8 @@ __device__ double heston_path_f32(const HestonArgs& a, uint64_t path) {
 }
 
 // Terminal spot of one path, the reference's FP64 arithmetic (heston.cpp:136-148).
-__device__ double heston_path_f64(const HestonArgs& a, uint64_t path) {
-    const uint32_t plo = (uint32_t)path, phh = (uint32_t)(path >> 32);
+__device__ double heston_path_f64(const HestonArgs& a, uint64_t path, uint32_t phh) {
+    const uint32_t plo = (uint32_t)path;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     double v = a.v0;
     double log_s = a.log_s0;
@@ -115,9 +116,10 @@ __device__ double heston_path_f64(const HestonArgs& a, uint64_t path) {
 }
 
 template <bool FP64>
-__device__ __forceinline__ double heston_terminal(const HestonArgs& a, uint64_t path) {
-    if constexpr (FP64) return heston_path_f64(a, path);
-    else return heston_path_f32(a, path);
+__device__ __forceinline__ double heston_terminal(const HestonArgs& a, uint64_t path,
+                                                 uint32_t phh) {
+    if constexpr (FP64) return heston_path_f64(a, path, phh);
+    else return heston_path_f32(a, path, phh);
 }
 
 // One tile of 1024 paths per block. Each lane keeps its 4 terminal spots, then every strike
@@ -130,9 +132,11 @@ __global__ __launch_bounds__(kTileLanes) void heston_tile_kernel(HestonArgs a) {
     bool valid[kPathsPerLane];
 #pragma unroll
     for (int k = 0; k < kPathsPerLane; ++k) {
-        const uint64_t path = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
+        // base is a multiple of 256 and threadIdx.x < 256: every lane's path >> 32 is base >> 32.
+        const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
+        const uint64_t path = base + threadIdx.x;
         valid[k] = path < a.n_paths;
-        st[k] = valid[k] ? heston_terminal<FP64>(a, path) : 0.0;
+        st[k] = valid[k] ? heston_terminal<FP64>(a, path, (uint32_t)(base >> 32)) : 0.0;
     }
     const double phi = a.type == 0 ? 1.0 : -1.0;
     for (int s = 0; s < a.n_strikes; ++s) {
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(256) void heston_samples_kernel(HestonArgs a, uint6
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double phi = a.type == 0 ? 1.0 : -1.0;
-    const double st = heston_terminal<FP64>(a, a.path0 + i);
+    const double st = heston_terminal<FP64>(a, a.path0 + i, (uint32_t)((a.path0 + i) >> 32));
     a.samples[i] = a.discount * smax_h(phi * (st - a.strikes[0]), 0.0);
 }
 

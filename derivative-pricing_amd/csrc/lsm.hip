@@ -222,9 +222,11 @@ __device__ __forceinline__ double store_get(const LsmArgs& a, int date, uint64_t
 // [date][path] writes. FP32 mode steps in log2 space like the path kernels.
 template <bool FP64>
 __global__ __launch_bounds__(256) void lsm_paths_kernel(LsmArgs a) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Launched with 256 lanes per block: the block's paths share p >> 32, a scalar here.
+    const uint64_t base = (uint64_t)blockIdx.x * 256u;
+    const uint64_t p = base + threadIdx.x;
     if (p >= a.n_paths) return;
-    const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
+    const uint32_t plo = (uint32_t)p, phh = (uint32_t)(base >> 32);
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     if constexpr (FP64) {
         double s = a.spot;
@@ -613,7 +615,8 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
     // outside the table range, and its own price is not formed.
     PolicyState ps = {false, q.judge == 0, 0, 0, 0.0, 0.0, 0u, 0u, 0u};
     if (p < a.n_paths) {
-        const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
+        // t < 1024 = kTilePaths: the block's paths share p >> 32, a scalar here.
+        const uint32_t plo = (uint32_t)p, phh = (uint32_t)((tile * kTilePaths) >> 32);
         if constexpr (FP64) {
             double sp = a.spot;
             for (int step = 0; step < a.m && !(ps.n_done && ps.q_done); ++step) {
@@ -779,7 +782,8 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
     Stop c0 = {false, 0, 0.0}, c1 = c0, c2 = c0, c3 = c0, c4 = c0;
     auto all_done = [&] { return c0.done && c1.done && c2.done && c3.done && c4.done; };
     if (p < a.n_paths) {
-        const uint32_t plo = (uint32_t)p, phh = (uint32_t)(p >> 32);
+        // t < 1024 = kTilePaths: the block's paths share p >> 32, a scalar here.
+        const uint32_t plo = (uint32_t)p, phh = (uint32_t)((tile * kTilePaths) >> 32);
         if constexpr (FP64) {
             double s0 = a.spot, s1 = g.spot_up, s2 = g.spot_dn, s3 = a.spot, s4 = a.spot;
             for (int step = 0; step < a.m && !all_done(); ++step) {
