@@ -15,10 +15,10 @@
 // one FP64 vector. Each date runs one fused kernel: solve the later date's regression from
 // its block factors, apply its exercise decision, discount, and fold the ITM rows
 // [basis(S/K) | cf] of each lane into an FP64 QR state (R upper-triangular 4x4 plus Q^T y;
-// blocked Householder steps per lane), merge the lane states along the block's pairwise tree
-// by Givens rotations (tall-skinny QR) and write the block's factor. The least-squares
-// solution is the reference's; only the rounding differs from its sequential Householder
-// sweep.
+// blocked Householder steps per lane), reduce the block's stacked lane factors to one factor
+// by Householder reflections with block-wide sums (tall-skinny QR) and write it. The
+// least-squares solution is the reference's; only the rounding differs from its sequential
+// Householder sweep.
 #include <math.h>
 
 #include <vector>
@@ -43,65 +43,11 @@ __host__ __device__ __forceinline__ int ridx(int i, int j) { // i <= j
     return i * kLsmBasis - (i * (i - 1)) / 2 + (j - i);
 }
 
-// One Givens rotation at column J between R's row J and a row being streamed in (a[0..3],
-// y; its entries before J are already zero). c = rjj / h, s = aj / h with
-// h = sqrt(rjj^2 + aj^2): one reciprocal square root (hardware seed + Newton) instead of a
-// square root and two divisions; the updates are fused multiply-adds (four FP64 operations
-// per column instead of six: the regression kernels issue FP64 work back to back, so their
-// operation count is their time). Branch-free: a zero a[J] selects c = 1, s = 0, which
-// reproduces R's row and the remaining entries exactly (the skip of the textbook loop), also
-// when R(J,J) = 0 makes the reciprocal square root infinite.
-template <int J>
-__host__ __device__ __forceinline__ void givens_rot(QrState& st, double* a, double& y) {
-    const double aj = a[J];
-    const double rjj = st.r[ridx(J, J)];
-    const double inv = fast_rsqrt_f64(__builtin_fma(rjj, rjj, aj * aj));
-    const bool skip = aj == 0.0;
-    const double c = skip ? 1.0 : rjj * inv, s = skip ? 0.0 : aj * inv;
-#pragma unroll
-    for (int k = J; k < kLsmBasis; ++k) {
-        const double rk = st.r[ridx(J, k)];
-        st.r[ridx(J, k)] = __builtin_fma(c, rk, s * a[k]);
-        a[k] = __builtin_fma(c, a[k], -s * rk);
-    }
-    const double zj = st.z[J];
-    st.z[J] = __builtin_fma(c, zj, s * y);
-    y = __builtin_fma(c, y, -s * zj);
-}
-
-template <int J>
-__host__ __device__ __forceinline__ void givens_rot_if(int j, QrState& st, double* a, double& y) {
-    if (j == J) givens_rot<J>(st, a, y);
-}
-
-// Fold NR rows into (R, z) in anti-diagonal order: step t applies the rotation (row r,
-// column t - r) of every row r. Rotation (r, j) needs (r, j - 1) and (r - 1, j), both done
-// at step t - 1, so the rotations of one step are independent of each other; each rotation
-// still sees exactly the operands of the row-by-row order, so R and z are bit-identical to
-// it. With TRI, row r starts at column r (the rows of another R factor, zero before their
-// diagonal).
-template <int NR, bool TRI>
-__host__ __device__ __forceinline__ void givens_fold_rows(QrState& st, double (&rows)[NR][4],
-                                                          double (&ys)[NR]) {
-#pragma unroll
-    for (int t = 0; t < NR + kLsmBasis - 1; ++t) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const int j = t - r;
-            if (j < (TRI ? r : 0) || j >= kLsmBasis) continue;
-            givens_rot_if<0>(j, st, rows[r], ys[r]);
-            givens_rot_if<1>(j, st, rows[r], ys[r]);
-            givens_rot_if<2>(j, st, rows[r], ys[r]);
-            givens_rot_if<3>(j, st, rows[r], ys[r]);
-        }
-    }
-}
-
 // Fold NR rows into (R, z) by Householder reflections of the stacked (4 + NR) x 5 matrix
 // [R z; rows ys] (linalg.cpp:7-57 applied to the lane's block): per column, the reflection
 // v = x - alpha e1 with alpha = -sign(x1) |x| and tau = 1 / (alpha (alpha - x1)) maps the
 // column below the diagonal to zero and is applied to the columns on its right. About a third
-// of the FP64 operations of NR Givens row folds. A column that is zero from the diagonal down
+// of the FP64 operations of NR Givens row rotations. A column that is zero from the diagonal down
 // is left as it is (tau = 0), as is every all-zero row (out-of-the-money paths).
 template <int NR>
 __device__ __forceinline__ void householder_fold_rows(QrState& st, const double (&rows)[NR][4],
@@ -153,21 +99,6 @@ __device__ __forceinline__ void householder_fold_rows(QrState& st, const double 
         for (int j = i; j < kLsmBasis; ++j) st.r[ridx(i, j)] = A[i][j];
         st.z[i] = A[i][kLsmBasis];
     }
-}
-
-// Stream b's rows [R_b | z_b] into a (row i is zero before column i).
-__host__ __device__ __forceinline__ QrState qr_merge(QrState a, const QrState& b) {
-    static_assert(kLsmBasis == 4, "the merge streams four rows");
-    double rows[4][4], ys[4];
-#pragma unroll
-    for (int i = 0; i < kLsmBasis; ++i) {
-#pragma unroll
-        for (int j = 0; j < kLsmBasis; ++j) rows[i][j] = j >= i ? b.r[ridx(i, j)] : 0.0;
-        ys[i] = b.z[i];
-    }
-    givens_fold_rows<4, true>(a, rows, ys);
-    a.rows += b.rows;
-    return a;
 }
 
 __host__ __device__ __forceinline__ void laguerre4(double x, double* out) { // lsm.cpp:21-25
@@ -267,22 +198,15 @@ __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
     a.cf[p] = intrinsic_v(store_get<FP64>(a, a.m, p), a.strike, a.type);
 }
 
-__device__ __forceinline__ void qr_shfl_down(const QrState& s, QrState& o, int off) {
-#pragma unroll
-    for (int i = 0; i < 10; ++i) o.r[i] = __shfl_down(s.r[i], off, 64);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) o.z[i] = __shfl_down(s.z[i], off, 64);
-    o.rows = __shfl_down(s.rows, off, 64);
-}
-
 // ---- Backward induction -------------------------------------------------------------------
 // Path layout: B = min(256, ceil(n / 4096)) blocks of 256 lanes; lane t of block b owns
 // paths ((j * B + b) * 256 + t) for j < P = ceil(n / (256 B)), coalesced per j. Each date
-// runs one kernel (merge the later date's B block factors along a fixed tree and
-// back-substitute, decision at the later date, discount, fold the ITM rows into the lane's
-// QR state, block tree). The layout depends on n only, so the coefficients are bitwise
-// reproducible.
+// runs one kernel (reduce the later date's B block factors to one and back-substitute,
+// decision at the later date, discount, fold the ITM rows into the lane's QR state, reduce
+// the block's lane states to the block factor). The layout depends on n only and every sum
+// runs in a fixed order, so the coefficients are bitwise reproducible.
 constexpr uint32_t kLsmMaxBlocks = 256;
+static_assert(kLsmMaxBlocks <= (uint32_t)kLsmBlock, "merge_solve_all loads one block factor per lane");
 constexpr uint32_t kLsmChunk = 8; // paths per lane whose loads are issued together
 
 __host__ __device__ __forceinline__ uint64_t lane_path(uint32_t j, uint32_t block,
@@ -324,76 +248,156 @@ __device__ __forceinline__ bool fold_row(const LsmArgs& a, int date, uint64_t p,
     return itm;
 }
 
-__device__ __forceinline__ QrState bcast_qr(const QrState& s) {
+// Block-wide sums of NV doubles, bit-identical in every lane: a butterfly inside each wave
+// (lanes l and l ^ off add the same two operands), then the four wave totals as
+// (w0 + w1) + (w2 + w3). The block's lane 0 also publishes NP values through the same
+// exchange. lds holds at least 4 NV + NP doubles.
+template <int NV, int NP>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double (&pub)[NP], double* lds) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] += __shfl_xor(v[i], off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads(); // readers of the previous exchange are done
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) lds[w * NV + i] = v[i];
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) lds[4 * NV + i] = pub[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = (lds[i] + lds[NV + i]) + (lds[2 * NV + i] + lds[3 * NV + i]);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) pub[i] = lds[4 * NV + i];
+}
+constexpr int kLsmLdsDoubles = 4 * 6 + 5 + 8; // block_sums at column 0, plus the solve's results
+
+// One column step of the block's stacked Householder reduction (block_factor).
+template <int J>
+__device__ __forceinline__ void stacked_reflect(double (&A)[kLsmBasis][kLsmBasis + 1],
+                                                bool lead, double& rows, double* lds) {
+    constexpr int NK = kLsmBasis - J;          // columns J+1 .. kLsmBasis (the last is z)
+    constexpr int NV = 1 + NK + (J == 0 ? 1 : 0); // sum of squares, dot products, row count
+    double v[NV], pub[1 + NK];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = 0.0;
+    // Rows taking part: every row of every lane except lead's rows 0 .. J (its finished rows
+    // and the pivot row J).
+#pragma unroll
+    for (int r = 0; r < kLsmBasis; ++r) {
+        const double x = (!lead || r > J) ? A[r][J] : 0.0;
+        v[0] = __builtin_fma(x, x, v[0]);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) v[1 + k] = __builtin_fma(x, A[r][J + 1 + k], v[1 + k]);
+    }
+    if constexpr (J == 0) v[NV - 1] = rows;
+#pragma unroll
+    for (int k = 0; k <= NK; ++k) pub[k] = lead ? A[J][J + k] : 0.0;
+    block_sums(v, pub, lds);
+    if constexpr (J == 0) rows = v[NV - 1];
+    // The reflection v = x - alpha e_J of the stacked column (alpha = -sign(x1) |x|,
+    // tau = 1 / (alpha (alpha - x1))), applied to the columns on its right; a column that is
+    // zero from the pivot down is left as it is.
+    const double x1 = pub[0];
+    const double norm2 = __builtin_fma(x1, x1, v[0]);
+    const bool live = norm2 > 0.0;
+    const double nrm = __builtin_sqrt(norm2);
+    const double alpha = x1 > 0.0 ? -nrm : nrm;
+    const double tau = live ? 1.0 / (alpha * (alpha - x1)) : 0.0;
+    const double v0 = x1 - alpha;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const double f = tau * __builtin_fma(v0, pub[1 + k], v[1 + k]);
+#pragma unroll
+        for (int r = 0; r < kLsmBasis; ++r)
+            if (!lead || r > J) A[r][J + 1 + k] = __builtin_fma(-f, A[r][J], A[r][J + 1 + k]);
+        if (lead) A[J][J + 1 + k] = __builtin_fma(-f, v0, A[J][J + 1 + k]);
+    }
+#pragma unroll
+    for (int r = 0; r < kLsmBasis; ++r)
+        if (!lead || r > J) A[r][J] = live ? 0.0 : A[r][J];
+    if (lead) A[J][J] = live ? alpha : x1;
+}
+
+// The block's combined factor: the lanes' [R | z] rows stacked in lane order (1024 x 5)
+// reduced to one upper-triangular [R | z] by Householder reflections, as linalg.cpp:7-57
+// reduces the full design matrix. Column j pivots on lane 0's row j; each column costs one
+// block-wide sum of the dot products it needs, in a fixed order, so the result is bitwise
+// reproducible. The factor is returned in lane 0 (other lanes: partial garbage).
+__device__ __forceinline__ QrState block_factor(const QrState& in, double* lds) {
+    double A[kLsmBasis][kLsmBasis + 1];
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+        for (int j = 0; j < kLsmBasis; ++j) A[i][j] = j >= i ? in.r[ridx(i, j)] : 0.0;
+        A[i][kLsmBasis] = in.z[i];
+    }
+    const bool lead = threadIdx.x == 0;
+    double rows = (double)in.rows; // exact: at most 2^32
+    stacked_reflect<0>(A, lead, rows, lds);
+    stacked_reflect<1>(A, lead, rows, lds);
+    stacked_reflect<2>(A, lead, rows, lds);
+    stacked_reflect<3>(A, lead, rows, lds);
     QrState o;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) o.r[i] = bcast_f64(s.r[i]);
+    for (int i = 0; i < kLsmBasis; ++i) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o.z[i] = bcast_f64(s.z[i]);
-    o.rows = __builtin_amdgcn_readfirstlane(s.rows);
+        for (int j = i; j < kLsmBasis; ++j) o.r[ridx(i, j)] = A[i][j];
+        o.z[i] = A[i][kLsmBasis];
+    }
+    o.rows = (uint32_t)rows;
     return o;
 }
 
-// Block total of the lanes' QR states along a fixed tree, returned to every lane (no
-// thread-0-only region: the drivers run it inside loops with barriers).
-__device__ __forceinline__ QrState block_tree_all(QrState st, QrState* lds4) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll 1
-    for (int off = 1; off < 64; off <<= 1) {
-        QrState o;
-        qr_shfl_down(st, o, off);
-        if ((lane & (2 * off - 1)) == 0) st = qr_merge(st, o);
-    }
-    const QrState w = bcast_qr(st);
-    __syncthreads(); // lds4 free from any previous use
-    lds4[threadIdx.x >> 6] = w;
-    __syncthreads();
-    return qr_merge(qr_merge(lds4[0], lds4[1]), qr_merge(lds4[2], lds4[3]));
-}
-
-// Merge B block partials (256 groups by static partition, then a pairwise tree) and
-// back-substitute (linalg.cpp:58-69) when at least four ITM rows were folded
-// (lsm.cpp:102-104). Every lane of the 256-lane block receives beta and has.
-__device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t nb,
-                                                QrState* lds, double* beta, int* has) {
-    const uint32_t g = threadIdx.x;
-    const uint32_t q = nb / 256, rem = nb % 256;
-    const uint32_t b = g * q + (g < rem ? g : rem);
-    const uint32_t e = b + q + (g < rem ? 1 : 0);
+// Combine nb <= 256 block factors (lane b loads factor b) and back-substitute
+// (linalg.cpp:58-69) when at least four ITM rows were folded (lsm.cpp:102-104). Every lane of
+// the 256-lane block receives beta and has.
+__device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t nb, double* lds,
+                                                double* beta, int* has) {
     QrState acc = {};
-    if (b < e) acc = parts[b];
-    for (uint32_t i = b + 1; i < e; ++i) acc = qr_merge(acc, parts[i]);
-    // The pairwise tree over the 256 groups: shuffles inside each wave, then the four wave
-    // totals (the same merge order as a tree over an LDS array, without its barriers).
-    const QrState t = block_tree_all(acc, lds);
-    *has = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
+    if (threadIdx.x < nb) acc = parts[threadIdx.x];
+    const QrState t = block_factor(acc, lds);
+    double* res = lds + kLsmLdsDoubles - 8; // not touched by block_sums
+    if (threadIdx.x == 0) {
+        const int h = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
+        double b[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = kLsmBasis - 1; i >= 0; --i) {
+            double sum = t.z[i];
+            for (int j = i + 1; j < kLsmBasis; ++j) sum -= t.r[ridx(i, j)] * b[j];
+            const double diag = t.r[ridx(i, i)];
+            b[i] = diag != 0.0 ? sum / diag : 0.0;
+        }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) beta[i] = 0.0;
-    for (int i = kLsmBasis - 1; i >= 0; --i) {
-        double sum = t.z[i];
-        for (int j = i + 1; j < kLsmBasis; ++j) sum -= t.r[ridx(i, j)] * beta[j];
-        const double diag = t.r[ridx(i, i)];
-        beta[i] = diag != 0.0 ? sum / diag : 0.0;
+        for (int i = 0; i < 4; ++i) res[i] = b[i];
+        res[4] = (double)h;
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) beta[i] = res[i];
+    *has = res[4] != 0.0 ? 1 : 0;
 }
 
-// Multi-kernel driver, one date: apply the later date's decision, discount, fold.
 // One date of the backward induction. With fit, the later date's regression is solved here:
-// every block merges the later date's block factors (written by the previous launch, in the
-// other half of the double-buffered partials) along the same fixed tree, so every block holds
+// every block reduces the later date's block factors (written by the previous launch, in the
+// other half of the double-buffered partials) in the same fixed order, so every block holds
 // the same coefficients, and block 0 records them; no separate solve launch per date. Without
 // fit, the policy in coeffs/has is applied.
 template <bool FP64>
 __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress,
                                                              int fit) {
-    __shared__ QrState lds4[4];
+    __shared__ double lds[kLsmLdsDoubles];
     const int later = date + 1;
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
     bool use = false;
     if (later <= a.m - 1) { // uniform
         if (fit) {
             int has = 0;
-            merge_solve_all(a.qr_partials + (size_t)(later & 1) * a.blocks, a.blocks, lds4, beta,
+            merge_solve_all(a.qr_partials + (size_t)(later & 1) * a.blocks, a.blocks, lds, beta,
                             &has);
             use = has != 0;
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
         householder_fold_rows<kLsmChunk>(st, rows, ys);
     }
     if (!regress) return; // uniform across the grid
-    const QrState t = block_tree_all(st, lds4);
+    const QrState t = block_factor(st, lds);
     if (threadIdx.x == 0) a.qr_partials[(size_t)(date & 1) * a.blocks + blockIdx.x] = t;
 }
 
