@@ -248,57 +248,90 @@ __device__ __forceinline__ bool fold_row(const LsmArgs& a, int date, uint64_t p,
     return itm;
 }
 
-// Block-wide sums of NV doubles, bit-identical in every lane: a butterfly inside each wave
-// (lanes l and l ^ off add the same two operands), then the four wave totals as
-// (w0 + w1) + (w2 + w3). The block's lane 0 also publishes NP values through the same
-// exchange. lds holds at least 4 NV + NP doubles.
+// DPP lane permutation of a double (both halves moved by the same control).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the wave's 64 lanes, bit-identical in every lane. Inside each row of 16 lanes:
+// lane ^ 1, lane ^ 2 (quad permutations), then the half-row and row mirrors, each pairing a
+// lane with one holding the other group's total, so both lanes of a pair add the same two
+// operands; then the four row totals as (r0 + r1) + (r2 + r3) from scalar reads.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v); // row_half_mirror
+    v += dpp_f64<0x140>(v); // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Block-wide sums of NV doubles, bit-identical in every lane: wave sums, then the four wave
+// totals as (w0 + w1) + (w2 + w3). The block's lane 0 also publishes NP values through the
+// same exchange. Consecutive exchanges alternate between two LDS buffers (buf = 0, 1), so one
+// barrier per exchange suffices: a buffer is rewritten only after every lane has passed the
+// other buffer's barrier, i.e. finished reading this one.
+constexpr int kLsmXchg = 4 * 6 + 5; // doubles per exchange buffer (NV <= 6, NP <= 5)
 template <int NV, int NP>
-__device__ __forceinline__ void block_sums(double (&v)[NV], double (&pub)[NP], double* lds) {
+__device__ __forceinline__ void block_sums(double (&v)[NV], double (&pub)[NP], double* lds,
+                                           int buf) {
+    static_assert(4 * NV + NP <= kLsmXchg, "exchange buffer too small");
+    double* x = lds + buf * kLsmXchg;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] += __shfl_xor(v[i], off, 64);
-    }
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum_f64(v[i]);
     const int w = threadIdx.x >> 6;
-    __syncthreads(); // readers of the previous exchange are done
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) lds[w * NV + i] = v[i];
+        for (int i = 0; i < NV; ++i) x[w * NV + i] = v[i];
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NP; ++i) lds[4 * NV + i] = pub[i];
+        for (int i = 0; i < NP; ++i) x[4 * NV + i] = pub[i];
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = (lds[i] + lds[NV + i]) + (lds[2 * NV + i] + lds[3 * NV + i]);
+    for (int i = 0; i < NV; ++i) v[i] = (x[i] + x[NV + i]) + (x[2 * NV + i] + x[3 * NV + i]);
 #pragma unroll
-    for (int i = 0; i < NP; ++i) pub[i] = lds[4 * NV + i];
+    for (int i = 0; i < NP; ++i) pub[i] = x[4 * NV + i];
 }
-constexpr int kLsmLdsDoubles = 4 * 6 + 5 + 8; // block_sums at column 0, plus the solve's results
+constexpr int kLsmLdsDoubles = 2 * kLsmXchg + 8; // two exchange buffers, the solve's results
 
-// One column step of the block's stacked Householder reduction (block_factor).
+// One column step of the block's stacked Householder reduction (block_factor). Every lane's
+// rows keep the upper-triangular zero pattern of a factor (row r is zero before column r:
+// the reflections of earlier columns add multiples of those zeros), so column J has non-zero
+// entries only in rows 0 .. J of each lane; lane 0's rows 0 .. J - 1 are finished and its row
+// J is the pivot, so the other lanes' rows 0 .. J form the column below the pivot.
 template <int J>
 __device__ __forceinline__ void stacked_reflect(double (&A)[kLsmBasis][kLsmBasis + 1],
                                                 bool lead, double& rows, double* lds) {
-    constexpr int NK = kLsmBasis - J;          // columns J+1 .. kLsmBasis (the last is z)
+    constexpr int NK = kLsmBasis - J;             // columns J+1 .. kLsmBasis (the last is z)
     constexpr int NV = 1 + NK + (J == 0 ? 1 : 0); // sum of squares, dot products, row count
     double v[NV], pub[1 + NK];
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = 0.0;
-    // Rows taking part: every row of every lane except lead's rows 0 .. J (its finished rows
-    // and the pivot row J).
+    if (!lead) {
 #pragma unroll
-    for (int r = 0; r < kLsmBasis; ++r) {
-        const double x = (!lead || r > J) ? A[r][J] : 0.0;
-        v[0] = __builtin_fma(x, x, v[0]);
+        for (int r = 0; r <= J; ++r) {
+            const double x = A[r][J];
+            v[0] = __builtin_fma(x, x, v[0]);
 #pragma unroll
-        for (int k = 0; k < NK; ++k) v[1 + k] = __builtin_fma(x, A[r][J + 1 + k], v[1 + k]);
+            for (int k = 0; k < NK; ++k) v[1 + k] = __builtin_fma(x, A[r][J + 1 + k], v[1 + k]);
+        }
     }
     if constexpr (J == 0) v[NV - 1] = rows;
 #pragma unroll
     for (int k = 0; k <= NK; ++k) pub[k] = lead ? A[J][J + k] : 0.0;
-    block_sums(v, pub, lds);
+    block_sums(v, pub, lds, J & 1);
     if constexpr (J == 0) rows = v[NV - 1];
     // The reflection v = x - alpha e_J of the stacked column (alpha = -sign(x1) |x|,
     // tau = 1 / (alpha (alpha - x1))), applied to the columns on its right; a column that is
@@ -313,15 +346,19 @@ __device__ __forceinline__ void stacked_reflect(double (&A)[kLsmBasis][kLsmBasis
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
         const double f = tau * __builtin_fma(v0, pub[1 + k], v[1 + k]);
+        if (lead) {
+            A[J][J + 1 + k] = __builtin_fma(-f, v0, A[J][J + 1 + k]);
+        } else {
 #pragma unroll
-        for (int r = 0; r < kLsmBasis; ++r)
-            if (!lead || r > J) A[r][J + 1 + k] = __builtin_fma(-f, A[r][J], A[r][J + 1 + k]);
-        if (lead) A[J][J + 1 + k] = __builtin_fma(-f, v0, A[J][J + 1 + k]);
+            for (int r = 0; r <= J; ++r) A[r][J + 1 + k] = __builtin_fma(-f, A[r][J], A[r][J + 1 + k]);
+        }
     }
+    if (lead) {
+        A[J][J] = live ? alpha : x1;
+    } else {
 #pragma unroll
-    for (int r = 0; r < kLsmBasis; ++r)
-        if (!lead || r > J) A[r][J] = live ? 0.0 : A[r][J];
-    if (lead) A[J][J] = live ? alpha : x1;
+        for (int r = 0; r <= J; ++r) A[r][J] = live ? 0.0 : A[r][J];
+    }
 }
 
 // The block's combined factor: the lanes' [R | z] rows stacked in lane order (1024 x 5)
@@ -354,15 +391,13 @@ __device__ __forceinline__ QrState block_factor(const QrState& in, double* lds) 
     return o;
 }
 
-// Combine nb <= 256 block factors (lane b loads factor b) and back-substitute
+// Combine nb <= 256 block factors (lane b holds factor b, the other lanes zeros) and back-substitute
 // (linalg.cpp:58-69) when at least four ITM rows were folded (lsm.cpp:102-104). Every lane of
 // the 256-lane block receives beta and has.
-__device__ __forceinline__ void merge_solve_all(const QrState* parts, uint32_t nb, double* lds,
-                                                double* beta, int* has) {
-    QrState acc = {};
-    if (threadIdx.x < nb) acc = parts[threadIdx.x];
+__device__ __forceinline__ void merge_solve_all(const QrState& acc, double* lds, double* beta,
+                                                int* has) {
     const QrState t = block_factor(acc, lds);
-    double* res = lds + kLsmLdsDoubles - 8; // not touched by block_sums
+    double* res = lds + 2 * kLsmXchg; // not touched by block_sums
     if (threadIdx.x == 0) {
         const int h = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
         double b[4] = {0.0, 0.0, 0.0, 0.0};
@@ -392,13 +427,34 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
                                                              int fit) {
     __shared__ double lds[kLsmLdsDoubles];
     const int later = date + 1;
+    // Chunks of kLsmChunk paths per lane, every load of a chunk issued before its first use;
+    // the first chunk's loads go out before the later date's regression is solved, so their
+    // latency overlaps it (issuing each next chunk's loads a chunk early measured slower).
+    // Loads are unconditional at in-bounds indices (path 0 past the end, store column 1 for
+    // the final date 0), so no load waits inside a branch; unused values are ignored.
+    const int col = date >= 1 ? date : 1;
+    auto load_chunk = [&](uint32_t j0, double (&cf)[kLsmChunk], double (&s1)[kLsmChunk],
+                          double (&s0)[kLsmChunk]) {
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u) {
+            const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
+            const uint64_t q = (j0 + u < a.per_lane && p < a.n_paths) ? p : 0;
+            cf[u] = a.cf[q];
+            s1[u] = store_get<FP64>(a, later, q);
+            s0[u] = store_get<FP64>(a, col, q);
+        }
+    };
+    const bool solve = fit && later <= a.m - 1; // uniform
+    QrState acc = {};
+    if (solve && threadIdx.x < a.blocks) acc = a.qr_partials[(size_t)(later & 1) * a.blocks + threadIdx.x];
+    double cf[kLsmChunk], s1[kLsmChunk], s0[kLsmChunk];
+    load_chunk(0, cf, s1, s0);
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
     bool use = false;
     if (later <= a.m - 1) { // uniform
         if (fit) {
             int has = 0;
-            merge_solve_all(a.qr_partials + (size_t)(later & 1) * a.blocks, a.blocks, lds, beta,
-                            &has);
+            merge_solve_all(acc, lds, beta, &has);
             use = has != 0;
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 a.has[later] = (uint8_t)has;
@@ -412,22 +468,9 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
         }
     }
     QrState st = {};
-    // Chunks of kLsmChunk paths: every load of a chunk is issued before the first use, so a
-    // lane waits for memory once per chunk instead of twice per path. Rows are folded in
-    // ascending path order, as one path at a time would.
+    // Rows are folded in ascending path order, as one path at a time would.
     for (uint32_t j0 = 0; j0 < a.per_lane; j0 += kLsmChunk) {
-        // Unconditional loads at in-bounds indices (path 0 past the end, store column 1 for
-        // the final date 0), so no load waits inside a branch; unused values are ignored.
-        double cf[kLsmChunk], s1[kLsmChunk], s0[kLsmChunk];
-        const int col = date >= 1 ? date : 1;
-#pragma unroll
-        for (uint32_t u = 0; u < kLsmChunk; ++u) {
-            const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
-            const uint64_t q = (j0 + u < a.per_lane && p < a.n_paths) ? p : 0;
-            cf[u] = a.cf[q];
-            s1[u] = store_get<FP64>(a, later, q);
-            s0[u] = store_get<FP64>(a, col, q);
-        }
+        if (j0 > 0) load_chunk(j0, cf, s1, s0);
 #pragma unroll
         for (uint32_t u = 0; u < kLsmChunk; ++u) {
             const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
@@ -438,19 +481,20 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
             a.cf[p] = c;
             cf[u] = c;
         }
-        if (!regress) continue; // uniform across the grid
-        // The chunk's rows (out-of-the-money and past-the-end paths as zero rows) folded into
-        // the lane's factor in one blocked Householder step.
-        double rows[kLsmChunk][4], ys[kLsmChunk];
+        if (regress) { // uniform across the grid
+            // The chunk's rows (out-of-the-money and past-the-end paths as zero rows) folded
+            // into the lane's factor in one blocked Householder step.
+            double rows[kLsmChunk][4], ys[kLsmChunk];
 #pragma unroll
-        for (uint32_t u = 0; u < kLsmChunk; ++u) {
-            const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
-            const bool in = j0 + u < a.per_lane && p < a.n_paths;
-            const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[u]);
-            ys[u] = itm ? cf[u] : 0.0;
-            st.rows += itm ? 1u : 0u;
+            for (uint32_t u = 0; u < kLsmChunk; ++u) {
+                const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
+                const bool in = j0 + u < a.per_lane && p < a.n_paths;
+                const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[u]);
+                ys[u] = itm ? cf[u] : 0.0;
+                st.rows += itm ? 1u : 0u;
+            }
+            householder_fold_rows<kLsmChunk>(st, rows, ys);
         }
-        householder_fold_rows<kLsmChunk>(st, rows, ys);
     }
     if (!regress) return; // uniform across the grid
     const QrState t = block_factor(st, lds);
