@@ -137,7 +137,23 @@ struct LsmArgs {
     float* cap_f1;
     float* cap_f2;
     float* cap_y;
+    // A batch of independent runs of the same contract on different stores (grid y = run):
+    // the distance between consecutive runs' store (bytes), cash flows, block factors and
+    // tile partials; coeffs and has advance by (M+1) x 4 and M+1. Zero for a single run.
+    uint64_t run_store_bytes, run_cf, run_qr, run_tiles;
 };
+
+// The arguments of run blockIdx.y of a batch (itself for a single run). The capture
+// buffers are not batched: only single runs capture.
+__device__ __forceinline__ void select_run(LsmArgs& a) {
+    const uint64_t y = blockIdx.y;
+    a.store = (char*)a.store + y * a.run_store_bytes;
+    a.cf += y * a.run_cf;
+    a.coeffs += y * (uint64_t)(a.m + 1) * 4;
+    a.has += y * (uint64_t)(a.m + 1);
+    a.qr_partials += y * a.run_qr;
+    a.partials += y * a.run_tiles;
+}
 
 constexpr double kFeatScale = 4.0;  // f1 = kFeatScale * (S/K - 1)
 constexpr double kTargetScale = 10.0; // y = kTargetScale * cf / K
@@ -193,6 +209,7 @@ __global__ __launch_bounds__(256) void lsm_paths_kernel(LsmArgs a) {
 // cf = intrinsic at maturity (lsm.cpp:76-81).
 template <bool FP64>
 __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
+    select_run(a);
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_paths) return;
     a.cf[p] = intrinsic_v(store_get<FP64>(a, a.m, p), a.strike, a.type);
@@ -425,6 +442,7 @@ __device__ __forceinline__ void merge_solve_all(const QrState& acc, double* lds,
 template <bool FP64>
 __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress,
                                                              int fit) {
+    select_run(a);
     __shared__ double lds[kLsmLdsDoubles];
     const int later = date + 1;
     // Chunks of kLsmChunk paths per lane, every load of a chunk issued before its first use;
@@ -503,6 +521,7 @@ __global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date
 
 // Welford over the final cash flows in the canonical tile order.
 __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
+    select_run(a);
     __shared__ Welford lds4[4];
     const uint64_t tile = blockIdx.x;
     Welford w = {0, 0.0, 0.0};
@@ -521,7 +540,10 @@ __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
     if (threadIdx.x == 0) a.partials[tile] = t;
 }
 
+// Grid y = run of a batch (runs' policies are consecutive).
 __global__ void lsm_clear_policy_kernel(uint8_t* has, double* coeffs, int m) {
+    has += (size_t)blockIdx.y * (size_t)(m + 1);
+    coeffs += (size_t)blockIdx.y * (size_t)(m + 1) * 4;
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
     if (i <= m) {
         has[i] = 0;
@@ -921,15 +943,18 @@ namespace {
 struct LsmRun {
     LsmArgs a;
     uint64_t blocks, tiles;
+    uint32_t runs; // independent runs in the batch (grid y)
 };
 
 // Lays out the scratch: [store][cf][coeffs][has][qr partials][tile partials] followed by
 // `extra_bytes` for the caller (returned in *extra when non-null). `with_store` false leaves
-// out the path store and the cash-flow vector (streaming pricing).
+// out the path store and the cash-flow vector (streaming pricing). `runs` > 1 lays out that
+// many independent runs, each region holding the runs one after another (the tile partials:
+// runs x tiles, then one merged result per run).
 int lsm_setup(double spot, double strike, double rate, double carry, double vol, double time,
               uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, ThreadCtx* ctx,
               LsmRun* run, size_t extra_bytes = 0, char** extra = nullptr,
-              bool with_store = true) {
+              bool with_store = true, uint32_t runs = 1) {
     if (type > 1 || m < 1 || n == 0 || n > (1ull << 30)) return MCDX_EINVAL;
     if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
     const bool fp64 = precision == MCDX_PREC_FP64;
@@ -955,17 +980,23 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     if (bb > kLsmMaxBlocks) bb = kLsmMaxBlocks;
     a.blocks = (uint32_t)bb;
     a.per_lane = (uint32_t)((n + bb * kLsmBlock - 1) / (bb * kLsmBlock));
+    if (runs < 1 || (runs > 1 && !with_store)) return MCDX_EINVAL;
     const size_t store_bytes = with_store ? (size_t)m * n * (fp64 ? 8 : 4) : 0;
+    a.run_store_bytes = runs > 1 ? store_bytes : 0;
+    a.run_cf = runs > 1 ? n : 0;
+    a.run_qr = runs > 1 ? 2 * (uint64_t)a.blocks : 0;
+    a.run_tiles = runs > 1 ? tiles : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
-    const size_t o_store = take(store_bytes), o_cf = take(with_store ? n * 8 : 0),
-                 o_coeffs = take((size_t)(m + 1) * 4 * 8), o_has = take((size_t)m + 1),
-                 o_qr = take(with_store ? 2 * (size_t)a.blocks * sizeof(QrState) : 0),
-                 o_part = take((tiles + 1) * sizeof(Welford)), o_extra = take(extra_bytes);
+    const size_t o_store = take(store_bytes * runs), o_cf = take(with_store ? n * 8 * runs : 0),
+                 o_coeffs = take((size_t)(m + 1) * 4 * 8 * runs),
+                 o_has = take(((size_t)m + 1) * runs),
+                 o_qr = take(with_store ? 2 * (size_t)a.blocks * sizeof(QrState) * runs : 0),
+                 o_part = take((tiles + 1) * runs * sizeof(Welford)), o_extra = take(extra_bytes);
     int st = ensure_dev(ctx, off);
     if (st) return st;
     char* base = (char*)ctx->dev;
@@ -979,6 +1010,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     run->a = a;
     run->blocks = blocks;
     run->tiles = tiles;
+    run->runs = runs;
     return MCDX_OK;
 }
 
@@ -986,14 +1018,15 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
 template <bool FP64>
 hipError_t enqueue_init(const LsmRun& r, hipStream_t s) {
     const unsigned g = (unsigned)r.blocks;
-    hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
-    hipLaunchKernelGGL(lsm_clear_policy_kernel, dim3((r.a.m + 1 + 255) / 256), dim3(256), 0, s,
-                       r.a.has, r.a.coeffs, r.a.m);
+    hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g, r.runs), dim3(256), 0, s, r.a);
+    hipLaunchKernelGGL(lsm_clear_policy_kernel, dim3((r.a.m + 1 + 255) / 256, r.runs), dim3(256),
+                       0, s, r.a.has, r.a.coeffs, r.a.m);
     return hipGetLastError();
 }
 
 template <bool FP64>
 hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
+    if (r.runs != 1) return hipErrorInvalidValue; // Philox paths: one run at a time
     const unsigned g = (unsigned)r.blocks;
     hipLaunchKernelGGL(lsm_paths_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
     hipError_t e = hipGetLastError();
@@ -1012,41 +1045,51 @@ struct PathSource {
 // Backward induction with regression (fit) or with a given policy already in coeffs/has.
 template <bool FP64>
 hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
-    const unsigned g = r.a.blocks;
+    const dim3 g(r.a.blocks, r.runs);
     for (int date = r.a.m - 1; date >= 1; --date)
-        hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, date,
+        hipLaunchKernelGGL(lsm_date_kernel<FP64>, g, dim3(kLsmBlock), 0, s, r.a, date,
                            fit ? 1 : 0, fit ? 1 : 0);
     // date 0: the decisions at date 1 (fitting date 1 first when fitting), no regression
-    hipLaunchKernelGGL(lsm_date_kernel<FP64>, dim3(g), dim3(kLsmBlock), 0, s, r.a, 0, 0,
+    hipLaunchKernelGGL(lsm_date_kernel<FP64>, g, dim3(kLsmBlock), 0, s, r.a, 0, 0,
                        fit ? 1 : 0);
-    hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles), dim3(kTileLanes), 0, s, r.a);
+    hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles, r.runs), dim3(kTileLanes), 0,
+                       s, r.a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_final_merge(r.a.partials, nullptr, r.tiles, 1, r.a.partials + r.tiles, s);
+    // run k's result at partials[runs x tiles + k]
+    return launch_final_merge(r.a.partials, nullptr, r.tiles, r.runs,
+                              r.a.partials + r.tiles * r.runs, s);
 }
 
 } // namespace
 
+// QR-LSM over `runs` stores: Philox paths (runs = 1), or Sobol + bridge paths, shifted by
+// replicates src.replicate .. src.replicate + runs - 1 when src.shifted; the runs share every
+// launch (grid y = run). out[k] = run k's price after the inception rule; the policy and the
+// inception flag are those of run 0.
 int lsm_price_src(double spot, double strike, double rate, double carry, double vol, double time,
                   uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision,
-                  const PathSource& src, mcdx_result* out, double* coeffs_out, uint8_t* has_out,
-                  uint8_t* inception_out) {
-    if (!out) return MCDX_EINVAL;
+                  const PathSource& src, uint32_t runs, mcdx_result* out, double* coeffs_out,
+                  uint8_t* has_out, uint8_t* inception_out) {
+    if (!out || runs < 1 || (runs > 1 && !src.qmc)) return MCDX_EINVAL;
     ThreadCtx* ctx;
     int st = thread_ctx(&ctx);
     if (st) return st;
     LsmRun r;
     char* scratch = nullptr;
     if ((st = lsm_setup(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, ctx, &r,
-                        src.qmc ? qmc_store_scratch_bytes(m) : 0, src.qmc ? &scratch : nullptr)))
+                        src.qmc ? qmc_store_scratch_bytes(m, runs) : 0,
+                        src.qmc ? &scratch : nullptr, true, runs)))
         return st;
     const size_t pol_bytes = (size_t)(m + 1) * 4 * 8 + (size_t)(m + 1);
-    if ((st = ensure_pinned(ctx, sizeof(Welford) + pol_bytes + 64))) return st;
+    const size_t res_bytes = ((size_t)runs * sizeof(Welford) + 63) & ~(size_t)63;
+    if ((st = ensure_pinned(ctx, res_bytes + pol_bytes))) return st;
     const bool fp64 = precision == MCDX_PREC_FP64;
     hipError_t e;
     if (src.qmc) {
         if ((st = qmc_fill_store(spot, rate, carry, vol, time, m, n, fp64, src.shifted,
-                                 src.shift_seed, src.replicate, r.a.store, scratch, ctx->stream)))
+                                 src.shift_seed, src.replicate, runs, r.a.store, scratch,
+                                 ctx->stream)))
             return st;
         e = fp64 ? enqueue_init<true>(r, ctx->stream) : enqueue_init<false>(r, ctx->stream);
     } else {
@@ -1056,29 +1099,29 @@ int lsm_price_src(double spot, double strike, double rate, double carry, double 
         e = fp64 ? enqueue_backward<true>(r, true, ctx->stream) : enqueue_backward<false>(r, true, ctx->stream);
     char* h = (char*)ctx->pinned;
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(h, r.a.partials + r.tiles * runs, (size_t)runs * sizeof(Welford),
+                           hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h + 64, r.a.coeffs, (size_t)(m + 1) * 32, hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(h + res_bytes, r.a.coeffs, (size_t)(m + 1) * 32, hipMemcpyDeviceToHost,
+                           ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h + 64 + (size_t)(m + 1) * 32, r.a.has, (size_t)(m + 1),
+        e = hipMemcpyAsync(h + res_bytes + (size_t)(m + 1) * 32, r.a.has, (size_t)(m + 1),
                            hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return status_of(e);
-    const mcdx_result res = finish(*(const Welford*)h);
-    const double* hc = (const double*)(h + 64);
-    const uint8_t* hh = (const uint8_t*)(h + 64 + (size_t)(m + 1) * 32);
+    const double* hc = (const double*)(h + res_bytes);
+    const uint8_t* hh = (const uint8_t*)(h + res_bytes + (size_t)(m + 1) * 32);
     if (coeffs_out)
         for (int i = 0; i < (m + 1) * 4; ++i) coeffs_out[i] = hc[i];
     if (has_out)
         for (int i = 0; i <= m; ++i) has_out[i] = hh[i];
     // inception decision (lsm.cpp:141-148)
     const double inc = intrinsic_v(spot, strike, type);
-    if (inc > res.price) {
-        if (inception_out) *inception_out = 1;
-        *out = mcdx_result{inc, 0.0, res.path_count};
-    } else {
-        if (inception_out) *inception_out = 0;
-        *out = res;
+    for (uint32_t k = 0; k < runs; ++k) {
+        const mcdx_result res = finish(((const Welford*)h)[k]);
+        const bool exercise = inc > res.price;
+        if (k == 0 && inception_out) *inception_out = exercise ? 1 : 0;
+        out[k] = exercise ? mcdx_result{inc, 0.0, res.path_count} : res;
     }
     return MCDX_OK;
 }
@@ -1087,8 +1130,11 @@ int lsm_price(double spot, double strike, double rate, double carry, double vol,
               uint8_t type, int32_t m, uint64_t n, uint64_t seed, int precision, mcdx_result* out,
               double* coeffs_out, uint8_t* has_out, uint8_t* inception_out) {
     return lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, seed, precision,
-                         PathSource{}, out, coeffs_out, has_out, inception_out);
+                         PathSource{}, 1, out, coeffs_out, has_out, inception_out);
 }
+
+// Device bytes of path stores one batch of shifted QMC-LSM runs may hold.
+constexpr size_t kLsmBatchStoreBytes = (size_t)2 << 30;
 
 // QR-LSM on Sobol + bridge paths: one plain run (replicates == 0) or `replicates` digitally
 // shifted runs whose prices give the estimate and its standard error.
@@ -1098,17 +1144,25 @@ int lsm_qmc(double spot, double strike, double rate, double carry, double vol, d
     if (!out || replicates == 1 || replicates > 4096 || n > (1ull << 32) - 2) return MCDX_EINVAL;
     if (replicates == 0)
         return lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, 0, precision,
-                             PathSource{true, false, 0, 0}, out, nullptr, nullptr, nullptr);
+                             PathSource{true, false, 0, 0}, 1, out, nullptr, nullptr, nullptr);
+    // The replicates run as batches sharing every launch, as many per batch as keep the
+    // batch's stores within kLsmBatchStoreBytes.
+    const size_t store_bytes = (size_t)m * n * (precision == MCDX_PREC_FP64 ? 8 : 4);
+    uint64_t per = store_bytes > 0 ? kLsmBatchStoreBytes / store_bytes : replicates;
+    per = per < 1 ? 1 : (per > replicates ? replicates : per);
     double sum = 0.0, ss = 0.0;
     std::vector<double> prices(replicates);
-    for (uint32_t r = 0; r < replicates; ++r) {
-        mcdx_result one;
+    std::vector<mcdx_result> batch(per);
+    for (uint32_t r0 = 0; r0 < replicates; r0 += (uint32_t)per) {
+        const uint32_t k = replicates - r0 < per ? replicates - r0 : (uint32_t)per;
         const int st = lsm_price_src(spot, strike, rate, carry, vol, time, type, m, n, 0, precision,
-                                     PathSource{true, true, shift_seed, r}, &one, nullptr,
-                                     nullptr, nullptr);
+                                     PathSource{true, true, shift_seed, r0}, k, batch.data(),
+                                     nullptr, nullptr, nullptr);
         if (st) return st;
-        prices[r] = one.price;
-        sum += one.price;
+        for (uint32_t j = 0; j < k; ++j) {
+            prices[r0 + j] = batch[j].price;
+            sum += batch[j].price;
+        }
     }
     const double mean = sum / (double)replicates;
     for (uint32_t r = 0; r < replicates; ++r) ss += (prices[r] - mean) * (prices[r] - mean);

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kMergeGroups) void final_merge_kernel(
     uint64_t n_tiles_single, Welford* __restrict__ out) {
     __shared__ Welford lds[kMergeGroups];
     const uint32_t c = blockIdx.x;
-    uint64_t base = 0, n_tiles = n_tiles_single;
+    uint64_t base = (uint64_t)c * n_tiles_single, n_tiles = n_tiles_single;
     if (tile_offsets) {
         base = tile_offsets[c];
         n_tiles = tile_offsets[c + 1] - base;

@@ -34,8 +34,9 @@ inline hipStream_t pick_stream(ThreadCtx* ctx, void* stream) {
     return stream ? (hipStream_t)stream : ctx->stream;
 }
 
-// Launch the fixed-order merge of `n_tiles` tile partials (one group per contract when
-// batched) into `out[contract]`. `tile_offsets` may be null for a single contract.
+// Launch the fixed-order merge of each contract's tile partials into `out[contract]`: contract
+// c's tiles are tile_offsets[c] .. tile_offsets[c + 1] - 1, or, with tile_offsets null,
+// c * n_tiles .. (c + 1) * n_tiles - 1.
 hipError_t launch_final_merge(const Welford* partials, const uint64_t* tile_offsets,
                               uint64_t n_tiles, uint32_t n_contracts, Welford* out,
                               hipStream_t stream);

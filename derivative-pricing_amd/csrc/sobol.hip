@@ -716,6 +716,9 @@ __global__ __launch_bounds__(256) void qmc_store_kernel(const uint32_t* __restri
                                                         double spot, double vol, float vol2,
                                                         uint64_t n, void* store) {
     extern __shared__ unsigned char smem[];
+    // grid y = run: its shift words and its store follow the previous run's
+    shift += (size_t)blockIdx.y * (size_t)n_ops;
+    store = (char*)store + (size_t)blockIdx.y * (size_t)n_ops * n * (FP64 ? 8 : 4);
     const uint32_t t = threadIdx.x;
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + t;
     if (p >= n) return; // no barrier follows
@@ -818,27 +821,31 @@ int rqmc_path(const mcdx_gbm_contract* c, int precision, uint32_t replicates,
     return MCDX_OK;
 }
 
-size_t qmc_store_scratch_bytes(int32_t m) {
+size_t qmc_store_scratch_bytes(int32_t m, uint32_t runs) {
     const size_t ops = ((size_t)(m > 0 ? m : 1) * sizeof(BridgeOp) + 255) & ~(size_t)255;
-    const size_t sh = ((size_t)(m > 0 ? m : 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t sh = ((size_t)(m > 0 ? m : 1) * runs * sizeof(uint32_t) + 255) & ~(size_t)255;
     return ops + sh;
 }
 
 int qmc_fill_store(double spot, double rate, double carry, double vol, double time, int32_t m,
                    uint64_t n, bool fp64, bool shifted, uint64_t shift_seed, uint32_t replicate,
-                   void* store, void* scratch, hipStream_t s) {
+                   uint32_t runs, void* store, void* scratch, hipStream_t s) {
     if (m < 1 || m > (int32_t)kSobolDims || n == 0 || n > (1ull << 32) - 2) return MCDX_EINVAL;
+    if (runs < 1 || runs > 65535 || (runs > 1 && !shifted)) return MCDX_EINVAL;
     const uint32_t* v;
     int st = device_tables(&v);
     if (st) return st;
     std::vector<BridgeOp> ops;
     const int slots = build_bridge(m, spot, rate, carry, vol, time, ops);
-    if (slots < 0) return MCDX_EINVAL;
+    // one op per date: the kernel's per-run strides are ops.size() shift words and dates
+    if (slots < 0 || ops.size() != (size_t)m) return MCDX_EINVAL;
     const size_t shmem = (size_t)slots * 256 * (fp64 ? sizeof(double) : sizeof(float));
     if (shmem > 64 * 1024) return MCDX_EINVAL;
-    std::vector<uint32_t> shift((size_t)m, 0u);
+    std::vector<uint32_t> shift((size_t)m * runs, 0u);
     if (shifted)
-        for (int32_t d = 0; d < m; ++d) shift[(size_t)d] = philox_bits(shift_seed, replicate, (uint32_t)d, 3u);
+        for (uint32_t k = 0; k < runs; ++k)
+            for (int32_t d = 0; d < m; ++d)
+                shift[(size_t)k * m + d] = philox_bits(shift_seed, replicate + k, (uint32_t)d, 3u);
     BridgeOp* dops = (BridgeOp*)scratch;
     uint32_t* dshift = (uint32_t*)((char*)scratch + (((size_t)m * sizeof(BridgeOp) + 255) & ~(size_t)255));
     hipError_t e = hipMemcpyAsync(dops, ops.data(), ops.size() * sizeof(BridgeOp),
@@ -853,14 +860,14 @@ int qmc_fill_store(double spot, double rate, double carry, double vol, double ti
     const float vol2 = (float)(vol * log2e);
     if (fp64) {
         if (shifted)
-            hipLaunchKernelGGL((qmc_store_kernel<true, true>), dim3(grid), dim3(256), shmem, s, v, dops,
+            hipLaunchKernelGGL((qmc_store_kernel<true, true>), dim3(grid, runs), dim3(256), shmem, s, v, dops,
                                (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
         else
             hipLaunchKernelGGL((qmc_store_kernel<true, false>), dim3(grid), dim3(256), shmem, s, v, dops,
                                (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
     } else {
         if (shifted)
-            hipLaunchKernelGGL((qmc_store_kernel<false, true>), dim3(grid), dim3(256), shmem, s, v, dops,
+            hipLaunchKernelGGL((qmc_store_kernel<false, true>), dim3(grid, runs), dim3(256), shmem, s, v, dops,
                                (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
         else
             hipLaunchKernelGGL((qmc_store_kernel<false, false>), dim3(grid), dim3(256), shmem, s, v, dops,

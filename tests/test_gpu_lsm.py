@@ -158,6 +158,17 @@ def test_qmc_lsm_fp64_matches_oracle(typ, s0, vol, m, n):
     assert abs(p - reps.mean()) <= 1e-14 * p and abs(se - reps.std(ddof=1) / np.sqrt(3)) <= 1e-12
 
 
+def test_qmc_lsm_replicates_independent_of_batch():
+    # The shifted runs share every launch (grid y = run); each run's price must not depend on
+    # which runs share its batch: 3 runs equal the first 3 of 5, bit for bit, in both modes.
+    args = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 20, 12_345)
+    for prec in (FP32, FP64):
+        _, _, _, r3 = mcdx.lsm_qmc(*args, 3, 99, prec)
+        _, _, _, r5 = mcdx.lsm_qmc(*args, 5, 99, prec)
+        assert np.array_equal(r3, r5[:3]), (prec, r3, r5)
+        assert len(set(r5.tolist())) == 5  # distinct shifts give distinct prices
+
+
 def test_qmc_lsm_against_crr_and_monte_carlo():
     # 16 shifted runs of 2^16 Sobol paths: the estimate sits where the Philox LSM does (both
     # low-biased by the same regression), with a much smaller error than Monte Carlo's.
