@@ -6,8 +6,8 @@ namespace mcdx {
 
 // One tile of 1024 paths per block: lane folds its 4 paths (Welford, stats.hpp:23-29),
 // then the block's pairwise tree writes the tile partial.
-template <int KIND, bool FP64, bool ANTI>
-__global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
+template <int KIND, bool FP64, bool ANTI, int SUMS>
+__device__ __forceinline__ void tile_body(const GbmArgs& a) {
     __shared__ Welford lds[4];
     const uint64_t tile = blockIdx.x;
     Welford w = {0, 0.0, 0.0};
@@ -17,7 +17,7 @@ __global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
         const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
         const uint64_t path = base + threadIdx.x;
         if (path < a.n_paths) {
-            const double x = path_sample<KIND, FP64, ANTI>(a, path, (uint32_t)(base >> 32));
+            const double x = path_sample<KIND, FP64, ANTI, SUMS>(a, path, (uint32_t)(base >> 32));
             w.n += 1;
             const double d = x - w.mean;
             w.mean += d * (1.0 / (double)(k + 1));
@@ -26,6 +26,24 @@ __global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
     }
     const Welford t = block_merge_256(w, lds);
     if (threadIdx.x == 0) a.partials[tile] = t;
+}
+
+template <int KIND, bool FP64, bool ANTI>
+__global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
+    tile_body<KIND, FP64, ANTI, kSumsAny>(a);
+}
+
+// FP32 Asian tiles with the sums fixed at compile time: one step loop and fewer registers
+// than the per-path dispatch (86 VGPRs, 5 waves per SIMD); without antithetics the single-sum
+// loops are held to 64 VGPRs (8 waves per SIMD; one 8-byte spill outside the step loop).
+template <bool ANTI, int SUMS>
+__global__ __launch_bounds__(kTileLanes) void gbm_asian_tile_kernel(GbmArgs a) {
+    tile_body<kAsian, false, ANTI, SUMS>(a);
+}
+template <int SUMS>
+__global__ __launch_bounds__(kTileLanes) __attribute__((amdgpu_waves_per_eu(8))) void
+gbm_asian_tile_kernel_w8(GbmArgs a) {
+    tile_body<kAsian, false, false, SUMS>(a);
 }
 
 template <int KIND, bool FP64, bool ANTI>
@@ -40,7 +58,22 @@ template <bool FP64, bool ANTI>
 void launch_tiles_k(int kind, const GbmArgs& a, unsigned grid, hipStream_t s) {
     switch (kind) {
     case kTerminal: hipLaunchKernelGGL((gbm_tile_kernel<kTerminal, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a); break;
-    case kAsian: hipLaunchKernelGGL((gbm_tile_kernel<kAsian, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a); break;
+    case kAsian:
+        if constexpr (!FP64) {
+            const int sums = asian_sums(a);
+            if constexpr (!ANTI) {
+                if (sums == kSumsArith) hipLaunchKernelGGL((gbm_asian_tile_kernel_w8<kSumsArith>), dim3(grid), dim3(kTileLanes), 0, s, a);
+                else if (sums == kSumsGeo) hipLaunchKernelGGL((gbm_asian_tile_kernel_w8<kSumsGeo>), dim3(grid), dim3(kTileLanes), 0, s, a);
+                else hipLaunchKernelGGL((gbm_asian_tile_kernel<false, kSumsBoth>), dim3(grid), dim3(kTileLanes), 0, s, a);
+            } else {
+                if (sums == kSumsArith) hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsArith>), dim3(grid), dim3(kTileLanes), 0, s, a);
+                else if (sums == kSumsGeo) hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsGeo>), dim3(grid), dim3(kTileLanes), 0, s, a);
+                else hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsBoth>), dim3(grid), dim3(kTileLanes), 0, s, a);
+            }
+        } else {
+            hipLaunchKernelGGL((gbm_tile_kernel<kAsian, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a);
+        }
+        break;
     case kBarrier: hipLaunchKernelGGL((gbm_tile_kernel<kBarrier, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a); break;
     case kBarrierBB: hipLaunchKernelGGL((gbm_tile_kernel<kBarrierBB, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a); break;
     default: hipLaunchKernelGGL((gbm_tile_kernel<kLookback, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a); break;

@@ -152,7 +152,12 @@ __device__ __forceinline__ double settle_f32(const GbmArgs& a, const StateF32& s
 // `phh` = path >> 32, passed separately so that a caller whose lanes share it (a block of 256
 // aligned paths) can hand it over in a scalar register: the first Philox round's left word and
 // the second round's M0 product then stay on the scalar unit.
-template <int KIND, bool ANTI>
+// Which running sums an Asian path keeps: decided per path from the contract (kSumsAny) or
+// fixed at compile time (arithmetic, geometric, both), which leaves one step loop in the
+// kernel and fewer registers.
+enum AsianSums : int { kSumsAny = 0, kSumsArith = 1, kSumsGeo = 2, kSumsBoth = 3 };
+
+template <int KIND, bool ANTI, int SUMS = kSumsAny>
 __device__ double path_f32(const GbmArgs& a, uint64_t path, uint32_t phh) {
 #pragma clang fp contract(fast)
     const uint32_t plo = (uint32_t)path;
@@ -232,7 +237,13 @@ __device__ double path_f32(const GbmArgs& a, uint64_t path, uint32_t phh) {
         };
         using T = std::true_type;
         using F = std::false_type;
-        if constexpr (KIND == kAsian) {
+        if constexpr (KIND == kAsian && SUMS == kSumsArith) {
+            steps(T{}, F{});
+        } else if constexpr (KIND == kAsian && SUMS == kSumsGeo) {
+            steps(F{}, T{});
+        } else if constexpr (KIND == kAsian && SUMS == kSumsBoth) {
+            steps(T{}, T{});
+        } else if constexpr (KIND == kAsian) {
             if (need_arith && need_geo) steps(T{}, T{});
             else if (need_arith) steps(T{}, F{});
             else steps(F{}, T{});
@@ -361,10 +372,17 @@ __device__ double path_f64(const GbmArgs& a, uint64_t path, uint32_t phh) {
 }
 
 // One path's sample; phh must equal path >> 32.
-template <int KIND, bool FP64, bool ANTI>
+template <int KIND, bool FP64, bool ANTI, int SUMS = kSumsAny>
 __device__ __forceinline__ double path_sample(const GbmArgs& a, uint64_t path, uint32_t phh) {
     if constexpr (FP64) return path_f64<KIND, ANTI>(a, path, phh);
-    else return path_f32<KIND, ANTI>(a, path, phh);
+    else return path_f32<KIND, ANTI, SUMS>(a, path, phh);
+}
+
+// The Asian sums a contract needs (the same choice path_f32 makes per path).
+__host__ __device__ inline int asian_sums(const GbmArgs& a) {
+    const bool arith = a.average_style == 0 || a.cv;
+    const bool geo = a.average_style == 1 || a.cv;
+    return arith && geo ? kSumsBoth : (arith ? kSumsArith : kSumsGeo);
 }
 
 
