@@ -43,8 +43,8 @@ OPS_OBSERVE = {"asian": 3, "barrier_uo": 2, "lookback_float": 2}
 
 # HBM bytes per launch of each 2^24 x 252 path kernel: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE per dispatch, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by the same counters'
-# reading of a plain 1 GiB device copy (FETCH 353.1, 223.5, 221.0 KiB; WRITE 768 KiB).
-TRAFFIC_BYTES = {"asian": 1509632, "barrier_uo": 1244160, "lookback_float": 1239040}
+# reading of a plain 1 GiB device copy (FETCH 454.8, 224.1, 223.0 KiB; WRITE 768 KiB).
+TRAFFIC_BYTES = {"asian": 1717862, "barrier_uo": 1245440, "lookback_float": 1243136}
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FMA)
 
 

@@ -34,16 +34,12 @@ __global__ __launch_bounds__(kTileLanes) void gbm_tile_kernel(GbmArgs a) {
 }
 
 // FP32 Asian tiles with the sums fixed at compile time: one step loop and fewer registers
-// than the per-path dispatch (86 VGPRs, 5 waves per SIMD); without antithetics the single-sum
-// loops are held to 64 VGPRs (8 waves per SIMD; one 8-byte spill outside the step loop).
+// than the per-path dispatch (65 instead of 86 VGPRs without antithetics: 7 waves per SIMD
+// instead of 5). Holding it to 64 VGPRs (8 waves) spilled a register to scratch and measured
+// no faster.
 template <bool ANTI, int SUMS>
 __global__ __launch_bounds__(kTileLanes) void gbm_asian_tile_kernel(GbmArgs a) {
     tile_body<kAsian, false, ANTI, SUMS>(a);
-}
-template <int SUMS>
-__global__ __launch_bounds__(kTileLanes) __attribute__((amdgpu_waves_per_eu(8))) void
-gbm_asian_tile_kernel_w8(GbmArgs a) {
-    tile_body<kAsian, false, false, SUMS>(a);
 }
 
 template <int KIND, bool FP64, bool ANTI>
@@ -61,15 +57,9 @@ void launch_tiles_k(int kind, const GbmArgs& a, unsigned grid, hipStream_t s) {
     case kAsian:
         if constexpr (!FP64) {
             const int sums = asian_sums(a);
-            if constexpr (!ANTI) {
-                if (sums == kSumsArith) hipLaunchKernelGGL((gbm_asian_tile_kernel_w8<kSumsArith>), dim3(grid), dim3(kTileLanes), 0, s, a);
-                else if (sums == kSumsGeo) hipLaunchKernelGGL((gbm_asian_tile_kernel_w8<kSumsGeo>), dim3(grid), dim3(kTileLanes), 0, s, a);
-                else hipLaunchKernelGGL((gbm_asian_tile_kernel<false, kSumsBoth>), dim3(grid), dim3(kTileLanes), 0, s, a);
-            } else {
-                if (sums == kSumsArith) hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsArith>), dim3(grid), dim3(kTileLanes), 0, s, a);
-                else if (sums == kSumsGeo) hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsGeo>), dim3(grid), dim3(kTileLanes), 0, s, a);
-                else hipLaunchKernelGGL((gbm_asian_tile_kernel<true, kSumsBoth>), dim3(grid), dim3(kTileLanes), 0, s, a);
-            }
+            if (sums == kSumsArith) hipLaunchKernelGGL((gbm_asian_tile_kernel<ANTI, kSumsArith>), dim3(grid), dim3(kTileLanes), 0, s, a);
+            else if (sums == kSumsGeo) hipLaunchKernelGGL((gbm_asian_tile_kernel<ANTI, kSumsGeo>), dim3(grid), dim3(kTileLanes), 0, s, a);
+            else hipLaunchKernelGGL((gbm_asian_tile_kernel<ANTI, kSumsBoth>), dim3(grid), dim3(kTileLanes), 0, s, a);
         } else {
             hipLaunchKernelGGL((gbm_tile_kernel<kAsian, FP64, ANTI>), dim3(grid), dim3(kTileLanes), 0, s, a);
         }
