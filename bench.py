@@ -296,7 +296,7 @@ def c4_entry(torch, mcdx, L, stream, handle):
         book.enqueue(handle.value)
         batched = timed_events(torch, stream, lambda: book.enqueue(handle.value), reps=2)
         res = book.results()
-        greeks = book.greeks() if flags & mcdx.Book.GREEKS else None
+        greeks = (book.greeks(), book.pathwise()) if flags & mcdx.Book.GREEKS else None
         sched = book.schedule()
         book.close()
         return batched, tuned, res, greeks, sched
@@ -344,7 +344,7 @@ def c4_entry(torch, mcdx, L, stream, handle):
     out["distinct_seeds"] = {"batched_ms": d_ms, "per_contract_ms": dp_ms,
                              "speedup": dp_ms / d_ms, "bitwise_equal_to_per_contract": dres == dpres}
     progress("C4 distinct-seed book timed")
-    g_ms, _, gres, greeks, _ = run(crn, mcdx.Book.GREEKS)
+    g_ms, _, gres, (greeks, pathwise), _ = run(crn, mcdx.Book.GREEKS)
     bumped = []
     for c in crn:
         hs, hv = max(0.01 * c.spot, 1e-4), max(0.01 * c.vol, 1e-4)
@@ -354,13 +354,31 @@ def c4_entry(torch, mcdx, L, stream, handle):
             setattr(x, field, val)
             bumped.append(x)
     bp_ms, _ = per(bumped)
+    # Pathwise delta and vega on the same paths (European and Asian contracts), against the
+    # finite differences: the two estimators of the same Greeks agree within their errors.
+    agree = tot_pw = 0
+    for (d, _, v), (pd, pv) in zip(greeks, pathwise):
+        if pd[2] == 0:
+            continue
+        tot_pw += 1
+        agree += (abs(pd[0] - d[0]) <= 4 * (pd[1] ** 2 + d[1] ** 2) ** 0.5 and
+                  abs(pv[0] - v[0]) <= 4 * (pv[1] ** 2 + v[1] ** 2) ** 0.5)
+    first_pw = next(i for i, (pd, _) in enumerate(pathwise) if pd[2] > 0)
     out["greeks"] = {"what": "price + CRN central-difference delta, gamma, vega per contract "
-                             "(default bumps: 1% spot, 1% vol)",
+                             "(default bumps: 1% spot, 1% vol) + pathwise delta, vega "
+                             "(European and Asian contracts) on the same paths",
                      "batched_ms": g_ms, "per_contract_ms": p_ms + bp_ms,
                      "per_contract_launches": 5 * len(crn), "speedup": (p_ms + bp_ms) / g_ms,
                      "prices_bitwise_equal": gres == pres,
+                     "pathwise_contracts": tot_pw,
+                     "pathwise_within_4se_of_fd": agree,
                      "example_contract_0": {"delta": greeks[0][0][:2], "gamma": greeks[0][1][:2],
-                                            "vega": greeks[0][2][:2]}}
+                                            "vega": greeks[0][2][:2]},
+                     "example_pathwise_contract": {
+                         "index": first_pw, "fd_delta": greeks[first_pw][0][:2],
+                         "pathwise_delta": pathwise[first_pw][0][:2],
+                         "fd_vega": greeks[first_pw][2][:2],
+                         "pathwise_vega": pathwise[first_pw][1][:2]}}
     return out
 
 

@@ -24,9 +24,14 @@ namespace mcdx {
 
 // A shared-draw group: up to kFuseMax contracts of one kind, seed and step count (path counts
 // descending), evaluated on the same paths from one normal per step. With `greeks` the
-// members are a contract and its four bumped scenarios (spot up/down, vol up/down) and the
-// per-path central differences go to three more result slots.
+// members are a contract and its four bumped scenarios (spot up/down, vol up/down); the
+// per-path central differences go to three more result slots and, for a European or Asian
+// contract (no control variate), the pathwise delta and vega of the unbumped member
+// (pathwise.cpp:28-147) to two more.
 constexpr int kFuseMax = 8;
+constexpr int kGreekSlots = 5; // FD delta, gamma, vega; pathwise delta, vega
+
+enum PathwiseKind : int32_t { kPwNone = 0, kPwTerminal = 1, kPwAverage = 2 };
 
 struct FusedGroup {
     uint32_t c[kFuseMax];
@@ -34,7 +39,18 @@ struct FusedGroup {
     uint32_t greeks, gslot;
     double inv_2hs, inv_hs2, inv_2hv;
     int32_t kind;
+    int32_t pw;                  // PathwiseKind of member 0
+    double pw_sqrt_t, pw_vol_t;  // European: sqrt(T), vol T
+    float pw_sdt, pw_vdt;        // Asian: sqrt(dt), vol dt
 };
+
+// Member 0's pathwise estimator: the reference has one for European calls and puts
+// (pathwise.cpp:28-62) and for Asians (:64-147); none with a control variate.
+inline int32_t pathwise_kind(const mcdx_gbm_contract& c) {
+    if (c.product == MCDX_EUROPEAN) return kPwTerminal;
+    if (c.product == MCDX_ASIAN && !c.control_variate) return kPwAverage;
+    return kPwNone;
+}
 
 } // namespace mcdx
 
@@ -110,11 +126,11 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
         if ((uint32_t)i < g.k && A[i]->n_paths > tile * kTilePaths) kact = i + 1;
     }
     const uint32_t k0 = (uint32_t)A[0]->seed, k1 = (uint32_t)(A[0]->seed >> 32);
-    Welford w[kFuseMax], gw[3];
+    Welford w[kFuseMax], gw[kGreekSlots];
 #pragma unroll
     for (int i = 0; i < kFuseMax; ++i) w[i] = Welford{0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gw[j] = Welford{0, 0.0, 0.0};
+    for (int j = 0; j < kGreekSlots; ++j) gw[j] = Welford{0, 0.0, 0.0};
 #pragma unroll 1
     for (int k = 0; k < kPathsPerLane; ++k) {
         // base is a multiple of 256 and threadIdx.x < 256: every lane's path >> 32 is base >> 32.
@@ -122,6 +138,7 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
         const uint64_t path = base + threadIdx.x;
         const uint32_t plo = (uint32_t)path, phh = (uint32_t)(base >> 32);
         double v[kFuseMax];
+        double pwd = 0.0, pwv = 0.0; // member 0's pathwise delta and vega samples
         if constexpr (KIND == kTerminal) {
             const float z = inv_normal_f32(philox_w0(plo, phh, 0u, 0u, k0, k1));
 #pragma unroll
@@ -130,6 +147,14 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                                       terminal_payoff(*A[i], A[i]->spot * (double)fast_exp2(
                                                                  fmaf(A[i]->sig2, z, A[i]->mu2)))
                                 : 0.0;
+            if (g.pw == kPwTerminal) { // pathwise.cpp:38-58 on this path's FP32 draw
+                const GbmArgs& a0 = *A[0];
+                const double st = a0.spot * (double)fast_exp2(fmaf(a0.sig2, z, a0.mu2));
+                const double phi = a0.type == 0 ? 1.0 : -1.0;
+                const double slope = phi * (st - a0.strike) > 0.0 ? phi : 0.0;
+                pwd = a0.discount * slope * (st / a0.spot);
+                pwv = a0.discount * slope * (st * (g.pw_sqrt_t * (double)z - g.pw_vol_t));
+            }
         } else {
             StateF32 s[kFuseMax];
             // Step constants, read once per item for every member (A[i] is always a valid
@@ -149,6 +174,11 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                 mu2[i] = A[i]->mu2;
                 sig2[i] = A[i]->sig2;
             }
+            // Member 0's pathwise state (Asian): cumulative normal, step count, d ln S / d vol
+            // at the last date, and its sums over the dates (pathwise.cpp:84-97).
+            const bool pw = KIND == kAsian && g.pw == kPwAverage; // uniform
+            float pw_cz = 0.f, pw_i = 0.f, pw_dlv = 0.f;
+            double pw_ssv = 0.0, pw_sdv = 0.0;
             auto apply = [&](float z) {
 #pragma unroll
                 for (int i = 0; i < kFuseMax; ++i) {
@@ -166,6 +196,15 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                             t.xmax = fmaxf(t.xmax, t.x);
                             t.xmin = fminf(t.xmin, t.x);
                         }
+                    }
+                }
+                if constexpr (KIND == kAsian) {
+                    if (pw) { // d ln S_i / d vol = -vol t_i + sqrt(dt) (z_1 + ... + z_i)
+                        pw_cz += z;
+                        pw_i += 1.f;
+                        pw_dlv = fmaf(g.pw_sdt, pw_cz, -g.pw_vdt * pw_i);
+                        if (na[0]) pw_ssv += (double)(s[0].last_u * pw_dlv); // sum (S_i/S0) d ln S_i
+                        else pw_sdv += (double)pw_dlv;
                     }
                 }
             };
@@ -191,6 +230,28 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             flush();
 #pragma unroll
             for (int i = 0; i < kFuseMax; ++i) v[i] = i < kact ? settle_f32<KIND>(*A[i], s[i]) : 0.0;
+            if constexpr (KIND == kAsian) {
+                if (pw) { // pathwise.cpp:99-143, averages formed as settle_f32 forms them
+                    const GbmArgs& a0 = *A[0];
+                    const bool arith = a0.average_style == 0;
+                    const double mm = (double)a0.m;
+                    const double last = a0.spot * (double)(arith ? s[0].last_u : fast_exp2(s[0].x));
+                    const double avg = arith ? a0.spot * (s[0].sum_u / mm)
+                                             : a0.spot * exp2(s[0].sum_x / mm);
+                    const double davg_s = avg / a0.spot;
+                    const double davg_v = arith ? a0.spot * (pw_ssv / mm) : avg * (pw_sdv / mm);
+                    const double ref = a0.strike_style == 0 ? avg - a0.strike : last - avg;
+                    const double phi = a0.type == 0 ? 1.0 : -1.0;
+                    const double slope = phi * ref > 0.0 ? phi : 0.0;
+                    if (a0.strike_style == 0) {
+                        pwd = a0.discount * (slope * davg_s);
+                        pwv = a0.discount * (slope * davg_v);
+                    } else {
+                        pwd = a0.discount * (slope * (last / a0.spot - davg_s));
+                        pwv = a0.discount * (slope * (last * (double)pw_dlv - davg_v));
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int i = 0; i < kFuseMax; ++i)
@@ -199,6 +260,10 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             welford_add_k(gw[0], (v[1] - v[2]) * g.inv_2hs, k);
             welford_add_k(gw[1], ((v[1] - 2.0 * v[0]) + v[2]) * g.inv_hs2, k);
             welford_add_k(gw[2], (v[3] - v[4]) * g.inv_2hv, k);
+            if (g.pw != kPwNone) {
+                welford_add_k(gw[3], pwd, k);
+                welford_add_k(gw[4], pwv, k);
+            }
         }
     }
 #pragma unroll
@@ -210,7 +275,7 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
     }
     if (g.greeks) {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < kGreekSlots; ++j) {
             const Welford t = block_merge_256_all(gw[j], lds4);
             partials[toff[g.gslot + j] + tile] = t;
         }
@@ -424,6 +489,19 @@ void free_book(mcdx_book* b) {
 
 using namespace mcdx;
 
+namespace {
+// The Greek slots of every contract (kGreekSlots each, consecutive from greek_slot[0]).
+int book_greek_results(mcdx_book* b, std::vector<Welford>& h) {
+    hipStream_t s = b->last_stream ? b->last_stream : b->stream;
+    h.resize((size_t)kGreekSlots * b->n);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+        e = hipMemcpy(h.data(), b->d_results + b->greek_slot[0], h.size() * sizeof(Welford),
+                      hipMemcpyDeviceToHost);
+    return status_of(e);
+}
+} // namespace
+
 extern "C" {
 
 int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
@@ -448,7 +526,8 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
     b->precision = precision;
     b->contracts.assign(contracts, contracts + n);
     // Greeks: four bumped scenarios per contract (default_bump_sizes,
-    // finite_difference.cpp:31-38), then three result slots (delta, gamma, vega).
+    // finite_difference.cpp:31-38), then kGreekSlots result slots (FD delta, gamma, vega;
+    // pathwise delta, vega).
     std::vector<double> hs(n), hv(n);
     if (greeks) {
         for (uint32_t i = 0; i < n; ++i) {
@@ -469,7 +548,7 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
         b->greek_slot.resize(n);
         for (uint32_t i = 0; i < n; ++i) {
             b->greek_slot[i] = (uint32_t)b->contracts.size();
-            for (int j = 0; j < 3; ++j) b->contracts.push_back(contracts[i]);
+            for (int j = 0; j < kGreekSlots; ++j) b->contracts.push_back(contracts[i]);
         }
     }
     const uint32_t slots = (uint32_t)b->contracts.size();
@@ -514,8 +593,15 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
             g.inv_2hs = 1.0 / (2.0 * hs[i]);
             g.inv_hs2 = 1.0 / (hs[i] * hs[i]);
             g.inv_2hv = 1.0 / (2.0 * hv[i]);
+            const mcdx_gbm_contract& c = contracts[i];
+            g.pw = pathwise_kind(c);
+            const double dt = g.kind == kTerminal ? c.time : c.time / (double)c.monitoring_points;
+            g.pw_sqrt_t = sqrt(c.time);
+            g.pw_vol_t = c.vol * c.time;
+            g.pw_sdt = (float)sqrt(dt);
+            g.pw_vdt = (float)(c.vol * dt);
             for (uint32_t j = 0; j < g.k; ++j) b->group_of[g.c[j]] = (int32_t)b->groups.size();
-            for (int j = 0; j < 3; ++j) b->group_of[g.gslot + j] = -2;
+            for (int j = 0; j < kGreekSlots; ++j) b->group_of[g.gslot + j] = -2;
             b->groups.push_back(g);
         }
     } else if (share) {
@@ -648,15 +734,25 @@ int mcdx_book_results(mcdx_book* b, mcdx_result* out) {
 
 int mcdx_book_greeks(mcdx_book* b, mcdx_result* out) {
     if (!b || !out || !(b->flags & MCDX_BOOK_GREEKS)) return MCDX_EINVAL;
-    hipStream_t s = b->last_stream ? b->last_stream : b->stream;
-    const uint32_t first = b->greek_slot[0];
-    const uint32_t count = 3 * b->n;
-    std::vector<Welford> h(count);
-    hipError_t e = hipStreamSynchronize(s);
-    if (e == hipSuccess)
-        e = hipMemcpy(h.data(), b->d_results + first, count * sizeof(Welford), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return status_of(e);
-    for (uint32_t i = 0; i < count; ++i) out[i] = finish(h[i]);
+    std::vector<Welford> h;
+    const int st = book_greek_results(b, h);
+    if (st) return st;
+    for (uint32_t i = 0; i < b->n; ++i)
+        for (int j = 0; j < 3; ++j) out[3 * i + j] = finish(h[(size_t)kGreekSlots * i + j]);
+    return MCDX_OK;
+}
+
+int mcdx_book_pathwise(mcdx_book* b, mcdx_result* out) {
+    if (!b || !out || !(b->flags & MCDX_BOOK_GREEKS)) return MCDX_EINVAL;
+    std::vector<Welford> h;
+    const int st = book_greek_results(b, h);
+    if (st) return st;
+    for (uint32_t i = 0; i < b->n; ++i) {
+        const bool has = pathwise_kind(b->contracts[i]) != kPwNone;
+        for (int j = 0; j < 2; ++j)
+            out[2 * i + j] = has ? finish(h[(size_t)kGreekSlots * i + 3 + j])
+                                 : mcdx_result{NAN, NAN, 0};
+    }
     return MCDX_OK;
 }
 

@@ -150,6 +150,7 @@ SIGNATURES = {
     "mcdx_book_get_schedule": (C.c_int, [C.c_void_p, P(u32), P(u32), P(u32)]),
     "mcdx_book_enqueue": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mcdx_book_results": (C.c_int, [C.c_void_p, P(Result)]),
+    "mcdx_book_pathwise": (C.c_int, [C.c_void_p, P(Result)]),
     "mcdx_book_price": (C.c_int, [C.c_void_p, P(Result)]),
     "mcdx_book_destroy": (None, [C.c_void_p]),
 }
@@ -406,6 +407,14 @@ class Book:
         check(lib().mcdx_book_greeks(self.h, out), "mcdx_book_greeks")
         r = [(x.price, x.standard_error, x.path_count) for x in out]
         return [tuple(r[3 * i:3 * i + 3]) for i in range(self.n)]
+
+    def pathwise(self):
+        """[(delta, vega)] pathwise per contract, each (value, se, paths); NaN where the
+        reference has no pathwise estimator."""
+        out = (Result * (2 * self.n))()
+        check(lib().mcdx_book_pathwise(self.h, out), "mcdx_book_pathwise")
+        r = [(x.price, x.standard_error, x.path_count) for x in out]
+        return [tuple(r[2 * i:2 * i + 2]) for i in range(self.n)]
 
     def price(self):
         out = (Result * self.n)()

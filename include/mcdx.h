@@ -368,6 +368,11 @@ int mcdx_book_price(mcdx_book* book, mcdx_result* out);
 /* Books created with MCDX_BOOK_GREEKS: out[3 i + {0, 1, 2}] = delta, gamma, vega of contract
  * i, each the mean of its per-path central difference with that mean's standard error. */
 int mcdx_book_greeks(mcdx_book* book, mcdx_result* out);
+/* Books created with MCDX_BOOK_GREEKS: out[2 i + {0, 1}] = pathwise delta and vega of contract
+ * i on the same paths (pathwise_european / pathwise_asian, src/greeks/pathwise.cpp:28-147,
+ * with the book's FP32 draws); {NaN, NaN, 0} for contracts the reference has no pathwise
+ * estimator for (digital, barrier, lookback, control-variate Asian). */
+int mcdx_book_pathwise(mcdx_book* book, mcdx_result* out);
 void mcdx_book_destroy(mcdx_book* book);
 
 #ifdef __cplusplus

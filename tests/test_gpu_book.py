@@ -166,6 +166,67 @@ def test_book_greeks_european_match_black_scholes():
     book.close()
 
 
+def pathwise_book():
+    E, A = mcdx.EUROPEAN, mcdx.ASIAN
+    kw = dict(rate=0.05, carry_yield=0.01, vol=0.25, time=1.0, path_count=40_000, seed=21)
+    cs = [mcdx.contract(E, strike=95.0, type=0, **kw),
+          mcdx.contract(E, strike=105.0, type=1, **kw)]
+    for ss, avs, typ in [(0, 0, 0), (0, 1, 1), (1, 0, 1), (1, 1, 0)]:
+        cs.append(mcdx.contract(A, strike=98.0, type=typ, strike_style=ss, average_style=avs,
+                                monitoring_points=19, **kw))
+    cs.append(mcdx.contract(mcdx.DIGITAL, **kw))
+    cs.append(mcdx.contract(mcdx.BARRIER, barrier=130.0, monitoring_points=12, **kw))
+    cs.append(mcdx.contract(A, monitoring_points=12, control_variate=1, **kw))
+    return cs
+
+
+def test_book_pathwise_matches_per_contract_pathwise():
+    # The book's pathwise delta and vega (FP32 draws on the book's paths) against the
+    # per-contract pathwise entry points (the reference's FP64 arithmetic on the same Philox
+    # draws, pinned to the oracle in test_gpu_greeks.py): the same estimator on the same
+    # paths, so they agree far inside one standard error.
+    cs = pathwise_book()
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    book.price()
+    pw = book.pathwise()
+    for c, (d, v) in zip(cs, pw):
+        if c.product == mcdx.EUROPEAN:
+            ref = mcdx.greeks("mcdx_pathwise_european", c.spot, c.strike, c.rate, c.carry_yield,
+                              c.vol, c.time, c.type, c.path_count, c.seed, n_out=3)
+        elif c.product == mcdx.ASIAN and not c.control_variate:
+            ref = mcdx.greeks("mcdx_pathwise_asian", c.spot, c.strike, c.rate, c.carry_yield,
+                              c.vol, c.time, c.type, c.strike_style, c.average_style,
+                              c.monitoring_points, c.path_count, c.seed, n_out=3)
+        else:
+            assert np.isnan(d[0]) and np.isnan(v[0]) and d[2] == v[2] == 0
+            continue
+        for got, want in ((d, ref[0]), (v, ref[1])):
+            assert got[2] == c.path_count
+            assert abs(got[0] - want[0]) <= 0.02 * want[1] + 1e-9, (c.product, got, want)
+            assert abs(got[1] - want[1]) <= 0.01 * want[1] + 1e-12, (c.product, got, want)
+    book.close()
+
+
+def test_book_pathwise_european_near_black_scholes_and_fd():
+    import mcd
+    cs = [mcdx.contract(mcdx.EUROPEAN, strike=k, vol=0.25, time=1.0, path_count=1 << 20, seed=5)
+          for k in (90.0, 110.0)]
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    book.price()
+    for c, (d, g, v), (pd, pv) in zip(cs, book.greeks(), book.pathwise()):
+        h = 1e-3
+        f = lambda s=c.spot, vol=c.vol: mcd.black_scholes_merton(s, c.strike, c.rate, 0.0, vol,
+                                                                 1.0, mcd.OptionType.Call)
+        bd = (f(s=c.spot + h) - f(s=c.spot - h)) / (2 * h)
+        bv = (f(vol=c.vol + h) - f(vol=c.vol - h)) / (2 * h)
+        assert abs(pd[0] - bd) <= 4 * pd[1], (pd, bd)
+        assert abs(pv[0] - bv) <= 4 * pv[1], (pv, bv)
+        # pathwise and finite differences on the same paths estimate the same Greeks
+        assert abs(pd[0] - d[0]) <= 4 * np.hypot(pd[1], d[1])
+        assert abs(pv[0] - v[0]) <= 4 * np.hypot(pv[1], v[1])
+    book.close()
+
+
 def test_greeks_book_rejects_unsupported_contracts():
     cs = crn_book(8)[:4]
     cs[0].antithetic = 1
