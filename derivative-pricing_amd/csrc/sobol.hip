@@ -705,9 +705,13 @@ int qmc_path(const mcdx_gbm_contract* c, int precision, mcdx_result* out) {
     return qmc_run(a, precision, ops, slots, out);
 }
 
-// One Sobol index per thread (index p + 1 for path p), 256 threads, bridge values in LDS
-// slots [slot][thread]; each op's spot goes to its date's store column (coalesced across the
-// block). Any index layout works here: the integer is folded from its own Gray code.
+// One Sobol index per thread, 256 threads, bridge values in LDS slots [slot][thread]; each
+// op's spot goes to its date's store column (coalesced across the block). Block b holds the
+// raw indices i = 256 b + t and path p = i - 1 (index 0 has no path), so the Gray-code bits
+// of i split as in the pricing tiles: bits >= 8 are the Gray code of b (their XOR per
+// dimension, with the run's shift, is formed once per block in LDS), bits 6..7 are
+// wave-uniform and bits 0..5 vary by lane. One dimension then costs an LDS read and eight
+// masked XORs instead of a loop over the index's set bits.
 template <bool FP64, bool SHIFTED>
 __global__ __launch_bounds__(256) void qmc_store_kernel(const uint32_t* __restrict__ v,
                                                         const BridgeOp* __restrict__ ops,
@@ -716,21 +720,29 @@ __global__ __launch_bounds__(256) void qmc_store_kernel(const uint32_t* __restri
                                                         double spot, double vol, float vol2,
                                                         uint64_t n, void* store) {
     extern __shared__ unsigned char smem[];
+    __shared__ uint32_t tdir[kSobolDims];
     // grid y = run: its shift words and its store follow the previous run's
     shift += (size_t)blockIdx.y * (size_t)n_ops;
     store = (char*)store + (size_t)blockIdx.y * (size_t)n_ops * n * (FP64 ? 8 : 4);
     const uint32_t t = threadIdx.x;
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + t;
-    if (p >= n) return; // no barrier follows
-    const uint64_t i = p + 1;
-    const uint32_t g = (uint32_t)(i ^ (i >> 1));
+    const uint32_t bg = blockIdx.x ^ (blockIdx.x >> 1); // Gray bits >= 8 of the block's indices
+    for (int32_t d = (int32_t)t; d < n_ops; d += (int32_t)blockDim.x) { // dims 0 .. n_ops - 1
+        uint32_t x = SHIFTED ? shift[d] : 0u;
+        for (uint32_t g = bg; g; g &= g - 1) x ^= v[32 * d + 8 + __builtin_ctz(g)];
+        tdir[d] = x;
+    }
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + t;
+    if (i == 0 || i > n) return; // after the only barrier
+    const uint64_t p = i - 1;
+    const uint32_t gl = (uint32_t)((i ^ (i >> 1)) & 255u); // Gray bits 0..7
+    const SobolLane L = sobol_lane(gl & 63u, __builtin_amdgcn_readfirstlane(gl >> 6));
     if constexpr (FP64) {
         double* w_lds = (double*)smem;
         w_lds[t] = 0.0;
         for (int k = 0; k < n_ops; ++k) {
             const BridgeOp& op = ops[k];
-            uint32_t x = sobol_fold(v + 32 * op.dim, g);
-            if (SHIFTED) x ^= shift[op.dim];
+            const uint32_t x = sobol_x(v + 32 * op.dim, L, tdir[op.dim]);
             const double z = qmc_normal_f64<true, SHIFTED>(x);
             const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
             const double wl = w_lds[sl * 256 + t], wr = w_lds[sr * 256 + t];
@@ -744,8 +756,7 @@ __global__ __launch_bounds__(256) void qmc_store_kernel(const uint32_t* __restri
         const float s0 = (float)spot;
         for (int k = 0; k < n_ops; ++k) {
             const BridgeOp& op = ops[k];
-            uint32_t x = sobol_fold(v + 32 * op.dim, g);
-            if (SHIFTED) x ^= shift[op.dim];
+            const uint32_t x = sobol_x(v + 32 * op.dim, L, tdir[op.dim]);
             const float z = qmc_normal_f32<SHIFTED>(x);
             const uint32_t sl = op.slots & 0xffu, sr = (op.slots >> 8) & 0xffu, so = op.slots >> 16;
             const float wl = w_lds[sl * 256 + t], wr = w_lds[sr * 256 + t];
@@ -856,7 +867,7 @@ int qmc_fill_store(double spot, double rate, double carry, double vol, double ti
     if (e == hipSuccess) e = hipStreamSynchronize(s); // the host vectors end with this call
     if (e != hipSuccess) return status_of(e);
     const double log2e = 1.44269504088896340735992468100189214;
-    const unsigned grid = (unsigned)((n + 255) / 256);
+    const unsigned grid = (unsigned)((n + 1 + 255) / 256); // raw indices 0 .. n
     const float vol2 = (float)(vol * log2e);
     if (fp64) {
         if (shifted)
