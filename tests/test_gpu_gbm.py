@@ -221,3 +221,45 @@ def test_reference_signature_entry_points():
         assert abs(r.price - op) <= 1e-12 * op
     finally:
         L.mcdx_set_precision(FP32)
+
+
+# --------------------------------------------------------------------------- full sizes
+def _reference_fixture():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_baseline.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+FULL = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.2, time=1.0, seed=42)
+
+
+@pytest.mark.parametrize("key,kind,kw,fp64", [
+    ("c1_european", mcdx.EUROPEAN, dict(path_count=1_000_000), True),
+    ("c2_asian_arith_call", mcdx.ASIAN, dict(monitoring_points=252, path_count=1 << 24), True),
+    ("c2_barrier_uo_call", mcdx.BARRIER,
+     dict(barrier=120.0, direction=0, knock=1, monitoring_points=252, path_count=1 << 24), True),
+    ("c2_barrier_uo_call_bb", mcdx.BARRIER,
+     dict(barrier=120.0, direction=0, knock=1, monitoring_points=252, path_count=1 << 24,
+          brownian_bridge=1), False),
+    ("c2_lookback_float_call", mcdx.LOOKBACK,
+     dict(strike_style=1, monitoring_points=252, path_count=1 << 24), True),
+])
+def test_full_size_against_reference_fp64_prices(key, kind, kw, fp64):
+    # The workload sizes themselves (2^24 paths x 252 dates; C1 at 1e6 paths) against the
+    # reference's own FP64 prices (tests/golden/reference_baseline.json, the reference built
+    # from its sources at 8 chunks): FP64 mode within 1e-12 relative (only the merge order
+    # differs), FP32 generation within 0.01 SE, same path counts.
+    ref = _reference_fixture()[key]
+    rp, rse = float.fromhex(ref["price"]), float.fromhex(ref["se"])
+    c = mcdx.contract(kind, **FULL, **kw)
+    p32, se32, n32 = mcdx.price(c, FP32)
+    assert n32 == ref["paths"]
+    assert abs(p32 - rp) <= 0.01 * rse, (key, p32, rp, rse)
+    assert abs(se32 - rse) <= 1e-3 * rse, (key, se32, rse)
+    if fp64:
+        p64, se64, n64 = mcdx.price(c, FP64)
+        assert n64 == ref["paths"]
+        assert abs(p64 - rp) <= 1e-12 * abs(rp), (key, p64, rp)
+        assert abs(se64 - rse) <= 1e-9 * rse, (key, se64, rse)
