@@ -234,3 +234,14 @@ def test_greeks_book_rejects_unsupported_contracts():
     h = C.c_void_p()
     assert mcdx.lib().mcdx_book_create_ex(arr, 4, FP32, 2, C.byref(h)) == mcdx.EINVAL
     assert mcdx.lib().mcdx_book_create_ex(arr, 4, FP64, 2, C.byref(h)) == mcdx.EINVAL
+
+
+def test_c4_full_book_equals_per_contract_calls_bitwise():
+    # The C4 workload itself (4096 contracts, about 1e9 paths, one Monte Carlo seed so that
+    # families share draws): every contract's result equals its own launch bit for bit.
+    cs = mcdx.c4_book(mc_seed=42)
+    book = mcdx.Book(cs, FP32)
+    got = book.price()
+    for c, g in zip(cs, got):
+        assert g == mcdx.price(c, FP32)
+    book.close()
