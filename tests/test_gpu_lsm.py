@@ -1,9 +1,10 @@
 """GPU Longstaff-Schwartz against the oracle (mcdx C ABI).
 
-The device regression is a tall-skinny QR in FP64 (blocked Householder steps per lane, Givens
-merges of the lanes' factors along a fixed tree): the same least-squares solution as the
-reference's sequential Householder sweep (src/core/linalg.cpp:7-71) with different rounding, so coefficients are compared at 1e-7 relative and prices at 0.02 SE (a decision
-within rounding of the exercise boundary may flip). FP32-generation prices are compared to
+The device regression is a tall-skinny QR in FP64 (blocked Householder steps per lane, then
+Householder reflections of the stacked lane and block factors with fixed-order block-wide
+sums): the same least-squares solution as the reference's sequential Householder sweep
+(src/core/linalg.cpp:7-71) with different rounding, so coefficients are compared at 1e-7
+relative and prices at 0.02 SE (a decision within rounding of the exercise boundary may flip). FP32-generation prices are compared to
 FP64 at 0.5 SE; acceptance against CRR follows the reference's tests/lsm_test.cpp:76-98.
 """
 import json
@@ -42,6 +43,24 @@ def test_fp64_matches_oracle(typ, s0, vol, m, n):
     mask = has.astype(bool)
     assert np.allclose(co[mask], oco[mask], rtol=1e-7, atol=1e-9)
     assert abs(p - op) <= 0.02 * ose
+
+
+def test_c5_full_size_fp64_matches_oracle():
+    # The C5 workload itself (2^20 paths x 50 dates, seed 42): FP64 coefficients within 1e-7
+    # relative and the price within 0.02 SE of the oracle's sequential Householder LSM; FP32
+    # generation within 0.5 SE of FP64; both within 3 SE (+ 0.05, the low bias of LSM) of
+    # CRR-4000.
+    args = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50, 1 << 20, 42)
+    p64, se64, co, has, _ = mcdx.lsm(*args, FP64)
+    op, ose, oco, ohas, _ = O.lsm(*args)
+    assert has.tolist() == ohas.tolist()
+    mask = has.astype(bool)
+    assert np.allclose(co[mask], oco[mask], rtol=1e-7, atol=1e-9)
+    assert abs(p64 - op) <= 0.02 * ose, (p64, op, ose)
+    p32, se32, _, _, _ = mcdx.lsm(*args, FP32)
+    assert abs(p32 - p64) <= 0.5 * se64
+    crr = L.orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 4000, 1)
+    assert abs(p32 - crr) < 3 * se32 + 0.05 and abs(p64 - crr) < 3 * se64 + 0.05
 
 
 def test_fp32_close_to_fp64_and_frozen_replay_exact():
