@@ -263,3 +263,20 @@ def test_full_size_against_reference_fp64_prices(key, kind, kw, fp64):
         assert n64 == ref["paths"]
         assert abs(p64 - rp) <= 1e-12 * abs(rp), (key, p64, rp)
         assert abs(se64 - rse) <= 1e-9 * rse, (key, se64, rse)
+
+
+@pytest.mark.parametrize("avs,ss,cv,anti", [(0, 0, 0, 0), (1, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0),
+                                            (0, 0, 1, 0), (0, 0, 0, 1), (1, 0, 0, 1), (0, 0, 1, 1)])
+def test_fp32_asian_tile_kernels_equal_their_path_samples(avs, ss, cv, anti):
+    # The FP32 Asian tile kernels are instantiated per combination of running sums (and
+    # antithetic or not); the per-path sample kernel decides the sums per path. The same
+    # per-path arithmetic must come out of both: the tile price equals the mean of the
+    # sample kernel's per-path values to summation-order rounding.
+    n = 50_000
+    c = mcdx.contract(mcdx.ASIAN, monitoring_points=37, path_count=n, seed=17, average_style=avs,
+                      strike_style=ss, control_variate=cv, antithetic=anti, strike=98.0)
+    p, se, cnt = mcdx.price(c, FP32)
+    x = mcdx.path_samples(c, FP32, 0, n)
+    assert cnt == n
+    assert abs(p - x.mean()) <= 1e-12 * max(1.0, abs(p)), (p, x.mean())
+    assert abs(se - x.std(ddof=1) / math.sqrt(n)) <= 1e-9 * se, (se, x.std(ddof=1))
