@@ -206,10 +206,18 @@ __global__ __launch_bounds__(256) void lsm_paths_kernel(LsmArgs a) {
     }
 }
 
-// cf = intrinsic at maturity (lsm.cpp:76-81).
+// cf = intrinsic at maturity (lsm.cpp:76-81); block 0 of each run also clears the run's
+// policy (no coefficients at any date yet).
 template <bool FP64>
 __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
     select_run(a);
+    if (blockIdx.x == 0) {
+        for (int i = (int)threadIdx.x; i <= a.m; i += (int)blockDim.x) {
+            a.has[i] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a.coeffs[4 * i + j] = 0.0;
+        }
+    }
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_paths) return;
     a.cf[p] = intrinsic_v(store_get<FP64>(a, a.m, p), a.strike, a.type);
@@ -538,17 +546,6 @@ __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
     }
     const Welford t = block_merge_256(w, lds4);
     if (threadIdx.x == 0) a.partials[tile] = t;
-}
-
-// Grid y = run of a batch (runs' policies are consecutive).
-__global__ void lsm_clear_policy_kernel(uint8_t* has, double* coeffs, int m) {
-    has += (size_t)blockIdx.y * (size_t)(m + 1);
-    coeffs += (size_t)blockIdx.y * (size_t)(m + 1) * 4;
-    const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i <= m) {
-        has[i] = 0;
-        for (int j = 0; j < 4; ++j) coeffs[4 * i + j] = 0.0;
-    }
 }
 
 // ---- Streaming exercise by a policy (no path store) --------------------------------------
@@ -992,11 +989,13 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
+    // The tile partials (whose tail holds the merged results), the coefficients and the flags
+    // are adjacent, so one copy brings a single run's results and policy back.
     const size_t o_store = take(store_bytes * runs), o_cf = take(with_store ? n * 8 * runs : 0),
-                 o_coeffs = take((size_t)(m + 1) * 4 * 8 * runs),
-                 o_has = take(((size_t)m + 1) * runs),
                  o_qr = take(with_store ? 2 * (size_t)a.blocks * sizeof(QrState) * runs : 0),
-                 o_part = take((tiles + 1) * runs * sizeof(Welford)), o_extra = take(extra_bytes);
+                 o_part = take((tiles + 1) * runs * sizeof(Welford)),
+                 o_coeffs = take((size_t)(m + 1) * 4 * 8 * runs),
+                 o_has = take(((size_t)m + 1) * runs), o_extra = take(extra_bytes);
     int st = ensure_dev(ctx, off);
     if (st) return st;
     char* base = (char*)ctx->dev;
@@ -1019,8 +1018,6 @@ template <bool FP64>
 hipError_t enqueue_init(const LsmRun& r, hipStream_t s) {
     const unsigned g = (unsigned)r.blocks;
     hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g, r.runs), dim3(256), 0, s, r.a);
-    hipLaunchKernelGGL(lsm_clear_policy_kernel, dim3((r.a.m + 1 + 255) / 256, r.runs), dim3(256),
-                       0, s, r.a.has, r.a.coeffs, r.a.m);
     return hipGetLastError();
 }
 
@@ -1081,9 +1078,12 @@ int lsm_price_src(double spot, double strike, double rate, double carry, double 
                         src.qmc ? qmc_store_scratch_bytes(m, runs) : 0,
                         src.qmc ? &scratch : nullptr, true, runs)))
         return st;
-    const size_t pol_bytes = (size_t)(m + 1) * 4 * 8 + (size_t)(m + 1);
-    const size_t res_bytes = ((size_t)runs * sizeof(Welford) + 63) & ~(size_t)63;
-    if ((st = ensure_pinned(ctx, res_bytes + pol_bytes))) return st;
+    // One run: its merged result, coefficients and flags come back in one copy of the adjacent
+    // regions (lsm_setup); a batch: the runs' results only.
+    const char* res_dev = (const char*)(r.a.partials + r.tiles * runs);
+    const size_t span = runs == 1 ? (size_t)((const char*)(r.a.has + (m + 1)) - res_dev)
+                                  : (size_t)runs * sizeof(Welford);
+    if ((st = ensure_pinned(ctx, span))) return st;
     const bool fp64 = precision == MCDX_PREC_FP64;
     hipError_t e;
     if (src.qmc) {
@@ -1098,23 +1098,17 @@ int lsm_price_src(double spot, double strike, double rate, double carry, double 
     if (e == hipSuccess)
         e = fp64 ? enqueue_backward<true>(r, true, ctx->stream) : enqueue_backward<false>(r, true, ctx->stream);
     char* h = (char*)ctx->pinned;
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h, r.a.partials + r.tiles * runs, (size_t)runs * sizeof(Welford),
-                           hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h + res_bytes, r.a.coeffs, (size_t)(m + 1) * 32, hipMemcpyDeviceToHost,
-                           ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h + res_bytes + (size_t)(m + 1) * 32, r.a.has, (size_t)(m + 1),
-                           hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, res_dev, span, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return status_of(e);
-    const double* hc = (const double*)(h + res_bytes);
-    const uint8_t* hh = (const uint8_t*)(h + res_bytes + (size_t)(m + 1) * 32);
-    if (coeffs_out)
-        for (int i = 0; i < (m + 1) * 4; ++i) coeffs_out[i] = hc[i];
-    if (has_out)
-        for (int i = 0; i <= m; ++i) has_out[i] = hh[i];
+    if (runs == 1) {
+        const double* hc = (const double*)(h + ((const char*)r.a.coeffs - res_dev));
+        const uint8_t* hh = (const uint8_t*)(h + ((const char*)r.a.has - res_dev));
+        if (coeffs_out)
+            for (int i = 0; i < (m + 1) * 4; ++i) coeffs_out[i] = hc[i];
+        if (has_out)
+            for (int i = 0; i <= m; ++i) has_out[i] = hh[i];
+    }
     // inception decision (lsm.cpp:141-148)
     const double inc = intrinsic_v(spot, strike, type);
     for (uint32_t k = 0; k < runs; ++k) {
