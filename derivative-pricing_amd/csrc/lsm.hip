@@ -74,10 +74,13 @@ __device__ __forceinline__ void householder_fold_rows(QrState& st, const double 
 #pragma unroll
         for (int i = kLsmBasis; i < M; ++i) norm2 = __builtin_fma(A[i][j], A[i][j], norm2);
         const double x1 = A[j][j];
-        const double nrm = __builtin_sqrt(norm2);
-        const double alpha = x1 > 0.0 ? -nrm : nrm;
         const bool live = norm2 > 0.0;
-        const double tau = live ? 1.0 / (alpha * (alpha - x1)) : 0.0;
+        // |x| = norm2 / sqrt(norm2) and 1 / (alpha (alpha - x1)) by the hardware reciprocal
+        // square root and reciprocal refined by Newton steps (within an ulp or two of the IEEE
+        // square root and division, at a fraction of their dependent latency).
+        const double nrm = live ? norm2 * fast_rsqrt_f64(norm2) : 0.0;
+        const double alpha = x1 > 0.0 ? -nrm : nrm;
+        const double tau = live ? fast_rcp_f64(alpha * (alpha - x1)) : 0.0;
         const double v0 = x1 - alpha; // v = (v0, A[kLsmBasis.., j])
 #pragma unroll
         for (int k = j + 1; k <= kLsmBasis; ++k) {
@@ -364,9 +367,9 @@ __device__ __forceinline__ void stacked_reflect(double (&A)[kLsmBasis][kLsmBasis
     const double x1 = pub[0];
     const double norm2 = __builtin_fma(x1, x1, v[0]);
     const bool live = norm2 > 0.0;
-    const double nrm = __builtin_sqrt(norm2);
+    const double nrm = live ? norm2 * fast_rsqrt_f64(norm2) : 0.0; // as householder_fold_rows
     const double alpha = x1 > 0.0 ? -nrm : nrm;
-    const double tau = live ? 1.0 / (alpha * (alpha - x1)) : 0.0;
+    const double tau = live ? fast_rcp_f64(alpha * (alpha - x1)) : 0.0;
     const double v0 = x1 - alpha;
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
