@@ -104,12 +104,17 @@ __device__ __forceinline__ void householder_fold_rows(QrState& st, const double 
     }
 }
 
+// The reference's basis; x^3 / 6 is formed as x^3 * (1/6) (within an ulp of the division,
+// without its dependent FP64 division sequence). Every kernel that fits or replays a policy
+// evaluates this same function, so fits and replays agree bit for bit.
 __host__ __device__ __forceinline__ void laguerre4(double x, double* out) { // lsm.cpp:21-25
+    constexpr double kSixth = 1.0 / 6.0;
     const double e = exp(-x / 2.0);
+    const double x2 = x * x;
     out[0] = e;
     out[1] = e * (1.0 - x);
-    out[2] = e * (1.0 - 2.0 * x + 0.5 * x * x);
-    out[3] = e * (1.0 - 3.0 * x + 1.5 * x * x - x * x * x / 6.0);
+    out[2] = e * (1.0 - 2.0 * x + 0.5 * x2);
+    out[3] = e * (1.0 - 3.0 * x + 1.5 * x2 - x2 * x * kSixth);
 }
 
 __host__ __device__ __forceinline__ double intrinsic_v(double s, double k, int type) {
