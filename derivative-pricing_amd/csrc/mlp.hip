@@ -280,11 +280,10 @@ __device__ __forceinline__ void scatter_param(MlpLds& L, int i, float v) {
 // Single workgroup of 128 lanes (2 waves x 64 samples) running every Adam step in one
 // launch. Each lane keeps its 35 parameters and their Adam moments in registers; the two
 // waves' backward passes run concurrently into separate gradient arrays.
-static_assert(sizeof(MlpLds) + 2 * kParams * sizeof(float) <= 160 * 1024, "LDS budget");
+static_assert(sizeof(MlpLds) <= 160 * 1024, "LDS budget");
 
 struct TrainLds {
     MlpLds net;
-    float adam_m[kParams], adam_v[kParams];
 };
 
 __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
@@ -296,15 +295,13 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
     MlpLds::Wave& W = L.wave[wave];
     const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
     float bc1 = 1.f, bc2 = 1.f, lr = a.lr;
-    float wr[kPer];
+    float wr[kPer], am[kPer], av[kPer]; // parameters and Adam moments, lane-owned
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int i = tid + kTrainLanes * k;
         wr[k] = i < kParams ? a.w[i] : 0.f;
-        if (i < kParams) {
-            T.adam_m[i] = 0.f;
-            T.adam_v[i] = 0.f;
-        }
+        am[k] = 0.f;
+        av[k] = 0.f;
     }
     for (int step = 0; step < a.steps; ++step) {
 #pragma unroll
@@ -352,17 +349,18 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         // Adam (FP32)
         bc1 *= b1c;
         bc2 *= b2c;
+        // bias corrections once per step; the per-parameter square root and reciprocal at the
+        // hardware rate (within an ulp or two of sqrtf and the IEEE division)
+        const float c1 = 1.f / (1.f - bc1), c2 = 1.f / (1.f - bc2);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int i = tid + kTrainLanes * k;
             if (i < kParams) {
                 const float g = L.grad[0][i] + L.grad[1][i];
-                const float m = b1c * T.adam_m[i] + (1.f - b1c) * g;
-                const float v = b2c * T.adam_v[i] + (1.f - b2c) * g * g;
-                T.adam_m[i] = m; // lane-owned slots: no other lane touches them
-                T.adam_v[i] = v;
-                const float mh = m / (1.f - bc1), vh = v / (1.f - bc2);
-                wr[k] -= lr * mh / (sqrtf(vh) + eps);
+                am[k] = b1c * am[k] + (1.f - b1c) * g;
+                av[k] = b2c * av[k] + (1.f - b2c) * g * g;
+                const float mh = am[k] * c1, vh = av[k] * c2;
+                wr[k] -= lr * mh * fast_rcp(fast_sqrt(vh) + eps);
             }
         }
         lr *= a.lr_decay;
