@@ -1527,9 +1527,14 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     float* w = (float*)ex;
     const PolicyBufs pb = carve_policy(ex + align256((size_t)kMlpParams * 4), m, grid, tiles);
     const size_t tbl = (size_t)(m - 1) * grid;
+    // Host staging: results, table, then the policy laid out as on the device (coefficients
+    // and flags adjacent, lsm_setup), so one copy uploads both; then the weights. The result
+    // and table areas do not overlap the uploads' sources, so results can come back without
+    // waiting for the uploads first.
+    const size_t pol_span = (size_t)((const char*)(r.a.has + (m + 1)) - (const char*)r.a.coeffs);
     const size_t h_tbl = align256(sizeof(PolicyHost)), h_coef = h_tbl + align256(tbl * 4),
-                 h_has = h_coef + align256((size_t)(m + 1) * 32),
-                 h_w = h_has + align256((size_t)m + 1), h_end = h_w + align256((size_t)kMlpParams * 4);
+                 h_has = h_coef + (size_t)((const char*)r.a.has - (const char*)r.a.coeffs),
+                 h_w = h_coef + align256(pol_span), h_end = h_w + align256((size_t)kMlpParams * 4);
     if ((st = ensure_pinned(ctx, h_end))) return st;
     hipStream_t s = ctx->stream;
     hipError_t e = hipStreamSynchronize(s); // the pinned staging is free
@@ -1538,15 +1543,13 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     for (int i = 0; i < (m + 1) * 4; ++i) ((double*)(h + h_coef))[i] = coeffs[i];
     for (int i = 0; i <= m; ++i) ((uint8_t*)(h + h_has))[i] = has[i] ? 1 : 0;
     for (int i = 0; i < kMlpParams; ++i) ((float*)(h + h_w))[i] = weights[i];
-    e = hipMemcpyAsync(r.a.coeffs, h + h_coef, (size_t)(m + 1) * 32, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(r.a.has, h + h_has, (size_t)m + 1, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(r.a.coeffs, h + h_coef, pol_span, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
     if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, judge != 0, s)))
         return st;
-    e = hipStreamSynchronize(s); // uploads consumed before the staging is reused
     PolicyHost* ph = (PolicyHost*)h;
-    if (e == hipSuccess) e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&ph->qr, pb.pq + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
     // The table comes back only for the judge's boundary comparison: serving without the
