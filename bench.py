@@ -46,6 +46,12 @@ OPS_OBSERVE = {"asian": 3, "barrier_uo": 2, "lookback_float": 2}
 # reading of a plain 1 GiB device copy (FETCH 454.8, 224.1, 223.0 KiB; WRITE 768 KiB).
 TRAFFIC_BYTES = {"asian": 1717862, "barrier_uo": 1245440, "lookback_float": 1243136}
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FMA)
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector peak
+
+# C1 (European call, FP64 throughout) per path, the same counting rule: one normal draw =
+# Philox word 0 (80 integer ops) + uniform map 2 + Acklam central rational 24 + Halley step
+# 30 (erfc 20, exp 4, 6 others) = 136, plus the terminal spot, payoff and Welford update 8.
+OPS_C1_FP64_PER_PATH = 144
 
 
 def contracts(mcdx, n):
@@ -179,23 +185,35 @@ def extras(torch, mcdx, L, stream, handle, ref):
     fh = float.fromhex
     dev = torch.zeros(3, dtype=torch.float64, device="cuda")
 
-    def enq(c):
-        mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
+    def enq(c, prec=mcdx.PREC_FP32):
+        mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), prec, handle,
                                             C.c_void_p(dev.data_ptr())), "enqueue")
 
-    # C1: European call, 1e6 paths, seed 42 (launch-latency scale) and 2^24 steady state.
+    # C1: European call, 1e6 paths, seed 42, FP64 throughout (the configuration's precision;
+    # launch-latency scale), and 2^24 paths for the steady state. FP32 generation beside it.
     bsm = mcd.black_scholes_merton(100, 100, 0.05, 0.0, 0.2, 1.0, mcd.OptionType.Call)
     for n in (1_000_000, 1 << 24):
         c = mcdx.contract(mcdx.EUROPEAN, path_count=n, seed=42)
+        enq(c, mcdx.PREC_FP64)
+        ms64 = timed_events(torch, stream, lambda: enq(c, mcdx.PREC_FP64), reps=10)
         enq(c)
-        ms = timed_events(torch, stream, lambda: enq(c), reps=10)
+        ms32 = timed_events(torch, stream, lambda: enq(c), reps=10)
         p, se, _ = mcdx.price(c, mcdx.PREC_FP32)
         p64, se64, _ = mcdx.price(c, mcdx.PREC_FP64)
-        entry = {"paths_per_s": n / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se,
-                 "price_fp64": p64, "bsm": bsm, "dev_in_se": (p - bsm) / se}
+        tf = OPS_C1_FP64_PER_PATH * n / (ms64 / 1e3) / 1e12
+        entry = {"precision": "fp64", "paths_per_s": n / (ms64 / 1e3), "ms": ms64,
+                 "price_fp64": p64, "se_fp64": se64, "bsm": bsm,
+                 "dev_in_se": (p64 - bsm) / se64, "within_3se_of_bsm": abs(p64 - bsm) < 3 * se64,
+                 "roofline": {"bound": "valu", "ops_per_path": OPS_C1_FP64_PER_PATH,
+                              "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s (FP64)",
+                              "frac": tf / PEAK_FP64_TFLOPS,
+                              "note": "pricing call (tile kernel + merge) timed by HIP events"},
+                 "fp32_generation": {"paths_per_s": n / (ms32 / 1e3), "ms": ms32, "price": p,
+                                     "se": se, "dev_in_se": (p - bsm) / se}}
         if ref and n == 1_000_000:
             rp = fh(ref["c1_european"]["price"])
-            entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
+            entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p64 - rp) / se64)
+            entry["fp32_generation"]["bias_vs_reference_in_se"] = (p - rp) / se
         res[f"c1_european_{n}"] = entry
     progress("C1 done")
     # C2: barrier with the Brownian-bridge correction, 2^24 x 252.

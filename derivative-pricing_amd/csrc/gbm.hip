@@ -178,15 +178,11 @@ int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* c, int precision, void* stre
         return status_of(hipMemcpyAsync(dev_welford, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
     }
     const uint64_t tiles = tiles_for(c->path_count);
-    // Scratch is reused by later calls on this thread; keep it stream-ordered on `s`.
-    if (ctx->dev_bytes < tiles * sizeof(Welford)) {
-        if (stream) {
-            hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) return status_of(e);
-        }
-        if ((st = ensure_dev(ctx, tiles * sizeof(Welford)))) return st;
-    }
-    return status_of(enqueue_contract(*c, precision, (Welford*)ctx->dev, (Welford*)dev_welford, s));
+    void* scratch;
+    if ((st = enqueue_scratch(ctx, s, tiles * sizeof(Welford), &scratch))) return st;
+    hipError_t e = enqueue_contract(*c, precision, (Welford*)scratch, (Welford*)dev_welford, s);
+    if (e != hipSuccess) return status_of(e);
+    return enqueue_scratch_done(ctx, s);
 }
 
 int mcdx_gbm_path_samples(const mcdx_gbm_contract* c, int precision, uint64_t path0, uint64_t n,

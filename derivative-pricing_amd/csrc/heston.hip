@@ -8,8 +8,8 @@
 // The reference simulates every path again for each strike with the same draws, so one
 // pass here yields exactly the per-strike samples of separate calls.
 //
-// FP32 mode: the variance and log-spot state and the branch arithmetic are FP32; the
-// moment matching (m, s^2, psi and the psi <= psi_c switch) is FP64; payoffs and
+// FP32 mode: the draws, the variance and log-spot state and the branch arithmetic are FP32;
+// the moment matching (m, s^2, psi and the psi <= psi_c switch) is FP64; payoffs and
 // accumulation are FP64. FP64 mode repeats heston.cpp's arithmetic operation for operation.
 #include <math.h>
 
@@ -27,7 +27,6 @@ struct HestonArgs {
     double drift, k0, k1, k2, k3, k4, discount, log_s0, spot;
     double sA, sB;                           // s^2 = v*sA + sB
     float f_drift_k0, f_k1, f_k2, f_k3, f_k4; // FP32 log-spot step constants
-    float f_theta, f_e, f_sA, f_sB;           // FP32 moment-matching constants
     uint64_t seed, n_paths, path0;
     int32_t steps, n_strikes;
     uint8_t type;
@@ -39,9 +38,8 @@ struct HestonArgs {
 
 __device__ __forceinline__ double smax_h(double a, double b) { return (a < b) ? b : a; }
 
-// Terminal spot of one path, FP32 generation and moment matching (FP32 relative error a few
-// 1e-7 in m, s^2 and psi, below the FP32 path arithmetic's own; FP64 mode keeps the
-// reference's FP64).
+// Terminal spot of one path: FP32 generation (draws, variance and log-spot state, branch
+// arithmetic) with the QE moment matching (m, s^2, psi) and the psi <= psi_c switch in FP64.
 // phh = path >> 32 (wave-uniform in the tile kernel, where it then stays in scalar registers).
 __device__ double heston_path_f32(const HestonArgs& a, uint64_t path, uint32_t phh) {
     const uint32_t plo = (uint32_t)path;
@@ -52,23 +50,24 @@ __device__ double heston_path_f32(const HestonArgs& a, uint64_t path, uint32_t p
         const uint32_t draw = (uint32_t)step;
         const float zs = inv_normal_f32(philox_w0(plo, phh, draw, 0u, k0, k1));
         const uint32_t ub = philox_w0(plo, phh, draw, 1u, k0, k1);
-        // moment matching (heston.cpp:94-98); m >= theta (1 - e) > 0 keeps m^2 normal
-        const float m = __builtin_fmaf(v - a.f_theta, a.f_e, a.f_theta);
-        const float s2 = __builtin_fmaf(v, a.f_sA, a.f_sB);
-        const float m2 = m * m;
-        const float psi = s2 * fast_rcp(m2);
+        // moment matching in FP64 (heston.cpp:94-98); m >= theta (1 - e) > 0
+        const double vd = (double)v;
+        const double m = __builtin_fma(vd - a.theta, a.e, a.theta);
+        const double s2 = __builtin_fma(vd, a.sA, a.sB);
+        const double m2 = m * m;
+        const double psi = s2 * fast_rcp_f64(m2);
         float vn;
-        if (psi <= 1.5f) { // heston.cpp:99-107
-            const float inv_psi = m2 * fast_rcp(s2);
+        if (psi <= 1.5) { // heston.cpp:99-107
+            const float inv_psi = (float)(m2 * fast_rcp_f64(s2));
             const float two = 2.0f * inv_psi;
             const float b2 = two - 1.0f + fast_sqrt(two) * fast_sqrt(two - 1.0f);
-            const float aa = m * fast_rcp(1.0f + b2);
+            const float aa = (float)m * fast_rcp(1.0f + b2);
             const float zv = inv_normal_f32(ub);
             const float b = fast_sqrt(b2);
             vn = aa * (b + zv) * (b + zv);
         } else { // heston.cpp:108-114
-            const float p = (psi - 1.0f) * fast_rcp(psi + 1.0f);
-            const float beta = (1.0f - p) * fast_rcp(m);
+            const float p = (float)((psi - 1.0) * fast_rcp_f64(psi + 1.0));
+            const float beta = (1.0f - p) * fast_rcp((float)m);
             const float u = __builtin_fmaf((float)ub, 0x1p-32f, 0x1p-33f);
             const float one_minus_u = __builtin_fmaf((float)(~ub), 0x1p-32f, 0x1p-33f);
             // log((1-p)/(1-u)) / beta = ln2 * (log2(1-p) - log2(1-u)) / beta
@@ -194,10 +193,6 @@ int make_args(const mcdx_heston_params* p, uint8_t type, int32_t steps, uint64_t
     a.spot = p->spot;
     a.sA = p->xi * p->xi * a.e / p->kappa * (1.0 - a.e);
     a.sB = p->theta * p->xi * p->xi / (2.0 * p->kappa) * (1.0 - a.e) * (1.0 - a.e);
-    a.f_theta = (float)a.theta;
-    a.f_e = (float)a.e;
-    a.f_sA = (float)a.sA;
-    a.f_sB = (float)a.sB;
     a.f_drift_k0 = (float)(a.drift + a.k0);
     a.f_k1 = (float)a.k1;
     a.f_k2 = (float)a.k2;

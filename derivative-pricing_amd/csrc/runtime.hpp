@@ -20,6 +20,14 @@ struct ThreadCtx {
     size_t dev_bytes = 0;
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // Scratch of the enqueue-only entry points, which launch on the caller's streams: kept
+    // apart from `dev` (used on `stream` by the synchronous entry points) and ordered across
+    // caller streams by `enq_done`, recorded after every use (see enqueue_scratch).
+    void* enq_dev = nullptr;
+    size_t enq_bytes = 0;
+    hipEvent_t enq_done = nullptr;
+    hipStream_t enq_last = nullptr; // stream of the last recorded use
+    bool enq_used = false;
 };
 
 // Returns MCDX_OK or an error status; on success *ctx is ready on the current device.
@@ -27,6 +35,11 @@ int thread_ctx(ThreadCtx** ctx);
 // Ensure the device scratch holds at least `bytes`; contents are not preserved.
 int ensure_dev(ThreadCtx* ctx, size_t bytes);
 int ensure_pinned(ThreadCtx* ctx, size_t bytes);
+// Enqueue scratch of at least `bytes` for work about to be launched on `s`: waits (on the
+// device, through `s`) for the previous use on another stream, or (on the host) before
+// growing the buffer. The caller launches its work on `s`, then calls enqueue_scratch_done.
+int enqueue_scratch(ThreadCtx* ctx, hipStream_t s, size_t bytes, void** out);
+int enqueue_scratch_done(ThreadCtx* ctx, hipStream_t s);
 int status_of(hipError_t e);
 
 // Stream to launch on: the caller's stream if given, else the thread's own.

@@ -87,6 +87,8 @@ SIGNATURES = {
     "mcdx_version": (C.c_char_p, []),
     "mcdx_philox_bits": (C.c_int, [u64, u64, u64, u32, u32, P(u32)]),
     "mcdx_normal_draws": (C.c_int, [u64, u64, u64, u32, C.c_int, P(dbl)]),
+    "mcdx_inv_normal_f32_scan": (C.c_int, [u64, u64, P(dbl), P(u32)]),
+    "mcdx_inv_normal_words": (C.c_int, [P(u32), u64, C.c_int, P(dbl)]),
     "mcdx_monte_carlo_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Options), P(Result)]),
     "mcdx_monte_carlo_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Options),
                                                        P(Result)]),
@@ -194,6 +196,23 @@ def normal_draws(seed, path0, n, draw, precision):
     check(lib().mcdx_normal_draws(seed, path0, n, draw, precision, out.ctypes.data_as(P(dbl))),
           "mcdx_normal_draws")
     return out
+
+
+def inv_normal_words(words, precision):
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    out = np.empty(len(w), dtype=np.float64)
+    check(lib().mcdx_inv_normal_words(w.ctypes.data_as(P(u32)), len(w), precision,
+                                      out.ctypes.data_as(P(dbl))), "mcdx_inv_normal_words")
+    return out
+
+
+def inv_normal_f32_scan(begin=0, count=1 << 32):
+    """(max_err[central, tail], argmax_bits[central, tail]) of the FP32 inverse normal against
+    the FP64 one over Philox words [begin, begin + count)."""
+    err = (dbl * 2)()
+    bits = (u32 * 2)()
+    check(lib().mcdx_inv_normal_f32_scan(begin, count, err, bits), "mcdx_inv_normal_f32_scan")
+    return list(err), list(bits)
 
 
 def price(c, precision=PREC_FP32):

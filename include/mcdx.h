@@ -89,6 +89,17 @@ int mcdx_philox_bits(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw,
 /* out[i] = standard_normal_variate(seed, path0+i, draw) at `precision` (rng.cpp:59-66). */
 int mcdx_normal_draws(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw, int precision,
                       double* out);
+/* out[i] = inverse normal of u = (words[i] + 0.5) / 2^32 at `precision` (the FP64 mode is
+ * inverse_standard_normal_cdf, src/core/normal.cpp:70-82; FP32 is this engine's), i < n. */
+int mcdx_inv_normal_words(const uint32_t* words, uint64_t n, int precision, double* out);
+/* Accuracy scan of the FP32-mode inverse normal (this engine's approximation; no reference
+ * counterpart; the reference's FP64 Acklam + Halley is src/core/normal.cpp:70-82): over the
+ * Philox output words b in [begin, begin + count) (count <= 2^32 - begin), the largest
+ * |z32(b) - z64(b)| / (1 + |z64(b)|) against the FP64 inverse of u = (b + 0.5) / 2^32, with
+ * its word (smallest on ties); index 0 = central-branch words, 1 = tail-branch words
+ * (-1 where the range holds none). */
+int mcdx_inv_normal_f32_scan(uint64_t begin, uint64_t count, double max_err[2],
+                             uint32_t argmax_bits[2]);
 
 /* ---- GBM Monte Carlo pricers (src/pricers/monte_carlo.cpp:234-338) ---- */
 int mcdx_monte_carlo_european(double spot, double strike, double rate, double carry_yield,

@@ -1252,6 +1252,42 @@ double orc_crr_american(double s, double k, double r, double q, double v, double
     return price;
 }
 
+/* A Bermudan on the CRR tree (this engine's extension; no reference counterpart): the
+ * arithmetic of orc_crr_american, with early exercise allowed only at levels that are
+ * multiples of `exercise_every` (inception included). exercise_every = 1 is the American
+ * tree and exercise_every > steps the European tree, operation for operation. It is the
+ * oracle for an LSM with steps / exercise_every exercise dates. */
+double orc_crr_bermudan(double s, double k, double r, double q, double v, double t, int steps,
+                        int type, int exercise_every) {
+    const double dt = t / (double)steps;
+    const double u = exp(v * sqrt(dt));
+    const double d = 1.0 / u;
+    const double g = exp((r - q) * dt);
+    const double pi = (g - d) / (u - d);
+    const double disc = exp(-r * dt);
+    const double phi = sgn(type);
+    double* val = (double*)malloc(sizeof(double) * (size_t)(steps + 1));
+    for (int i = 0; i <= steps; ++i) {
+        const double sp = s * pow(u, (double)i) * pow(d, (double)steps - (double)i);
+        val[i] = smax(phi * (sp - k), 0.0);
+    }
+    for (int level = steps - 1; level >= 0; --level) {
+        const int exercisable = exercise_every > 0 && level % exercise_every == 0;
+        for (int i = 0; i <= level; ++i) {
+            const double cont = disc * (pi * val[i + 1] + (1.0 - pi) * val[i]);
+            if (exercisable) {
+                const double ns = s * pow(u, (double)i) * pow(d, (double)level - (double)i);
+                val[i] = smax(cont, smax(phi * (ns - k), 0.0));
+            } else {
+                val[i] = cont;
+            }
+        }
+    }
+    const double price = val[0];
+    free(val);
+    return price;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Greeks: src/greeks/pathwise.cpp, src/greeks/likelihood_ratio.cpp          */
 /* ------------------------------------------------------------------------ */

@@ -155,6 +155,9 @@ double orc_crr(double s, double k, double r, double q, double v, double t, int s
                double* pi, double* up, double* down);
 double orc_crr_american(double s, double k, double r, double q, double v, double t, int steps,
                         int type);
+/* Bermudan on the CRR tree: exercise at levels that are multiples of exercise_every. */
+double orc_crr_bermudan(double s, double k, double r, double q, double v, double t, int steps,
+                        int type, int exercise_every);
 
 /* ---- Greeks (src/greeks) ---- */
 typedef struct orc_greek {

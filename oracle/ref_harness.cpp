@@ -161,5 +161,56 @@ void ref_lr_european(double s, double k, double r, double q, double v, double t,
                           g.theta->standard_error};
     for (int i = 0; i < 10; ++i) out10[i] = o[i];
 }
+void ref_pathwise_asian(double s, double k, double r, double q, double v, double t, int type,
+                        int sstyle, int astyle, int m, std::uint64_t n, std::uint64_t seed,
+                        double* out6) {
+    const auto g = mcd::greeks::pathwise_asian(
+        s, k, r, q, v, t, ot(type), sstyle == 0 ? StrikeStyle::Fixed : StrikeStyle::Floating,
+        astyle == 0 ? AverageStyle::Arithmetic : AverageStyle::Geometric, m, n, seed);
+    const double o[6] = {g.delta.value, g.delta.standard_error, g.vega.value,
+                         g.vega.standard_error, g.rho.value, g.rho.standard_error};
+    for (int i = 0; i < 6; ++i) out6[i] = o[i];
+}
+void ref_lr_digital(double s, double k, double r, double q, double v, double t, int type,
+                    int style, double cash, std::uint64_t n, std::uint64_t seed, double* out10) {
+    const auto g = mcd::greeks::likelihood_ratio_digital(
+        s, k, r, q, v, t, ot(type),
+        style == 0 ? DigitalStyle::CashOrNothing : DigitalStyle::AssetOrNothing, cash, n, seed);
+    const double o[10] = {g.delta.value, g.delta.standard_error, g.gamma.value,
+                          g.gamma.standard_error, g.vega.value, g.vega.standard_error,
+                          g.rho.value, g.rho.standard_error, g.theta->value,
+                          g.theta->standard_error};
+    for (int i = 0; i < 10; ++i) out10[i] = o[i];
+}
+// out8: delta, gamma, vega, rho (value, SE); returns 1 when theta is absent, as the
+// reference documents for barriers.
+int ref_lr_barrier(double s, double k, double h, double r, double q, double v, double t,
+                   int type, int dir, int knock, double rebate, int m, std::uint64_t n,
+                   std::uint64_t seed, double* out8) {
+    const auto g = mcd::greeks::likelihood_ratio_barrier(
+        s, k, h, r, q, v, t, ot(type), dir == 0 ? BarrierDirection::Up : BarrierDirection::Down,
+        knock == 0 ? BarrierKnock::In : BarrierKnock::Out, rebate, m, n, seed);
+    const double o[8] = {g.delta.value, g.delta.standard_error, g.gamma.value,
+                         g.gamma.standard_error, g.vega.value, g.vega.standard_error,
+                         g.rho.value, g.rho.standard_error};
+    for (int i = 0; i < 8; ++i) out8[i] = o[i];
+    return g.theta.has_value() ? 0 : 1;
+}
+// Policy as the oracle passes it: coeffs[(m+1) x 4], has[m+1] (1 = populated date).
+double ref_reprice_frozen(double s, double k, double r, double q, double v, double t, int type,
+                          int m, std::uint64_t n, std::uint64_t seed, const double* coeffs,
+                          const std::uint8_t* has, std::uint8_t inception) {
+    LsmPolicy policy;
+    policy.coefficients_by_date.resize(static_cast<std::size_t>(m) + 1);
+    for (int d = 0; d <= m; ++d)
+        if (has[d])
+            policy.coefficients_by_date[static_cast<std::size_t>(d)].assign(coeffs + 4 * d,
+                                                                            coeffs + 4 * d + 4);
+    policy.exercise_at_inception = inception != 0;
+    return reprice_against_frozen_policy(s, k, r, q, v, t, ot(type), m, n, seed, policy);
+}
+double ref_black76(double f, double k, double r, double v, double t, int type) {
+    return black76(f, k, r, v, t, ot(type));
+}
 
 } // extern "C"

@@ -120,6 +120,61 @@ def small(R):
     o10 = (C.c_double * 10)()
     R.ref_lr_european(100, 100, .05, 0, .2, 1, 0, 20_000, 42, o10)
     out["lr_european"] = [H(x) for x in o10]
+    # The cases below use the GPU Greeks tests' exact arguments, so the device results can be
+    # compared with the reference's own bits as well as with the oracle.
+    pw = []
+    for ss, avs, typ in [(0, 0, 0), (1, 1, 1), (0, 1, 0)]:
+        R.ref_pathwise_asian(100, 95, .05, .01, .25, 1, typ, ss, avs, 12, 10_007, 9, o6)
+        pw.append({"args": [100, 95, .05, .01, .25, 1, typ, ss, avs, 12, 10_007, 9],
+                   "out": [H(x) for x in o6]})
+    out["pathwise_asian"] = pw
+    lrd = []
+    for a in ([100, 100, .05, 0, .25, 1, 0, 0, 1.0, 20_000, 7],
+              [100, 105, .03, .01, .3, .5, 1, 0, 2.5, 15_001, 3],
+              [100, 95, .05, .02, .2, 1, 0, 1, 1.0, 12_000, 5],
+              [100, 100, .05, 0, .25, 1, 1, 1, 1.0, 9_999, 8]):
+        R.ref_lr_digital(*a, o10)
+        lrd.append({"args": a, "out": [H(x) for x in o10]})
+    out["lr_digital"] = lrd
+    lrb = []
+    o8 = (C.c_double * 8)()
+    for a in ([100, 100, 120, .05, 0, .2, 1, 0, 0, 1, 0.0, 50, 10_000, 11],
+              [100, 100, 85, .05, .01, .25, 1, 1, 1, 0, 1.5, 20, 8_001, 4],
+              [100, 95, 110, .03, 0, .3, .5, 0, 0, 0, 0.0, 12, 6_000, 9],
+              [100, 105, 90, .05, 0, .2, 1, 0, 1, 1, 2.0, 30, 7_000, 13]):
+        no_theta = R.ref_lr_barrier(*a, o8)
+        lrb.append({"args": a, "out": [H(x) for x in o8], "theta_absent": no_theta})
+    out["lr_barrier"] = lrb
+    # Frozen-policy replays of the LSM fixture's policy at the base and bumped parameters
+    # (src/pricers/lsm.cpp:151-213), plus one replay of a put policy fitted with exercise at
+    # inception (deep in the money).
+    g = out["lsm"]
+    s0, k0, r0, q0, v0, t0, ty, m0, n0, sd = g["args"]
+    rep = []
+    for s, v, r in [(s0, v0, r0), (s0 + 1.0, v0, r0), (s0 - 1.0, v0, r0), (s0, v0 + .01, r0),
+                    (s0, v0, r0 + .01)]:
+        p = R.ref_reprice_frozen(s, k0, r, q0, v, t0, ty, m0, n0, sd,
+                                 co.ctypes.data_as(C.POINTER(C.c_double)),
+                                 has.ctypes.data_as(C.POINTER(C.c_uint8)), inc.value)
+        rep.append({"spot": s, "vol": v, "rate": r, "price": H(p)})
+    co2 = np.zeros(11 * 4)
+    has2 = np.zeros(11, np.uint8)
+    inc2 = C.c_uint8()
+    R.ref_lsm(60, 100, .05, 0, .2, 1, 1, 10, 5_000, 3, C.byref(res),
+              co2.ctypes.data_as(C.POINTER(C.c_double)), has2.ctypes.data_as(C.POINTER(C.c_uint8)),
+              C.byref(inc2))
+    p_inc = R.ref_reprice_frozen(61, 100, .05, 0, .2, 1, 1, 10, 5_000, 3,
+                                 co2.ctypes.data_as(C.POINTER(C.c_double)),
+                                 has2.ctypes.data_as(C.POINTER(C.c_uint8)), inc2.value)
+    out["lsm_frozen"] = {"replays": rep,
+                         "inception_case": {"args": [60, 100, .05, 0, .2, 1, 1, 10, 5_000, 3],
+                                            "fit_price": H(res.price), "inception": inc2.value,
+                                            "coeffs": [H(x) for x in co2], "has": has2.tolist(),
+                                            "replay_spot": 61, "replay_price": H(p_inc)}}
+    out["black76"] = [[f, k, r, v, t, ty, H(R.ref_black76(f, k, r, v, t, ty))]
+                      for f, k, r, v, t, ty in [(100, 95, .05, .2, 1, 0), (100, 95, .05, .2, 1, 1),
+                                                (80, 100, .02, .35, 2, 0), (120, 100, 0, .15, .25, 1),
+                                                (100, 100, .05, .2, 1, 0)]]
     return out
 
 

@@ -104,6 +104,7 @@ def lib():
             "orc_digital": (dbl, [dbl] * 6 + [i32, i32, dbl]),
             "orc_crr": (dbl, [dbl] * 6 + [i32, i32, P(dbl), P(dbl), P(dbl)]),
             "orc_crr_american": (dbl, [dbl] * 6 + [i32, i32]),
+            "orc_crr_bermudan": (dbl, [dbl] * 6 + [i32, i32, i32]),
             "orc_pathwise_european": (None, [dbl] * 6 + [i32, u64, u64, P(Greek)]),
             "orc_pathwise_asian": (None, [dbl] * 6 + [i32, i32, i32, i32, u64, u64, P(Greek)]),
             "orc_lr_terminal": (None, [dbl] * 6 + [i32, i32, i32, dbl, u64, u64, P(Greek)]),
@@ -201,6 +202,11 @@ def ref():
             "ref_crr_american": (dbl, [dbl] * 6 + [i32, i32]),
             "ref_pathwise_european": (None, [dbl] * 6 + [i32, u64, u64, P(dbl)]),
             "ref_lr_european": (None, [dbl] * 6 + [i32, u64, u64, P(dbl)]),
+            "ref_pathwise_asian": (None, [dbl] * 6 + [i32, i32, i32, i32, u64, u64, P(dbl)]),
+            "ref_lr_digital": (None, [dbl] * 6 + [i32, i32, dbl, u64, u64, P(dbl)]),
+            "ref_lr_barrier": (i32, [dbl] * 7 + [i32, i32, i32, dbl, i32, u64, u64, P(dbl)]),
+            "ref_reprice_frozen": (dbl, [dbl] * 6 + [i32, i32, u64, u64, P(dbl), P(u8), u8]),
+            "ref_black76": (dbl, [dbl] * 5 + [i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

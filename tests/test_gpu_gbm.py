@@ -57,6 +57,37 @@ def test_normal_draws_fp64_match_oracle():
     assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) <= 1e-14
 
 
+def test_fp32_inverse_normal_exhaustive_over_all_words():
+    """Every one of the 2^32 Philox words through the FP32 inverse normal, against the FP64
+    inverse (the reference's Acklam + Halley, src/core/normal.cpp:70-82) on the same word:
+    the largest |z32 - z64| / (1 + |z64|), central and tail words separately. The FP64 side
+    is first pinned to the oracle at the extreme and branch-edge words, and the scan's worst
+    words are re-evaluated one by one."""
+    L = O.lib()
+    edge = np.array([0, 1, 2, 2073810, 2073811, 104152956, 104152957, (1 << 31) - 1, 1 << 31,
+                     0xFFFFFFFF - 2073811, 0xFFFFFFFF - 2073810, 0xFFFFFFFE, 0xFFFFFFFF],
+                    dtype=np.uint32)
+    want = np.array([L.orc_inverse_normal_cdf(L.orc_uniform_open01(int(b))) for b in edge])
+    z64 = mcdx.inv_normal_words(edge, FP64)
+    assert np.all(np.abs(z64 - want) <= 1e-14 * (1 + np.abs(want))), (z64, want)
+    z32 = mcdx.inv_normal_words(edge, FP32)
+    assert np.all(np.abs(z32 - want) / (1 + np.abs(want)) <= 1.7e-7)
+    assert np.array_equal(np.sign(z32), np.sign(want))
+    err, bits = mcdx.inv_normal_f32_scan(0, 1 << 32)
+    print(f"\nFP32 inverse normal, all 2^32 words: central max {err[0]:.3e} at {bits[0]:#010x}, "
+          f"tail max {err[1]:.3e} at {bits[1]:#010x}")
+    assert 0.0 < err[0] <= 1.7e-7 and 0.0 < err[1] <= 1.7e-7, (err, [hex(b) for b in bits])
+    w = np.array(bits, dtype=np.uint32)
+    a32, a64 = mcdx.inv_normal_words(w, FP32), mcdx.inv_normal_words(w, FP64)
+    assert np.allclose(np.abs(a32 - a64) / (1 + np.abs(a64)), err, rtol=0, atol=0)
+    oz = np.array([L.orc_inverse_normal_cdf(L.orc_uniform_open01(int(b))) for b in w])
+    assert np.all(np.abs(a64 - oz) <= 1e-14 * (1 + np.abs(oz)))
+    # halves of the range give the same maxima as the whole (the reduction's order is fixed)
+    lo, lb = mcdx.inv_normal_f32_scan(0, 1 << 31)
+    hi, hb = mcdx.inv_normal_f32_scan(1 << 31, 1 << 31)
+    assert max(lo[0], hi[0]) == err[0] and max(lo[1], hi[1]) == err[1]
+
+
 def test_normal_draws_fp32_accuracy_and_tails():
     n = 1 << 18
     got = mcdx.normal_draws(7, 0, n, 0, FP32)

@@ -62,6 +62,31 @@ def test_lr_barrier_vs_oracle():
         assert close(got[i][0], g[i].value) and close(got[i][1], g[i].standard_error)
 
 
+def _flat(got):
+    return [v for g in got for v in g[:2]]
+
+
+def test_pathwise_asian_vs_reference_fixture():
+    for case in GOLD["pathwise_asian"]:
+        got = _flat(mcdx.greeks("mcdx_pathwise_asian", *case["args"], n_out=3))
+        want = [float.fromhex(x) for x in case["out"]]
+        assert all(close(a, b) for a, b in zip(got, want)), case["args"]
+
+
+def test_lr_digital_vs_reference_fixture():
+    for case in GOLD["lr_digital"]:
+        got = _flat(mcdx.greeks("mcdx_likelihood_ratio_digital", *case["args"], n_out=5))
+        want = [float.fromhex(x) for x in case["out"]]
+        assert all(close(a, b) for a, b in zip(got, want)), case["args"]
+
+
+def test_lr_barrier_vs_reference_fixture():
+    for case in GOLD["lr_barrier"]:
+        got = _flat(mcdx.greeks("mcdx_likelihood_ratio_barrier", *case["args"], n_out=4))
+        want = [float.fromhex(x) for x in case["out"]]
+        assert all(close(a, b) for a, b in zip(got, want)), case["args"]
+
+
 def test_fd_prices_vs_oracle_pricings():
     bumps = (1.0, 0.002, 0.01, 0.01)
     got = mcdx.greeks("mcdx_fd_european_prices", 100, 100, .05, 0, .2, 1, 0, 30_011, 42, *bumps,

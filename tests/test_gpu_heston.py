@@ -66,7 +66,7 @@ def test_fp32_bias_and_semi_analytic():
     for k, a, b in zip(ks, r32, r64):
         assert abs(a[0] - b[0]) <= 0.25 * b[1], (k, a, b)
         sa = L.orc_heston_semi_analytic(C.byref(h), k, 0)
-        assert abs(a[0] - sa) < 3 * a[1] + 0.01, (k, a, sa)
+        assert abs(a[0] - sa) < 3 * a[1] and abs(b[0] - sa) < 3 * b[1], (k, a, b, sa)
 
 
 def test_lewis_within_3se_via_reference_signature():
@@ -94,8 +94,8 @@ def test_invalid_heston_arguments():
 
 def test_c3_full_size_semi_analytic_and_fp64():
     # The C3 workload itself: 2^22 paths x 252 steps, 9 strikes from one pass. Every strike
-    # within 3 SE (+ 0.01 for the QE discretisation) of the semi-analytic price; FP32
-    # generation within 0.01 SE of the FP64 mode on the same paths.
+    # within 3 SE of the semi-analytic price in both modes; FP32 generation within 0.01 SE of
+    # the FP64 mode on the same paths.
     ks = [80.0 + 5 * i for i in range(9)]
     par = mcdx.heston(**C3)
     r32 = mcdx.heston_strikes(par, ks, 0, 252, 1 << 22, 42, FP32)
@@ -105,4 +105,4 @@ def test_c3_full_size_semi_analytic_and_fp64():
         assert a[2] == b[2] == 1 << 22
         assert abs(a[0] - b[0]) <= 0.01 * b[1], (k, a, b)
         sa = L.orc_heston_semi_analytic(C.byref(h), k, 0)
-        assert abs(a[0] - sa) < 3 * a[1] + 0.01, (k, a, sa)
+        assert abs(a[0] - sa) < 3 * a[1] and abs(b[0] - sa) < 3 * b[1], (k, a, b, sa)

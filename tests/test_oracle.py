@@ -409,6 +409,22 @@ def test_crr_american_fixture_and_european_limit():
                L.orc_crr(100, 100, .05, 0, .2, 1, 500, 0, None, None, None)) < 1e-12
 
 
+def test_crr_bermudan_reduces_to_american_and_european_trees():
+    # exercise at every level is the American tree, at none but maturity the European tree,
+    # operation for operation; in between the value lies between the two and rises with the
+    # number of exercise dates (every 200 levels of 10^4 = the 50-date LSM's dates).
+    for ty, (s, k) in [(1, (100, 100)), (1, (90, 100)), (0, (100, 95))]:
+        am = L.orc_crr_american(s, k, .06, .02, .2, 1, 400, ty)
+        eu = L.orc_crr(s, k, .06, .02, .2, 1, 400, ty, None, None, None)
+        assert L.orc_crr_bermudan(s, k, .06, .02, .2, 1, 400, ty, 1) == am
+        assert L.orc_crr_bermudan(s, k, .06, .02, .2, 1, 400, ty, 401) == eu
+        vals = [L.orc_crr_bermudan(s, k, .06, .02, .2, 1, 400, ty, e) for e in (200, 40, 8, 2)]
+        assert eu <= vals[0] <= vals[1] <= vals[2] <= vals[3] <= am
+    b50 = L.orc_crr_bermudan(100, 100, .06, 0, .2, 1, 10_000, 1, 200)
+    am = L.orc_crr_american(100, 100, .06, 0, .2, 1, 10_000, 1)
+    assert 0.0 < am - b50 < 0.05
+
+
 # ----------------------------------------------------------------------------- Greeks
 def test_pathwise_and_lr_match_reference_fixture_bits():
     g = (O.Greek * 3)()
@@ -419,6 +435,76 @@ def test_pathwise_and_lr_match_reference_fixture_bits():
     L.orc_lr_terminal(100, 100, .05, 0, .2, 1, 0, 0, 0, 1.0, 20_000, 42, g5)
     want = [fh(x) for x in GOLD["lr_european"]]
     assert [v for gg in g5 for v in (gg.value, gg.standard_error)] == want
+
+
+def test_pathwise_asian_matches_reference_fixture_bits():
+    # src/greeks/pathwise.cpp:64-147 through the reference harness (fixed, floating,
+    # geometric; call and put).
+    for case in GOLD["pathwise_asian"]:
+        a = case["args"]
+        g = (O.Greek * 3)()
+        L.orc_pathwise_asian(*a, g)
+        assert [v for gg in g for v in (gg.value, gg.standard_error)] == \
+            [fh(x) for x in case["out"]], a
+
+
+def test_lr_digital_matches_reference_fixture_bits():
+    # src/greeks/likelihood_ratio.cpp (digital): cash- and asset-or-nothing, call and put.
+    for case in GOLD["lr_digital"]:
+        s, k, r, q, v, t, typ, style, cash, n, seed = case["args"]
+        g = (O.Greek * 5)()
+        L.orc_lr_terminal(s, k, r, q, v, t, typ, 1, style, cash, n, seed, g)
+        assert [v for gg in g for v in (gg.value, gg.standard_error)] == \
+            [fh(x) for x in case["out"]], case["args"]
+
+
+def test_lr_barrier_matches_reference_fixture_bits():
+    # src/greeks/likelihood_ratio.cpp (barrier): up/down, in/out, with and without rebate;
+    # the reference leaves theta unset for barriers.
+    for case in GOLD["lr_barrier"]:
+        g = (O.Greek * 4)()
+        L.orc_lr_barrier(*case["args"], g)
+        assert [v for gg in g for v in (gg.value, gg.standard_error)] == \
+            [fh(x) for x in case["out"]], case["args"]
+        assert case["theta_absent"] == 1
+
+
+def test_frozen_policy_replay_matches_reference_fixture_bits():
+    # src/pricers/lsm.cpp:151-213: the LSM fixture's policy replayed at the base and bumped
+    # spot, vol and rate; the base replay also equals the fitted price (same paths, same
+    # decisions).
+    g = GOLD["lsm"]
+    s0, k0, r0, q0, v0, t0, ty, m0, n0, sd = g["args"]
+    co = np.array([fh(x) for x in g["coeffs"]])
+    has = np.array(g["has"], np.uint8)
+    cp = co.ctypes.data_as(C.POINTER(C.c_double))
+    hp = has.ctypes.data_as(C.POINTER(C.c_uint8))
+    for rep in GOLD["lsm_frozen"]["replays"]:
+        got = L.orc_lsm_reprice_frozen(rep["spot"], k0, rep["rate"], q0, rep["vol"], t0, ty, m0,
+                                       n0, sd, cp, hp, g["inception"])
+        assert got == fh(rep["price"]), rep
+    assert fh(GOLD["lsm_frozen"]["replays"][0]["price"]) == fh(g["price"])
+    # A policy that exercises at inception replays as the intrinsic value at the new spot.
+    ic = GOLD["lsm_frozen"]["inception_case"]
+    co2 = np.array([fh(x) for x in ic["coeffs"]])
+    has2 = np.array(ic["has"], np.uint8)
+    p, se, co_o, has_o, inc_o = O.lsm(*ic["args"])
+    assert inc_o == ic["inception"] == 1 and p == fh(ic["fit_price"])
+    assert all(x == y for x, y in zip(co_o.ravel(), co2)) and has_o.tolist() == ic["has"]
+    a = ic["args"]
+    got = L.orc_lsm_reprice_frozen(ic["replay_spot"], *a[1:], co2.ctypes.data_as(C.POINTER(C.c_double)),
+                                   has2.ctypes.data_as(C.POINTER(C.c_uint8)), 1)
+    assert got == fh(ic["replay_price"]) == 100.0 - ic["replay_spot"]
+
+
+def test_black76_matches_reference_fixture_bits_and_bsm():
+    # src/pricers/analytic.cpp black76; also Black-76 on F = S e^{(r-q)T} equals BSM.
+    for f, k, r, v, t, ty, want in GOLD["black76"]:
+        assert L.orc_black76(f, k, r, v, t, ty) == fh(want)
+    for ty in (0, 1):
+        s, k, r, q, v, t = 100, 95, .05, .02, .3, 1.5
+        f = s * math.exp((r - q) * t)
+        assert abs(L.orc_black76(f, k, r, v, t, ty) - L.orc_bsm(s, k, r, q, v, t, ty)) < 1e-12
 
 
 def test_pathwise_delta_near_bsm_delta():
