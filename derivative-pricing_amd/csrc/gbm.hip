@@ -1,8 +1,17 @@
 // Per-contract GBM launches: one tile of 1024 paths per block, then the fixed-order merge.
 // The path arithmetic lives in gbm_paths.hpp (shared with the batched book).
+#include <atomic>
+#include <mutex>
+
 #include "gbm_paths.hpp"
 
 namespace mcdx {
+
+// Tile partials of the allocation-free entry point: one array per device, held until the
+// process ends. Host code never uses the array's name as an address: each device's copy is
+// reached through the pointer hipGetSymbolAddress returned for it (noalloc_scratch).
+constexpr uint64_t kNoAllocMaxTiles = 65536; // 2^26 paths per call, 1.5 MiB
+static __device__ __align__(256) Welford scratch[kNoAllocMaxTiles];
 
 // One tile of 1024 paths per block: lane folds its 4 paths (Welford, stats.hpp:23-29),
 // then the block's pairwise tree writes the tile partial.
@@ -103,6 +112,33 @@ hipError_t enqueue_contract(const mcdx_gbm_contract& c, int precision, Welford* 
     return launch_final_merge(partials, nullptr, tiles, 1, result, s);
 }
 
+constexpr int kNoAllocMaxDevices = 64;
+std::atomic<Welford*> g_scratch_ptr[kNoAllocMaxDevices];
+std::mutex g_scratch_mu;
+
+// The current device's scratch address, looked up once per device with that device current.
+int noalloc_scratch(Welford** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return status_of(e);
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return MCDX_EDEVICE;
+    Welford* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+        if (!p) {
+            void* addr = nullptr;
+            e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return status_of(e);
+            if (!addr) return MCDX_EDEVICE;
+            p = (Welford*)addr;
+            g_scratch_ptr[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return MCDX_OK;
+}
+
 } // namespace
 
 // Shared with the batched book: price one contract synchronously on the thread's stream.
@@ -183,6 +219,24 @@ int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* c, int precision, void* stre
     hipError_t e = enqueue_contract(*c, precision, (Welford*)scratch, (Welford*)dev_welford, s);
     if (e != hipSuccess) return status_of(e);
     return enqueue_scratch_done(ctx, s);
+}
+
+int mcdx_price_gbm_enqueue_noalloc(const mcdx_gbm_contract* c, int precision, void* stream,
+                                   void* dev_welford) {
+    int st = validate(c, precision);
+    if (st) return st;
+    if (!dev_welford) return MCDX_EINVAL;
+    const uint64_t tiles = tiles_for(c->path_count);
+    if (tiles > kNoAllocMaxTiles) return MCDX_EINVAL;
+    Welford* partials;
+    if ((st = noalloc_scratch(&partials))) return st;
+    hipStream_t s = (hipStream_t)stream; // NULL: the device's null stream
+    hipError_t e;
+    if (tiles == 0) // a merge over no tiles writes the empty accumulator {0, 0, 0}
+        e = launch_final_merge(partials, nullptr, 0, 1, (Welford*)dev_welford, s);
+    else
+        e = enqueue_contract(*c, precision, partials, (Welford*)dev_welford, s);
+    return status_of(e);
 }
 
 int mcdx_gbm_path_samples(const mcdx_gbm_contract* c, int precision, uint64_t path0, uint64_t n,

@@ -100,6 +100,8 @@ SIGNATURES = {
                                                         P(Result)]),
     "mcdx_price_gbm": (C.c_int, [P(GbmContract), C.c_int, P(Result)]),
     "mcdx_price_gbm_enqueue": (C.c_int, [P(GbmContract), C.c_int, C.c_void_p, C.c_void_p]),
+    "mcdx_price_gbm_enqueue_noalloc": (C.c_int, [P(GbmContract), C.c_int, C.c_void_p,
+                                                 C.c_void_p]),
     "mcdx_gbm_path_samples": (C.c_int, [P(GbmContract), C.c_int, u64, u64, P(dbl)]),
     "mcdx_sobol_max_dimensions": (u32, []),
     "mcdx_sobol_extended_dimensions": (u32, []),

@@ -131,6 +131,16 @@ int mcdx_price_gbm(const mcdx_gbm_contract* contract, int precision, mcdx_result
  * (uint64_t, double, double) to the 24-byte device buffer `dev_welford`; returns at once. */
 int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* contract, int precision, void* stream,
                            void* dev_welford);
+/* mcdx_price_gbm_enqueue without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); it has
+ * no counterpart in the reference. Same kernels and results; the tile partials live in a
+ * fixed per-device array of the library (1.5 MiB, held until the process ends), so
+ * path_count is at most 2^26 (MCDX_EINVAL beyond). NULL stream = the device's null stream.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (any streams, any
+ * threads) race and must be ordered by the caller. */
+#define MCDX_NOALLOC_MAX_PATHS (1ull << 26)
+int mcdx_price_gbm_enqueue_noalloc(const mcdx_gbm_contract* contract, int precision,
+                                   void* stream, void* dev_welford);
 /* Discounted per-path samples exactly as the pricer accumulates them (antithetic pairs
  * averaged) for paths [path0, path0+n): the per-path parity hook. */
 int mcdx_gbm_path_samples(const mcdx_gbm_contract* contract, int precision, uint64_t path0,
