@@ -10,9 +10,13 @@
 //
 // The reference prices contracts one call at a time (src/pricers/monte_carlo.cpp:234-338);
 // this entry point has no reference counterpart and exists for batched books.
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
+#include <string>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -22,26 +26,32 @@
 
 namespace mcdx {
 
-// A shared-draw group: up to kFuseMax contracts of one kind, seed and step count (path counts
-// descending), evaluated on the same paths from one normal per step. With `greeks` the
-// members are a contract and its four bumped scenarios (spot up/down, vol up/down); the
-// per-path central differences go to three more result slots and, for a European or Asian
+// A shared-draw group: contracts of one kind, seed and step count (path counts descending),
+// evaluated on the same paths from one normal per step. A price group holds up to
+// kFusePrice contracts. A Greeks group holds up to kUnitCap units, a unit being a contract and
+// its four bumped scenarios (spot up/down, vol up/down) as five consecutive members; each
+// unit's per-path central differences go to three result slots and, for a European or Asian
 // contract (no control variate), the pathwise delta and vega of the unbumped member
-// (pathwise.cpp:28-147) to two more.
-constexpr int kFuseMax = 8;
+// (pathwise.cpp:28-147) to two more. How many units share one group's draws (the fusion
+// width) is a schedule parameter: results do not depend on it.
+constexpr int kFusePrice = 8;
+constexpr int kUnitCap = 3;
+constexpr int kUnitMembers = 5;
+constexpr int kFuseCap = kUnitCap * kUnitMembers; // 15 >= kFusePrice
 constexpr int kGreekSlots = 5; // FD delta, gamma, vega; pathwise delta, vega
 
 enum PathwiseKind : int32_t { kPwNone = 0, kPwTerminal = 1, kPwAverage = 2 };
 
 struct FusedGroup {
-    uint32_t c[kFuseMax];
+    uint32_t c[kFuseCap];
     uint32_t k, m;
-    uint32_t greeks, gslot;
-    double inv_2hs, inv_hs2, inv_2hv;
+    uint32_t units;              // Greek units (0: a price group)
     int32_t kind;
-    int32_t pw;                  // PathwiseKind of member 0
-    double pw_sqrt_t, pw_vol_t;  // European: sqrt(T), vol T
-    float pw_sdt, pw_vdt;        // Asian: sqrt(dt), vol dt
+    uint32_t gslot[kUnitCap];    // each unit's first Greek slot
+    double inv_2hs[kUnitCap], inv_hs2[kUnitCap], inv_2hv[kUnitCap];
+    int32_t pw[kUnitCap];        // PathwiseKind of each unit's base contract
+    double pw_sqrt_t[kUnitCap], pw_vol_t[kUnitCap]; // European: sqrt(T), vol T
+    float pw_sdt[kUnitCap], pw_vdt[kUnitCap];       // Asian: sqrt(dt), vol dt
 };
 
 // Member 0's pathwise estimator: the reference has one for European calls and puts
@@ -68,7 +78,7 @@ struct mcdx_book {
     std::vector<uint32_t> greek_slot;           // per user contract (greeks books)
     uint64_t n_items = 0;                       // plain items, then the group items of each
     uint64_t n_fused[4] = {0, 0, 0, 0};         // family (terminal, Asian, barrier, lookback)
-    int max_fused_blocks_per_cu[4] = {1, 1, 1, 1};
+    int max_fused_blocks_per_cu[4] = {1, 1, 1, 1}; // of the kernel the current width uses
     // device
     mcdx::GbmArgs* d_args = nullptr;
     uint8_t* d_kind = nullptr;                  // kind | (antithetic << 4)
@@ -80,6 +90,9 @@ struct mcdx_book {
     // schedule
     uint32_t blocks_per_cu = 0;                 // 0 = occupancy maximum
     uint32_t order = 1;                         // 0 natural, 1 longest tiles first
+    uint32_t width = 2;                         // Greek units per shared-draw group (1..3)
+    uint32_t tiles_per_item = 1;                // consecutive tiles one block takes per item
+    std::vector<double> hs, hv;                 // Greeks books: bump sizes per contract
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -110,60 +123,71 @@ __device__ __forceinline__ void welford_add_k(Welford& w, double x, int k) {
     w.m2 += d * (x - w.mean);
 }
 
-// One tile of a shared-draw group. Each member's state follows path_f32's arithmetic step for
-// step (the same normals, the same log2-space update, observe and settle functions), so each
-// member's tile partial equals the one its own launch writes.
-template <int KIND>
+// One tile of a shared-draw group with at most M members (U = 0: a price group) or U Greek
+// units (M = 5U). Each member's state follows path_f32's arithmetic step for step (the same
+// normals, the same log2-space update, observe and settle functions), so each member's tile
+// partial equals the one its own launch writes. In a Greeks group only the unit's base
+// contract keeps a price accumulator (the bumped scenarios enter through the differences).
+template <int KIND, int M, int U>
 __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, const FusedGroup& g,
                                            uint64_t tile, const uint64_t* __restrict__ toff,
                                            Welford* __restrict__ partials, Welford* lds4) {
 #pragma clang fp contract(fast)
-    const GbmArgs* A[kFuseMax];
+    static_assert(M <= kFuseCap && (U == 0 || M == kUnitMembers * U), "group shape");
+    constexpr int NW = U == 0 ? M : U; // price accumulators
+    const GbmArgs* A[M];
     int kact = 0;
 #pragma unroll
-    for (int i = 0; i < kFuseMax; ++i) {
+    for (int i = 0; i < M; ++i) {
         A[i] = args + g.c[(uint32_t)i < g.k ? i : 0];
         if ((uint32_t)i < g.k && A[i]->n_paths > tile * kTilePaths) kact = i + 1;
     }
     const uint32_t k0 = (uint32_t)A[0]->seed, k1 = (uint32_t)(A[0]->seed >> 32);
-    Welford w[kFuseMax], gw[kGreekSlots];
+    Welford w[NW], gw[U > 0 ? U : 1][kGreekSlots];
 #pragma unroll
-    for (int i = 0; i < kFuseMax; ++i) w[i] = Welford{0, 0.0, 0.0};
+    for (int i = 0; i < NW; ++i) w[i] = Welford{0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < kGreekSlots; ++j) gw[j] = Welford{0, 0.0, 0.0};
+    for (int u = 0; u < (U > 0 ? U : 1); ++u)
+#pragma unroll
+        for (int j = 0; j < kGreekSlots; ++j) gw[u][j] = Welford{0, 0.0, 0.0};
 #pragma unroll 1
     for (int k = 0; k < kPathsPerLane; ++k) {
         // base is a multiple of 256 and threadIdx.x < 256: every lane's path >> 32 is base >> 32.
         const uint64_t base = tile * kTilePaths + (uint64_t)(k * kTileLanes);
         const uint64_t path = base + threadIdx.x;
         const uint32_t plo = (uint32_t)path, phh = (uint32_t)(base >> 32);
-        double v[kFuseMax];
-        double pwd = 0.0, pwv = 0.0; // member 0's pathwise delta and vega samples
+        double v[M];
+        double pwd[U > 0 ? U : 1], pwv[U > 0 ? U : 1]; // each unit's pathwise samples
+#pragma unroll
+        for (int u = 0; u < (U > 0 ? U : 1); ++u) pwd[u] = pwv[u] = 0.0;
         if constexpr (KIND == kTerminal) {
             const float z = inv_normal_f32(philox_w0(plo, phh, 0u, 0u, k0, k1));
 #pragma unroll
-            for (int i = 0; i < kFuseMax; ++i)
+            for (int i = 0; i < M; ++i)
                 v[i] = i < kact ? A[i]->discount *
                                       terminal_payoff(*A[i], A[i]->spot * (double)fast_exp2(
                                                                  fmaf(A[i]->sig2, z, A[i]->mu2)))
                                 : 0.0;
-            if (g.pw == kPwTerminal) { // pathwise.cpp:38-58 on this path's FP32 draw
-                const GbmArgs& a0 = *A[0];
-                const double st = a0.spot * (double)fast_exp2(fmaf(a0.sig2, z, a0.mu2));
-                const double phi = a0.type == 0 ? 1.0 : -1.0;
-                const double slope = phi * (st - a0.strike) > 0.0 ? phi : 0.0;
-                pwd = a0.discount * slope * (st / a0.spot);
-                pwv = a0.discount * slope * (st * (g.pw_sqrt_t * (double)z - g.pw_vol_t));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (g.pw[u] == kPwTerminal) { // pathwise.cpp:38-58 on this path's FP32 draw
+                    const GbmArgs& a0 = *A[kUnitMembers * u];
+                    const double st = a0.spot * (double)fast_exp2(fmaf(a0.sig2, z, a0.mu2));
+                    const double phi = a0.type == 0 ? 1.0 : -1.0;
+                    const double slope = phi * (st - a0.strike) > 0.0 ? phi : 0.0;
+                    pwd[u] = a0.discount * slope * (st / a0.spot);
+                    pwv[u] = a0.discount * slope * (st * (g.pw_sqrt_t[u] * (double)z - g.pw_vol_t[u]));
+                }
             }
         } else {
-            StateF32 s[kFuseMax];
+            StateF32 s[M];
             // Step constants, read once per item for every member (A[i] is always a valid
             // contract), so the step loop runs on registers: observe_f32's arithmetic with
             // the member's own constants.
-            bool na[kFuseMax], ng[kFuseMax];
-            float mu2[kFuseMax], sig2[kFuseMax];
+            bool na[M], ng[M];
+            float mu2[M], sig2[M];
 #pragma unroll
-            for (int i = 0; i < kFuseMax; ++i) {
+            for (int i = 0; i < M; ++i) {
                 s[i] = StateF32{0.f, 0.f, 0.f, 1.f, 0.0, 0.0, false};
                 if constexpr (KIND == kBarrier) { // extrema over the monitoring dates only
                     s[i].xmax = -__builtin_inff();
@@ -174,22 +198,28 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                 mu2[i] = A[i]->mu2;
                 sig2[i] = A[i]->sig2;
             }
-            // Member 0's pathwise state (Asian): cumulative normal, step count, d ln S / d vol
-            // at the last date, and its sums over the dates (pathwise.cpp:84-97).
-            const bool pw = KIND == kAsian && g.pw == kPwAverage; // uniform
-            float pw_cz = 0.f, pw_i = 0.f, pw_dlv = 0.f;
-            double pw_ssv = 0.0, pw_sdv = 0.0;
+            // Each unit's pathwise state (Asian): the cumulative normal and step count are the
+            // group's (same draws); d ln S / d vol at the last date and its sums over the
+            // dates are the unit's (pathwise.cpp:84-97).
+            float pw_cz = 0.f, pw_i = 0.f;
+            float pw_dlv[U > 0 ? U : 1];
+            double pw_ssv[U > 0 ? U : 1], pw_sdv[U > 0 ? U : 1];
+#pragma unroll
+            for (int u = 0; u < (U > 0 ? U : 1); ++u) {
+                pw_dlv[u] = 0.f;
+                pw_ssv[u] = pw_sdv[u] = 0.0;
+            }
             auto apply = [&](float z) {
 #pragma unroll
-                for (int i = 0; i < kFuseMax; ++i) {
+                for (int i = 0; i < M; ++i) {
                     if (i < kact) {
                         StateF32& t = s[i];
                         t.x = t.x + fmaf(sig2[i], z, mu2[i]);
                         if constexpr (KIND == kAsian) {
                             if (na[i]) {
-                                const float u = fast_exp2(t.x);
-                                t.blk_u += u;
-                                t.last_u = u;
+                                const float uu = fast_exp2(t.x);
+                                t.blk_u += uu;
+                                t.last_u = uu;
                             }
                             if (ng[i]) t.sum_x += (double)t.x;
                         } else if constexpr (KIND == kBarrier || KIND == kLookback) {
@@ -198,13 +228,19 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
                         }
                     }
                 }
-                if constexpr (KIND == kAsian) {
-                    if (pw) { // d ln S_i / d vol = -vol t_i + sqrt(dt) (z_1 + ... + z_i)
-                        pw_cz += z;
-                        pw_i += 1.f;
-                        pw_dlv = fmaf(g.pw_sdt, pw_cz, -g.pw_vdt * pw_i);
-                        if (na[0]) pw_ssv += (double)(s[0].last_u * pw_dlv); // sum (S_i/S0) d ln S_i
-                        else pw_sdv += (double)pw_dlv;
+                if constexpr (KIND == kAsian && U > 0) {
+                    pw_cz += z;
+                    pw_i += 1.f;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (g.pw[u] == kPwAverage && kUnitMembers * u < kact) {
+                            // d ln S_i / d vol = -vol t_i + sqrt(dt) (z_1 + ... + z_i)
+                            pw_dlv[u] = fmaf(g.pw_sdt[u], pw_cz, -g.pw_vdt[u] * pw_i);
+                            if (na[kUnitMembers * u])
+                                pw_ssv[u] += (double)(s[kUnitMembers * u].last_u * pw_dlv[u]);
+                            else
+                                pw_sdv[u] += (double)pw_dlv[u];
+                        }
                     }
                 }
             };
@@ -215,7 +251,7 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             auto flush = [&]() {
                 if constexpr (KIND == kAsian) {
 #pragma unroll
-                    for (int i = 0; i < kFuseMax; ++i)
+                    for (int i = 0; i < M; ++i)
                         if (i < kact && na[i]) flush_f32<kAsian, true>(s[i]);
                 }
             };
@@ -229,74 +265,100 @@ __device__ __forceinline__ void fused_item(const GbmArgs* __restrict__ args, con
             for (; step < m; ++step) apply(inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
             flush();
 #pragma unroll
-            for (int i = 0; i < kFuseMax; ++i) v[i] = i < kact ? settle_f32<KIND>(*A[i], s[i]) : 0.0;
+            for (int i = 0; i < M; ++i) v[i] = i < kact ? settle_f32<KIND>(*A[i], s[i]) : 0.0;
             if constexpr (KIND == kAsian) {
-                if (pw) { // pathwise.cpp:99-143, averages formed as settle_f32 forms them
-                    const GbmArgs& a0 = *A[0];
-                    const bool arith = a0.average_style == 0;
-                    const double mm = (double)a0.m;
-                    const double last = a0.spot * (double)(arith ? s[0].last_u : fast_exp2(s[0].x));
-                    const double avg = arith ? a0.spot * (s[0].sum_u / mm)
-                                             : a0.spot * exp2(s[0].sum_x / mm);
-                    const double davg_s = avg / a0.spot;
-                    const double davg_v = arith ? a0.spot * (pw_ssv / mm) : avg * (pw_sdv / mm);
-                    const double ref = a0.strike_style == 0 ? avg - a0.strike : last - avg;
-                    const double phi = a0.type == 0 ? 1.0 : -1.0;
-                    const double slope = phi * ref > 0.0 ? phi : 0.0;
-                    if (a0.strike_style == 0) {
-                        pwd = a0.discount * (slope * davg_s);
-                        pwv = a0.discount * (slope * davg_v);
-                    } else {
-                        pwd = a0.discount * (slope * (last / a0.spot - davg_s));
-                        pwv = a0.discount * (slope * (last * (double)pw_dlv - davg_v));
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (g.pw[u] == kPwAverage && kUnitMembers * u < kact) {
+                        // pathwise.cpp:99-143, averages formed as settle_f32 forms them
+                        const StateF32& s0 = s[kUnitMembers * u];
+                        const GbmArgs& a0 = *A[kUnitMembers * u];
+                        const bool arith = a0.average_style == 0;
+                        const double mm = (double)a0.m;
+                        const double last = a0.spot * (double)(arith ? s0.last_u : fast_exp2(s0.x));
+                        const double avg = arith ? a0.spot * (s0.sum_u / mm)
+                                                 : a0.spot * exp2(s0.sum_x / mm);
+                        const double davg_s = avg / a0.spot;
+                        const double davg_v = arith ? a0.spot * (pw_ssv[u] / mm) : avg * (pw_sdv[u] / mm);
+                        const double ref = a0.strike_style == 0 ? avg - a0.strike : last - avg;
+                        const double phi = a0.type == 0 ? 1.0 : -1.0;
+                        const double slope = phi * ref > 0.0 ? phi : 0.0;
+                        if (a0.strike_style == 0) {
+                            pwd[u] = a0.discount * (slope * davg_s);
+                            pwv[u] = a0.discount * (slope * davg_v);
+                        } else {
+                            pwd[u] = a0.discount * (slope * (last / a0.spot - davg_s));
+                            pwv[u] = a0.discount * (slope * (last * (double)pw_dlv[u] - davg_v));
+                        }
                     }
                 }
             }
         }
+        if constexpr (U == 0) {
 #pragma unroll
-        for (int i = 0; i < kFuseMax; ++i)
-            if (i < kact && path < A[i]->n_paths) welford_add_k(w[i], v[i], k);
-        if (g.greeks && path < A[0]->n_paths) { // finite_difference.cpp:58-62, path by path
-            welford_add_k(gw[0], (v[1] - v[2]) * g.inv_2hs, k);
-            welford_add_k(gw[1], ((v[1] - 2.0 * v[0]) + v[2]) * g.inv_hs2, k);
-            welford_add_k(gw[2], (v[3] - v[4]) * g.inv_2hv, k);
-            if (g.pw != kPwNone) {
-                welford_add_k(gw[3], pwd, k);
-                welford_add_k(gw[4], pwv, k);
+            for (int i = 0; i < M; ++i)
+                if (i < kact && path < A[i]->n_paths) welford_add_k(w[i], v[i], k);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b0 = kUnitMembers * u;
+                if (b0 < kact && path < A[b0]->n_paths) { // finite_difference.cpp:58-62, path by path
+                    welford_add_k(w[u], v[b0], k);
+                    welford_add_k(gw[u][0], (v[b0 + 1] - v[b0 + 2]) * g.inv_2hs[u], k);
+                    welford_add_k(gw[u][1], ((v[b0 + 1] - 2.0 * v[b0]) + v[b0 + 2]) * g.inv_hs2[u], k);
+                    welford_add_k(gw[u][2], (v[b0 + 3] - v[b0 + 4]) * g.inv_2hv[u], k);
+                    if (g.pw[u] != kPwNone) {
+                        welford_add_k(gw[u][3], pwd[u], k);
+                        welford_add_k(gw[u][4], pwv[u], k);
+                    }
+                }
             }
         }
     }
+    if constexpr (U == 0) {
 #pragma unroll
-    for (int i = 0; i < kFuseMax; ++i) {
-        if (i < kact) { // uniform across the block
-            const Welford t = block_merge_256_all(w[i], lds4);
-            partials[toff[g.c[i]] + tile] = t;
+        for (int i = 0; i < M; ++i) {
+            if (i < kact) { // uniform across the block
+                const Welford t = block_merge_256_all(w[i], lds4);
+                partials[toff[g.c[i]] + tile] = t;
+            }
         }
-    }
-    if (g.greeks) {
+    } else {
 #pragma unroll
-        for (int j = 0; j < kGreekSlots; ++j) {
-            const Welford t = block_merge_256_all(gw[j], lds4);
-            partials[toff[g.gslot + j] + tile] = t;
+        for (int u = 0; u < U; ++u) {
+            if (kUnitMembers * u < kact) { // uniform across the block
+                const Welford t = block_merge_256_all(w[u], lds4);
+                partials[toff[g.c[kUnitMembers * u]] + tile] = t;
+#pragma unroll
+                for (int j = 0; j < kGreekSlots; ++j) {
+                    const Welford x = block_merge_256_all(gw[u][j], lds4);
+                    partials[toff[g.gslot[u] + j] + tile] = x;
+                }
+            }
         }
     }
 }
 
-// Persistent blocks over one payoff family's shared-draw items (group << 40 | tile), round
-// robin like book_kernel; one instantiation per family keeps each at its own register
+// Persistent blocks over one payoff family's shared-draw items, round robin like book_kernel:
+// an item is (group << 40 | first tile) and covers up to tiles_per_item consecutive tiles of
+// the group. One instantiation per family and group shape keeps each at its own register
 // budget. FP32 only.
-template <int KIND>
+template <int KIND, int M, int U>
 __global__ __launch_bounds__(kTileLanes) void book_fused_kernel(const GbmArgs* __restrict__ args,
                                                                 const uint64_t* __restrict__ toff,
                                                                 const uint64_t* __restrict__ items,
                                                                 uint64_t n_items,
                                                                 const FusedGroup* __restrict__ groups,
-                                                                Welford* __restrict__ partials) {
+                                                                Welford* __restrict__ partials,
+                                                                uint32_t tiles_per_item) {
     __shared__ Welford lds4[4];
     for (uint64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const uint64_t packed = items[it];
         const FusedGroup g = groups[packed >> 40];
-        fused_item<KIND>(args, g, packed & ((1ull << 40) - 1), toff, partials, lds4);
+        const uint64_t t0 = packed & ((1ull << 40) - 1);
+        const uint64_t tiles = (args[g.c[0]].n_paths + kTilePaths - 1) / kTilePaths;
+        const uint64_t t1 = t0 + tiles_per_item < tiles ? t0 + tiles_per_item : tiles; // uniform
+        for (uint64_t t = t0; t < t1; ++t) fused_item<KIND, M, U>(args, g, t, toff, partials, lds4);
     }
 }
 
@@ -342,16 +404,99 @@ __global__ __launch_bounds__(kTileLanes) void book_kernel(const GbmArgs* __restr
 
 namespace {
 
+// A book schedule: blocks per CU (0 = occupancy maximum), item order, fusion width (Greek
+// units per shared-draw group) and consecutive tiles per item. None changes any result.
+struct Schedule {
+    bool valid = false;
+    uint32_t blocks_per_cu = 0, order = 1, width = 2, tiles_per_item = 1;
+};
+
+bool schedule_ok(const Schedule& s) {
+    return s.blocks_per_cu <= 64 && s.order <= 1 && s.width >= 1 && s.width <= (uint32_t)kUnitCap &&
+           (s.tiles_per_item == 1 || s.tiles_per_item == 2 || s.tiles_per_item == 4 ||
+            s.tiles_per_item == 8);
+}
+
+// Tuned schedules by book shape: this process's, and those of earlier processes kept in a
+// text file ($MCDX_BOOK_SCHEDULE_CACHE, else ~/.cache/mcdx/book_schedules.txt; one line per
+// tuning, "mcdx-book-schedule 1 <shape key hex> <blocks/CU> <order> <width> <tiles/item>",
+// the last line of a shape wins). Lines that do not parse or hold values out of range are
+// ignored, so a damaged file falls back to the default schedule.
 std::mutex g_tune_mu;
-std::map<uint64_t, std::pair<uint32_t, uint32_t>> g_tuned; // book shape -> (blocks/CU, order)
+std::map<uint64_t, Schedule> g_tuned;
+bool g_cache_read = false;
+std::string g_cache_path; // the file g_tuned was read from
+
+std::string cache_path() {
+    if (const char* p = getenv("MCDX_BOOK_SCHEDULE_CACHE")) return p;
+    if (const char* h = getenv("HOME")) return std::string(h) + "/.cache/mcdx/book_schedules.txt";
+    return std::string();
+}
+
+// (Re)reads the cache file when it has not been read or its path changed since.
+void read_cache_locked() {
+    const std::string path = cache_path();
+    if (g_cache_read && path == g_cache_path) return;
+    g_cache_read = true;
+    g_cache_path = path;
+    g_tuned.clear();
+    if (path.empty()) return;
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        unsigned long long key = 0;
+        unsigned ver = 0, bpc = 0, order = 0, width = 0, tpi = 0;
+        char tail = 0;
+        if (sscanf(line, "mcdx-book-schedule %u %llx %u %u %u %u %c", &ver, &key, &bpc, &order,
+                   &width, &tpi, &tail) != 6 || ver != 1)
+            continue;
+        Schedule sc;
+        sc.blocks_per_cu = bpc;
+        sc.order = order;
+        sc.width = width;
+        sc.tiles_per_item = tpi;
+        sc.valid = schedule_ok(sc);
+        if (sc.valid) g_tuned[(uint64_t)key] = sc;
+    }
+    fclose(f);
+}
+
+Schedule tuned_schedule(uint64_t key) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    read_cache_locked();
+    auto it = g_tuned.find(key);
+    return it == g_tuned.end() ? Schedule{} : it->second;
+}
+
+void remember_schedule(uint64_t key, const Schedule& sc) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    read_cache_locked();
+    g_tuned[key] = sc;
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos && slash > 0) {
+        // create the directory chain (best effort: a missing cache only costs a retune)
+        std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); ++i)
+            if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    if (FILE* f = fopen(path.c_str(), "a")) {
+        fprintf(f, "mcdx-book-schedule 1 %llx %u %u %u %u\n", (unsigned long long)key,
+                sc.blocks_per_cu, sc.order, sc.width, sc.tiles_per_item);
+        fclose(f);
+    }
+}
 
 uint64_t shape_key(const mcdx_book& b) {
-    // product mix, log2 path-count histogram and step histogram (FNV-1a over the counts),
-    // plus the sharing / Greeks flags and the group count
+    // product mix, log2 path-count histogram and step histogram of the user contracts
+    // (FNV-1a over the counts), plus the sharing / Greeks flags and the contract count
     uint64_t hist[5 + 41 + 64 + 2] = {0};
     hist[5 + 41 + 64] = b.flags;
-    hist[5 + 41 + 64 + 1] = b.groups.size();
-    for (const auto& c : b.contracts) {
+    hist[5 + 41 + 64 + 1] = b.n;
+    for (uint32_t i = 0; i < b.n; ++i) {
+        const mcdx_gbm_contract& c = b.contracts[i];
         hist[kind_of(c)]++;
         int lg = 0;
         while ((1ull << (lg + 1)) <= c.path_count && lg < 40) ++lg;
@@ -367,6 +512,110 @@ uint64_t shape_key(const mcdx_book& b) {
     return h;
 }
 
+// The shared-draw kernel of family F for the book's group shape: price groups of up to
+// kFusePrice members, or Greeks groups of `width` units.
+template <int F>
+const void* fused_kernel_for(uint32_t units) {
+    switch (units) {
+    case 0: return (const void*)book_fused_kernel<kFusedKinds[F], kFusePrice, 0>;
+    case 1: return (const void*)book_fused_kernel<kFusedKinds[F], kUnitMembers, 1>;
+    case 2: return (const void*)book_fused_kernel<kFusedKinds[F], 2 * kUnitMembers, 2>;
+    default: return (const void*)book_fused_kernel<kFusedKinds[F], 3 * kUnitMembers, 3>;
+    }
+}
+
+const void* fused_kernel(int family, uint32_t units) {
+    switch (family) {
+    case 0: return fused_kernel_for<0>(units);
+    case 1: return fused_kernel_for<1>(units);
+    case 2: return fused_kernel_for<2>(units);
+    default: return fused_kernel_for<3>(units);
+    }
+}
+
+uint32_t group_units(const mcdx_book& b) {
+    return (b.flags & MCDX_BOOK_GREEKS) ? b.width : 0u;
+}
+
+// Shared-draw groups for the book's fusion width, and the occupancy of the kernels they use.
+hipError_t build_groups(mcdx_book& b) {
+    const uint32_t n = b.n;
+    const bool greeks = (b.flags & MCDX_BOOK_GREEKS) != 0;
+    const bool share = (b.flags & MCDX_BOOK_NO_SHARING) == 0 && b.precision == MCDX_PREC_FP32;
+    b.groups.clear();
+    std::fill(b.group_of.begin(), b.group_of.end(), -1);
+    // key (seed, kind, steps) -> user contracts by path count, descending
+    std::map<std::tuple<uint64_t, int, int>, std::vector<uint32_t>> buckets;
+    for (uint32_t i = 0; i < n; ++i) {
+        const mcdx_gbm_contract& c = b.contracts[i];
+        const int kind = kind_of(c);
+        if (b.precision != MCDX_PREC_FP32 || c.antithetic || kind == kBarrierBB) continue;
+        buckets[{c.seed, kind, kind == kTerminal ? 1 : c.monitoring_points}].push_back(i);
+    }
+    for (auto& kv : buckets) {
+        auto& mem = kv.second;
+        std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
+            return b.contracts[x].path_count > b.contracts[y].path_count;
+        });
+        const int kind = std::get<1>(kv.first);
+        const uint32_t m = (uint32_t)std::get<2>(kv.first);
+        if (greeks) {
+            for (size_t o = 0; o < mem.size(); o += b.width) {
+                FusedGroup g{};
+                g.kind = kind;
+                g.m = m;
+                g.units = (uint32_t)std::min<size_t>(b.width, mem.size() - o);
+                g.k = kUnitMembers * g.units;
+                const int32_t gi = (int32_t)b.groups.size();
+                for (uint32_t u = 0; u < g.units; ++u) {
+                    const uint32_t i = mem[o + u];
+                    const mcdx_gbm_contract& c = b.contracts[i];
+                    g.c[kUnitMembers * u] = i;
+                    for (int j = 0; j < 4; ++j) g.c[kUnitMembers * u + 1 + j] = n + 4 * i + j;
+                    g.gslot[u] = b.greek_slot[i];
+                    g.inv_2hs[u] = 1.0 / (2.0 * b.hs[i]);
+                    g.inv_hs2[u] = 1.0 / (b.hs[i] * b.hs[i]);
+                    g.inv_2hv[u] = 1.0 / (2.0 * b.hv[i]);
+                    g.pw[u] = pathwise_kind(c);
+                    const double dt = kind == kTerminal ? c.time : c.time / (double)c.monitoring_points;
+                    g.pw_sqrt_t[u] = sqrt(c.time);
+                    g.pw_vol_t[u] = c.vol * c.time;
+                    g.pw_sdt[u] = (float)sqrt(dt);
+                    g.pw_vdt[u] = (float)(c.vol * dt);
+                    for (int j = 0; j < kUnitMembers; ++j) b.group_of[g.c[kUnitMembers * u + j]] = gi;
+                    for (int j = 0; j < kGreekSlots; ++j) b.group_of[g.gslot[u] + j] = -2;
+                }
+                b.groups.push_back(g);
+            }
+        } else if (share && mem.size() >= 2) {
+            for (size_t o = 0; o < mem.size(); o += kFusePrice) {
+                FusedGroup g{};
+                g.k = (uint32_t)std::min<size_t>(kFusePrice, mem.size() - o);
+                if (g.k < 2) break; // a lone tail member stays a plain item
+                for (uint32_t j = 0; j < g.k; ++j) g.c[j] = mem[o + j];
+                g.kind = kind;
+                g.m = m;
+                for (uint32_t j = 0; j < g.k; ++j) b.group_of[g.c[j]] = (int32_t)b.groups.size();
+                b.groups.push_back(g);
+            }
+        }
+    }
+    const uint32_t units = group_units(b);
+    hipError_t e = hipSuccess;
+    for (int f = 0; f < 4 && e == hipSuccess; ++f) {
+        int occf = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occf, fused_kernel(f, units), kTileLanes, 0);
+        b.max_fused_blocks_per_cu[f] = occf > 0 ? occf : 1;
+    }
+    return e;
+}
+
+int upload_groups(mcdx_book& b) {
+    if (b.groups.empty()) return MCDX_OK;
+    return status_of(hipMemcpy(b.d_groups, b.groups.data(), b.groups.size() * sizeof(FusedGroup),
+                               hipMemcpyHostToDevice));
+}
+
 int upload_items(mcdx_book& b) {
     std::vector<uint64_t> items;
     std::vector<double> cost;
@@ -380,15 +629,16 @@ int upload_items(mcdx_book& b) {
     }
     std::vector<uint64_t> fused[4];
     std::vector<double> fcost[4];
+    const uint64_t tpi = b.tiles_per_item ? b.tiles_per_item : 1;
     for (uint32_t gi = 0; gi < b.groups.size(); ++gi) {
         const FusedGroup& g = b.groups[gi];
         int f = 0;
         while (kFusedKinds[f] != g.kind) ++f;
-        const uint64_t tiles = b.tile_offsets[g.c[0] + 1] - b.tile_offsets[g.c[0]];
+        const uint64_t tiles = tiles_for(b.contracts[g.c[0]].path_count);
         const double c1 = (double)g.m * (1.0 + 0.12 * (double)g.k);
-        for (uint64_t t = 0; t < tiles; ++t) {
+        for (uint64_t t = 0; t < tiles; t += tpi) {
             fused[f].push_back(((uint64_t)gi << 40) | t);
-            fcost[f].push_back(c1);
+            fcost[f].push_back(c1 * (double)std::min<uint64_t>(tpi, tiles - t));
         }
     }
     auto order_by_cost = [&](std::vector<uint64_t>& it, const std::vector<double>& c) {
@@ -422,12 +672,18 @@ int grid_blocks(const mcdx_book& b, int family = -1) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, n));
 }
 
-template <int F>
-void launch_family(const mcdx_book& b, uint64_t offset, hipStream_t s) {
-    if (!b.n_fused[F]) return;
-    hipLaunchKernelGGL(book_fused_kernel<kFusedKinds[F]>, dim3(grid_blocks(b, F)), dim3(kTileLanes),
-                       0, s, b.d_args, b.d_toff, b.d_items64 + offset, b.n_fused[F], b.d_groups,
-                       b.d_partials);
+hipError_t launch_family(const mcdx_book& b, int f, uint64_t offset, hipStream_t s) {
+    if (!b.n_fused[f]) return hipSuccess;
+    const GbmArgs* args = b.d_args;
+    const uint64_t* toff = b.d_toff;
+    const uint64_t* items = b.d_items64 + offset;
+    uint64_t n_items = b.n_fused[f];
+    const FusedGroup* groups = b.d_groups;
+    Welford* partials = b.d_partials;
+    uint32_t tpi = b.tiles_per_item;
+    void* params[] = {&args, &toff, &items, &n_items, &groups, &partials, &tpi};
+    return hipLaunchKernel(fused_kernel(f, group_units(b)), dim3(grid_blocks(b, f)),
+                           dim3(kTileLanes), params, 0, s);
 }
 
 hipError_t enqueue_raw(mcdx_book& b, hipStream_t s) {
@@ -441,13 +697,11 @@ hipError_t enqueue_raw(mcdx_book& b, hipStream_t s) {
                                b.d_kind, b.d_toff, b.d_items64, b.n_items, b.d_partials);
     }
     uint64_t off = b.n_items;
-    launch_family<0>(b, off, s);
-    off += b.n_fused[0];
-    launch_family<1>(b, off, s);
-    off += b.n_fused[1];
-    launch_family<2>(b, off, s);
-    off += b.n_fused[2];
-    launch_family<3>(b, off, s);
+    for (int f = 0; f < 4; ++f) {
+        hipError_t le = launch_family(b, f, off, s);
+        if (le != hipSuccess) return le;
+        off += b.n_fused[f];
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_final_merge(b.d_partials, b.d_toff, 0, b.slots, b.d_results, s);
@@ -528,7 +782,10 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
     // Greeks: four bumped scenarios per contract (default_bump_sizes,
     // finite_difference.cpp:31-38), then kGreekSlots result slots (FD delta, gamma, vega;
     // pathwise delta, vega).
-    std::vector<double> hs(n), hv(n);
+    std::vector<double>& hs = b->hs;
+    std::vector<double>& hv = b->hv;
+    hs.assign(n, 0.0);
+    hv.assign(n, 0.0);
     if (greeks) {
         for (uint32_t i = 0; i < n; ++i) {
             const mcdx_gbm_contract c = contracts[i];
@@ -562,7 +819,10 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
     for (uint32_t i = 0; i < slots; ++i) {
         const mcdx_gbm_contract& c = b->contracts[i];
         b->tile_offsets[i] = acc;
-        acc += tiles_for(c.path_count);
+        // a bumped scenario of a Greeks book has no price of its own to merge: its paths
+        // enter only through its unit's differences
+        const bool bumped = greeks && i >= n && i < 5 * n;
+        acc += bumped ? 0 : tiles_for(c.path_count);
         args[i] = make_args(c);
         const int kind = kind_of(c);
         kinds[i] = (uint8_t)(kind | (c.antithetic ? 16 : 0));
@@ -575,66 +835,6 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
         delete b;
         return MCDX_EINVAL;
     }
-    // Shared-draw groups.
-    auto fusable = [&](const mcdx_gbm_contract& c) {
-        const int kind = kind_of(c);
-        return precision == MCDX_PREC_FP32 && !c.antithetic && kind != kBarrierBB;
-    };
-    if (greeks) {
-        for (uint32_t i = 0; i < n; ++i) {
-            FusedGroup g{};
-            g.c[0] = i;
-            for (int j = 0; j < 4; ++j) g.c[1 + j] = n + 4 * i + j;
-            g.k = 5;
-            g.kind = kind_of(contracts[i]);
-            g.m = g.kind == kTerminal ? 1 : (uint32_t)contracts[i].monitoring_points;
-            g.greeks = 1;
-            g.gslot = b->greek_slot[i];
-            g.inv_2hs = 1.0 / (2.0 * hs[i]);
-            g.inv_hs2 = 1.0 / (hs[i] * hs[i]);
-            g.inv_2hv = 1.0 / (2.0 * hv[i]);
-            const mcdx_gbm_contract& c = contracts[i];
-            g.pw = pathwise_kind(c);
-            const double dt = g.kind == kTerminal ? c.time : c.time / (double)c.monitoring_points;
-            g.pw_sqrt_t = sqrt(c.time);
-            g.pw_vol_t = c.vol * c.time;
-            g.pw_sdt = (float)sqrt(dt);
-            g.pw_vdt = (float)(c.vol * dt);
-            for (uint32_t j = 0; j < g.k; ++j) b->group_of[g.c[j]] = (int32_t)b->groups.size();
-            for (int j = 0; j < kGreekSlots; ++j) b->group_of[g.gslot + j] = -2;
-            b->groups.push_back(g);
-        }
-    } else if (share) {
-        // key (seed, kind, steps) -> members by path count, descending
-        std::map<std::tuple<uint64_t, int, int>, std::vector<uint32_t>> buckets;
-        for (uint32_t i = 0; i < n; ++i) {
-            const mcdx_gbm_contract& c = contracts[i];
-            if (!fusable(c)) continue;
-            const int kind = kind_of(c);
-            buckets[{c.seed, kind, kind == kTerminal ? 1 : c.monitoring_points}].push_back(i);
-        }
-        for (auto& kv : buckets) {
-            auto& mem = kv.second;
-            if (mem.size() < 2) continue;
-            std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
-                return contracts[x].path_count > contracts[y].path_count;
-            });
-            for (size_t o = 0; o < mem.size(); o += kFuseMax) {
-                FusedGroup g{};
-                g.k = (uint32_t)std::min<size_t>(kFuseMax, mem.size() - o);
-                if (g.k < 2) break; // a lone tail member stays a plain item
-                for (uint32_t j = 0; j < g.k; ++j) g.c[j] = mem[o + j];
-                g.kind = std::get<1>(kv.first);
-                g.m = (uint32_t)std::get<2>(kv.first);
-                for (uint32_t j = 0; j < g.k; ++j) b->group_of[g.c[j]] = (int32_t)b->groups.size();
-                b->groups.push_back(g);
-            }
-        }
-    }
-    if (b->groups.size() >= (1u << 23)) {
-        delete b;
-        return MCDX_EINVAL;
-    }
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -644,14 +844,20 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
                 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, book_kernel<true>, kTileLanes, 0)
                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, book_kernel<false>, kTileLanes, 0);
     b->max_blocks_per_cu = occ > 0 ? occ : 1;
-    const void* fk[4] = {(const void*)book_fused_kernel<kTerminal>, (const void*)book_fused_kernel<kAsian>,
-                         (const void*)book_fused_kernel<kBarrier>, (const void*)book_fused_kernel<kLookback>};
-    for (int f = 0; f < 4 && e == hipSuccess; ++f) {
-        int occf = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occf, fk[f], kTileLanes, 0);
-        b->max_fused_blocks_per_cu[f] = occf > 0 ? occf : 1;
+    (void)share;
+    if (e == hipSuccess) {
+        const Schedule sc = tuned_schedule(shape_key(*b));
+        if (sc.valid) {
+            b->blocks_per_cu = sc.blocks_per_cu;
+            b->order = sc.order;
+            b->width = sc.width;
+            b->tiles_per_item = sc.tiles_per_item;
+        }
     }
-    const size_t ng = std::max<size_t>(1, b->groups.size());
+    if (e == hipSuccess) e = build_groups(*b);
+    // the most groups any width gives: one unit (Greeks) or two contracts (prices) each
+    const size_t ng = std::max<size_t>(1, greeks ? n : n / 2 + 1);
+
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->d_args, slots * sizeof(GbmArgs));
     if (e == hipSuccess) e = hipMalloc(&b->d_kind, slots);
@@ -664,18 +870,8 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
     if (e == hipSuccess) e = hipMemcpy(b->d_kind, kinds.data(), slots, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_toff, b->tile_offsets.data(), (slots + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !b->groups.empty())
-        e = hipMemcpy(b->d_groups, b->groups.data(), b->groups.size() * sizeof(FusedGroup),
-                      hipMemcpyHostToDevice);
     int st = status_of(e);
-    if (!st) {
-        std::lock_guard<std::mutex> lock(g_tune_mu);
-        auto it = g_tuned.find(shape_key(*b));
-        if (it != g_tuned.end()) {
-            b->blocks_per_cu = it->second.first;
-            b->order = it->second.second;
-        }
-    }
+    if (!st) st = upload_groups(*b);
     if (!st) st = upload_items(*b);
     if (!st) st = rebuild_graph(*b);
     if (st) {
@@ -691,27 +887,53 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
     return mcdx_book_create_ex(contracts, n, precision, 0u, out);
 }
 
-int mcdx_book_set_schedule(mcdx_book* b, uint32_t blocks_per_cu, uint32_t order) {
-    if (!b || order > 1) return MCDX_EINVAL;
+int mcdx_book_set_schedule2(mcdx_book* b, uint32_t blocks_per_cu, uint32_t order,
+                            uint32_t fusion_width, uint32_t tiles_per_item) {
+    if (!b) return MCDX_EINVAL;
+    Schedule sc;
+    sc.blocks_per_cu = blocks_per_cu;
+    sc.order = order;
+    sc.width = fusion_width ? fusion_width : b->width;
+    sc.tiles_per_item = tiles_per_item ? tiles_per_item : b->tiles_per_item;
+    if (!schedule_ok(sc)) return MCDX_EINVAL;
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e == hipSuccess && b->last_stream) e = hipStreamSynchronize(b->last_stream);
     if (e != hipSuccess) return status_of(e);
-    b->blocks_per_cu = blocks_per_cu;
-    if (order != b->order) {
-        b->order = order;
-        const int st = upload_items(*b);
-        if (st) return st;
+    const bool regroup = sc.width != b->width && (b->flags & MCDX_BOOK_GREEKS);
+    const bool reitem = regroup || sc.order != b->order || sc.tiles_per_item != b->tiles_per_item;
+    b->blocks_per_cu = sc.blocks_per_cu;
+    b->order = sc.order;
+    b->width = sc.width;
+    b->tiles_per_item = sc.tiles_per_item;
+    int st = MCDX_OK;
+    if (regroup) {
+        st = status_of(build_groups(*b));
+        if (!st) st = upload_groups(*b);
     }
-    return rebuild_graph(*b);
+    if (!st && reitem) st = upload_items(*b);
+    if (!st) st = rebuild_graph(*b);
+    return st;
 }
 
-int mcdx_book_get_schedule(const mcdx_book* b, uint32_t* blocks_per_cu, uint32_t* order,
-                           uint32_t* grid) {
+int mcdx_book_set_schedule(mcdx_book* b, uint32_t blocks_per_cu, uint32_t order) {
+    if (!b || order > 1) return MCDX_EINVAL;
+    return mcdx_book_set_schedule2(b, blocks_per_cu, order, 0, 0);
+}
+
+int mcdx_book_get_schedule2(const mcdx_book* b, uint32_t* blocks_per_cu, uint32_t* order,
+                            uint32_t* grid, uint32_t* fusion_width, uint32_t* tiles_per_item) {
     if (!b) return MCDX_EINVAL;
     if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu ? b->blocks_per_cu : (uint32_t)b->max_blocks_per_cu;
     if (order) *order = b->order;
     if (grid) *grid = (uint32_t)grid_blocks(*b);
+    if (fusion_width) *fusion_width = b->width;
+    if (tiles_per_item) *tiles_per_item = b->tiles_per_item;
     return MCDX_OK;
+}
+
+int mcdx_book_get_schedule(const mcdx_book* b, uint32_t* blocks_per_cu, uint32_t* order,
+                           uint32_t* grid) {
+    return mcdx_book_get_schedule2(b, blocks_per_cu, order, grid, nullptr, nullptr);
 }
 
 int mcdx_book_enqueue(mcdx_book* b, void* stream) {
@@ -762,8 +984,12 @@ int mcdx_book_price(mcdx_book* b, mcdx_result* out) {
     return mcdx_book_results(b, out);
 }
 
-// Times every candidate schedule (item order x blocks per CU) with HIP events on the book's
-// stream, keeps the fastest, and remembers it for books of the same shape in this process.
+// Searches the schedule one dimension at a time, starting from the current schedule: the
+// fusion width (Greeks books: 1, 2 or 3 units per group), the tiles per item (1, 2, 4), then
+// blocks per CU (powers of two up to the occupancy limit) with each item order. Every
+// candidate's replay is timed with HIP events on the book's stream after a warm-up replay;
+// the fastest is kept, applied, and remembered for books of the same shape, in this process
+// and in the schedule cache file.
 int mcdx_book_autotune(mcdx_book* b, float* best_ms) {
     if (!b) return MCDX_EINVAL;
     hipEvent_t t0, t1;
@@ -772,41 +998,71 @@ int mcdx_book_autotune(mcdx_book* b, float* best_ms) {
         (void)hipEventDestroy(t0);
         return MCDX_EDEVICE;
     }
-    float best = 1e30f;
-    uint32_t best_bpc = b->blocks_per_cu, best_order = b->order;
+    Schedule best;
+    best.blocks_per_cu = b->blocks_per_cu;
+    best.order = b->order;
+    best.width = b->width;
+    best.tiles_per_item = b->tiles_per_item;
+    float best_t = 1e30f;
     int st = MCDX_OK;
-    for (uint32_t order = 0; order <= 1 && !st; ++order) {
+    auto trial = [&](const Schedule& sc) {
+        if (st) return;
+        if ((st = mcdx_book_set_schedule2(b, sc.blocks_per_cu, sc.order, sc.width, sc.tiles_per_item)))
+            return;
+        float ms = 0.f;
+        hipError_t e = hipGraphLaunch(b->exec, b->stream); // warm-up
+        if (e == hipSuccess) e = hipEventRecord(t0, b->stream);
+        if (e == hipSuccess) e = hipGraphLaunch(b->exec, b->stream);
+        if (e == hipSuccess) e = hipEventRecord(t1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) {
+            st = status_of(e);
+            return;
+        }
+        if (ms < best_t) {
+            best_t = ms;
+            best = sc;
+        }
+    };
+    trial(best);
+    if (b->flags & MCDX_BOOK_GREEKS) {
+        const Schedule base = best;
+        for (uint32_t w = 1; w <= (uint32_t)kUnitCap; ++w) {
+            Schedule sc = base;
+            sc.width = w;
+            if (w != base.width) trial(sc);
+        }
+    }
+    {
+        const Schedule base = best;
+        for (uint32_t tpi : {1u, 2u, 4u}) {
+            Schedule sc = base;
+            sc.tiles_per_item = tpi;
+            if (tpi != base.tiles_per_item) trial(sc);
+        }
+    }
+    {
+        const Schedule base = best;
         uint32_t top = (uint32_t)b->max_blocks_per_cu;
         for (int f = 0; f < 4; ++f) top = std::max(top, (uint32_t)b->max_fused_blocks_per_cu[f]);
-        for (uint32_t bpc = 1; bpc <= top && !st; bpc *= 2) {
-            if ((st = mcdx_book_set_schedule(b, bpc, order))) break;
-            float ms = 0.f;
-            hipError_t e = hipGraphLaunch(b->exec, b->stream); // warm-up
-            if (e == hipSuccess) e = hipEventRecord(t0, b->stream);
-            if (e == hipSuccess) e = hipGraphLaunch(b->exec, b->stream);
-            if (e == hipSuccess) e = hipEventRecord(t1, b->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(t1);
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-            if (e != hipSuccess) {
-                st = status_of(e);
-                break;
+        for (uint32_t order = 0; order <= 1; ++order)
+            for (uint32_t bpc = 1; bpc <= top; bpc *= 2) {
+                Schedule sc = base;
+                sc.order = order;
+                sc.blocks_per_cu = bpc;
+                trial(sc);
             }
-            if (ms < best) {
-                best = ms;
-                best_bpc = bpc;
-                best_order = order;
-            }
-        }
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (st) return st;
-    if ((st = mcdx_book_set_schedule(b, best_bpc, best_order))) return st;
-    {
-        std::lock_guard<std::mutex> lock(g_tune_mu);
-        g_tuned[shape_key(*b)] = {best_bpc, best_order};
-    }
-    if (best_ms) *best_ms = best;
+    if ((st = mcdx_book_set_schedule2(b, best.blocks_per_cu, best.order, best.width,
+                                      best.tiles_per_item)))
+        return st;
+    best.valid = true;
+    remember_schedule(shape_key(*b), best);
+    if (best_ms) *best_ms = best_t;
     return MCDX_OK;
 }
 

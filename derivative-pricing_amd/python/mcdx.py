@@ -151,6 +151,8 @@ SIGNATURES = {
     "mcdx_book_create": (C.c_int, [P(GbmContract), u32, C.c_int, P(C.c_void_p)]),
     "mcdx_book_autotune": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "mcdx_book_set_schedule": (C.c_int, [C.c_void_p, u32, u32]),
+    "mcdx_book_set_schedule2": (C.c_int, [C.c_void_p, u32, u32, u32, u32]),
+    "mcdx_book_get_schedule2": (C.c_int, [C.c_void_p, P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "mcdx_book_get_schedule": (C.c_int, [C.c_void_p, P(u32), P(u32), P(u32)]),
     "mcdx_book_enqueue": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mcdx_book_results": (C.c_int, [C.c_void_p, P(Result)]),
@@ -459,11 +461,22 @@ class Book:
     def set_schedule(self, blocks_per_cu, order):
         check(lib().mcdx_book_set_schedule(self.h, blocks_per_cu, order), "mcdx_book_set_schedule")
 
+    def set_schedule2(self, blocks_per_cu, order, width=0, tiles_per_item=0):
+        check(lib().mcdx_book_set_schedule2(self.h, blocks_per_cu, order, width, tiles_per_item),
+              "mcdx_book_set_schedule2")
+
     def schedule(self):
         b, o, g = u32(), u32(), u32()
         check(lib().mcdx_book_get_schedule(self.h, C.byref(b), C.byref(o), C.byref(g)),
               "mcdx_book_get_schedule")
         return b.value, o.value, g.value
+
+    def schedule2(self):
+        """(blocks per CU, order, grid, fusion width, tiles per item)"""
+        v = [u32() for _ in range(5)]
+        check(lib().mcdx_book_get_schedule2(self.h, *[C.byref(x) for x in v]),
+              "mcdx_book_get_schedule2")
+        return tuple(x.value for x in v)
 
     def close(self):
         if self.h:

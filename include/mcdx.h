@@ -374,13 +374,24 @@ int mcdx_book_create(const mcdx_gbm_contract* contracts, uint32_t n, int precisi
                                    scenarios sharing each path's draws */
 int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int precision,
                         uint32_t flags, mcdx_book** out);
-/* Time every schedule (item order x blocks per CU), keep the fastest, remember it for
- * books of the same shape; best_ms may be NULL. */
+/* Search the schedule (fusion width for Greeks books, tiles per item, blocks per CU x item
+ * order, one dimension at a time, each candidate timed by replay), keep the fastest, and
+ * remember it for books of the same shape: in this process and in the schedule cache file
+ * ($MCDX_BOOK_SCHEDULE_CACHE, else ~/.cache/mcdx/book_schedules.txt; unreadable or damaged
+ * entries fall back to the default schedule). best_ms may be NULL. */
 int mcdx_book_autotune(mcdx_book* book, float* best_ms);
 /* order 0 = contract order, 1 = longest tiles first; blocks_per_cu 0 = occupancy limit. */
 int mcdx_book_set_schedule(mcdx_book* book, uint32_t blocks_per_cu, uint32_t order);
 int mcdx_book_get_schedule(const mcdx_book* book, uint32_t* blocks_per_cu, uint32_t* order,
                            uint32_t* grid);
+/* The full schedule: fusion_width = Greek units (contract + its 4 bumped scenarios) sharing
+ * each path's draws in one work item, 1..3 (Greeks books; 0 keeps the current one);
+ * tiles_per_item = consecutive 1024-path tiles one block takes per item, 1, 2, 4 or 8 (0
+ * keeps the current one). No schedule changes any result. */
+int mcdx_book_set_schedule2(mcdx_book* book, uint32_t blocks_per_cu, uint32_t order,
+                            uint32_t fusion_width, uint32_t tiles_per_item);
+int mcdx_book_get_schedule2(const mcdx_book* book, uint32_t* blocks_per_cu, uint32_t* order,
+                            uint32_t* grid, uint32_t* fusion_width, uint32_t* tiles_per_item);
 /* Replay the captured graph on `stream` (NULL = the book's own stream); returns at once. */
 int mcdx_book_enqueue(mcdx_book* book, void* stream);
 /* Wait for the last enqueue and copy one result per contract into out[0..n). */

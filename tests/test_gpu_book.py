@@ -245,3 +245,86 @@ def test_c4_full_book_equals_per_contract_calls_bitwise():
     for c, g in zip(cs, got):
         assert g == mcdx.price(c, FP32)
     book.close()
+
+
+def greeks_mix_book():
+    # Greeks units of every family with ragged path counts, so groups of 2 and 3 units hold
+    # members whose tiles end at different points.
+    cs = crn_book(30, seed=11)
+    for c in cs:
+        c.antithetic = 0
+        c.brownian_bridge = 0
+    return cs
+
+
+def test_greeks_book_bits_independent_of_fusion_width_and_tiles_per_item():
+    cs = greeks_mix_book()
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    book.set_schedule2(0, 1, 1, 1)
+    ref = (book.price(), book.greeks(), book.pathwise())
+    for c, p in zip(cs, ref[0]):
+        assert p == mcdx.price(c, FP32)
+    for width in (1, 2, 3):
+        for tpi in (1, 2, 4):
+            for bpc, order in ((0, 1), (1, 0)):
+                book.set_schedule2(bpc, order, width, tpi)
+                assert book.schedule2()[3:] == (width, tpi)
+                got = (book.price(), book.greeks(), book.pathwise())
+                assert got[0] == ref[0] and got[1] == ref[1]
+                # NaN (no pathwise estimator) compares by its place
+                assert [[tuple(np.nan_to_num(x)) for x in r] for r in got[2]] == \
+                       [[tuple(np.nan_to_num(x)) for x in r] for r in ref[2]]
+    with pytest.raises(mcdx.McdxError):
+        book.set_schedule2(0, 1, 4, 1)
+    with pytest.raises(mcdx.McdxError):
+        book.set_schedule2(0, 1, 1, 3)
+    book.close()
+
+
+def test_price_book_tiles_per_item_do_not_change_bits():
+    cs = crn_book()
+    book = mcdx.Book(cs, FP32)
+    ref = book.price()
+    for tpi in (2, 4, 8):
+        book.set_schedule2(0, 1, 0, tpi)
+        assert book.price() == ref
+    book.close()
+
+
+def test_schedule_cache_file_round_trip_and_damage(tmp_path, monkeypatch):
+    path = tmp_path / "sched.txt"
+    monkeypatch.setenv("MCDX_BOOK_SCHEDULE_CACHE", str(path))
+    cs = greeks_mix_book()
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    ref = (book.price(), book.greeks())
+    book.autotune()
+    tuned = book.schedule2()
+    assert (book.price(), book.greeks()) == ref
+    book.close()
+    lines = path.read_text().splitlines()
+    assert lines and lines[-1].startswith("mcdx-book-schedule 1 ")
+    key = lines[-1].split()[2]
+    # a new book of the same shape starts from the tuned schedule
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    assert book.schedule2()[3:] == tuned[3:]
+    book.close()
+    # a damaged cache (garbage, values out of range, wrong version) falls back to defaults
+    path.write_text("garbage\n" + f"mcdx-book-schedule 1 {key} 2 1 9 1\n" +
+                    f"mcdx-book-schedule 2 {key} 1 0 1 1\n" + f"mcdx-book-schedule 1 {key} 1 1 1\n")
+    monkeypatch.setenv("MCDX_BOOK_SCHEDULE_CACHE", str(tmp_path / "other.txt"))
+    mcdx.Book(cs[:3], FP32, mcdx.Book.GREEKS).close()  # the cache is re-read on a path change
+    monkeypatch.setenv("MCDX_BOOK_SCHEDULE_CACHE", str(path))
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    assert book.schedule2()[3:] == (2, 1)  # the default width and tiles per item
+    assert (book.price(), book.greeks()) == ref
+    book.close()
+    # a valid line for this shape is taken
+    path.write_text(f"mcdx-book-schedule 1 {key} 1 0 3 4\n")
+    monkeypatch.setenv("MCDX_BOOK_SCHEDULE_CACHE", str(tmp_path / "x.txt"))
+    b2 = mcdx.Book(cs[:3], FP32, mcdx.Book.GREEKS)  # another shape: reads x.txt (absent)
+    b2.close()
+    monkeypatch.setenv("MCDX_BOOK_SCHEDULE_CACHE", str(path))
+    book = mcdx.Book(cs, FP32, mcdx.Book.GREEKS)
+    assert book.schedule2()[1] == 0 and book.schedule2()[3:] == (3, 4)
+    assert (book.price(), book.greeks()) == ref
+    book.close()
