@@ -71,7 +71,7 @@ def build_host_lib():
             if not s.endswith("module.cpp")]
     out = os.path.join(LIBDIR, "libmcd.so")
     run(["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
-         "-I", os.path.join(ROOT, "include")] + srcs +
+         "-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs +
         ["-L", LIBDIR, "-lmcd_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-soname,libmcd.so",
          "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
@@ -85,7 +85,7 @@ def build_pybind():
     suffix = sysconfig.get_config_var("EXT_SUFFIX")
     out = os.path.join(PYDIR, "mcd" + suffix)
     cmd = ["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
-           "-I", os.path.join(ROOT, "include"), "-I", pybind11.get_include(),
+           "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", pybind11.get_include(),
            "-I", sysconfig.get_paths()["include"]] + hosts + [
            "-L", LIBDIR, "-lmcd_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", out + ".tmp"]
     run(cmd)
@@ -104,15 +104,13 @@ def build_cli():
 
 
 def build_cpp_tests():
-    """tests/cpp/api_test: C++ tests of the drop-in API, linked against libmcd.so."""
-    src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
-    if not os.path.exists(src):
-        return
-    out = os.path.join(ROOT, "tests", "cpp", "api_test")
-    run(["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), src,
-         "-L", LIBDIR, "-lmcd", "-lmcd_hip", "-Wl,-rpath,$ORIGIN/../../derivative-pricing_amd/lib",
-         "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+    """tests/cpp/*.cpp: C++ callers of the drop-in API, each linked against libmcd.so."""
+    for src in sorted(glob.glob(os.path.join(ROOT, "tests", "cpp", "*.cpp"))):
+        out = src[:-len(".cpp")]
+        run(["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+             src, "-L", LIBDIR, "-lmcd", "-lmcd_hip",
+             "-Wl,-rpath,$ORIGIN/../../derivative-pricing_amd/lib", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
 
 
 def build_oracle():

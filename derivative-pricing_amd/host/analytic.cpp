@@ -11,6 +11,7 @@
 #include <numbers>
 #include <vector>
 
+#include "mcd/core/gauss_legendre.hpp"
 #include "mcd/core/types.hpp"
 #include "mcd/greeks/finite_difference.hpp"
 #include "mcd/pricers/analytic.hpp"
@@ -188,36 +189,15 @@ cd heston_cf(const models::HestonParams& p, cd u) { // Little-Trap form (heston.
     return std::exp(big_c + big_d * p.v0);
 }
 
-void gl_rule(int n, std::vector<double>& x, std::vector<double>& w) { // Newton on P_n
-    x.assign(n, 0.0);
-    w.assign(n, 0.0);
-    for (int i = 0; i < (n + 1) / 2; ++i) {
-        double z = std::cos(std::numbers::pi * (i + 0.75) / (n + 0.5)), pp = 0.0;
-        for (int it = 0; it < 100; ++it) {
-            double p0 = 1.0, p1 = z;
-            for (int k = 2; k <= n; ++k) {
-                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
-                p0 = p1;
-                p1 = p2;
-            }
-            pp = n * (z * p1 - p0) / (z * z - 1.0);
-            const double dz = p1 / pp;
-            z -= dz;
-            if (std::abs(dz) < 1e-15) break;
-        }
-        x[i] = -z;
-        x[n - 1 - i] = z;
-        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
-    }
-}
 } // namespace
 
 double heston_semi_analytic(const models::HestonParams& p, double strike, OptionType type) noexcept {
     const cd i(0.0, 1.0);
     const double lm = std::log(strike / p.spot);
     const cd phi_ni = heston_cf(p, cd(0.0, -1.0));
-    std::vector<double> x, w;
-    gl_rule(128, x, w); // 128 nodes on [0, 200] (heston.cpp:42-44)
+    const GaussLegendreRule rule = gauss_legendre_rule(128); // 128 nodes on [0, 200] (heston.cpp:42-44)
+    const std::vector<double>& x = rule.nodes;
+    const std::vector<double>& w = rule.weights;
     const double hw = 100.0, mid = 100.0;
     double s1 = 0.0, s2 = 0.0;
     for (int k = 0; k < 128; ++k) {
