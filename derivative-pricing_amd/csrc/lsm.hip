@@ -569,7 +569,8 @@ struct PolicyArgs {
     Welford* part_nn;             // [tiles + 1]
     Welford* part_qr;             // [tiles + 1]
     uint32_t* tile_counts;        // [3][tiles] ITM decisions, disagreements, fallbacks
-    int judge;                    // 0: the QR policy only decides outside the table range
+    int judge;                    // 0: the QR policy only decides outside the table range;
+                                  // 1: both policies on every path; 2: the QR policy alone
 };
 
 // counters[c] = sum over tiles of tile_counts[c][tile], c < 3 (one block).
@@ -687,8 +688,9 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
     const uint64_t p = tile * kTilePaths + t;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     // Without the judge the QR policy starts stopped: it then only decides for the network
-    // outside the table range, and its own price is not formed.
-    PolicyState ps = {false, q.judge == 0, 0, 0, 0.0, 0.0, 0u, 0u, 0u};
+    // outside the table range, and its own price is not formed. Serving the QR policy alone,
+    // the network starts stopped and the table is never read.
+    PolicyState ps = {q.judge == 2, q.judge == 0, 0, 0, 0.0, 0.0, 0u, 0u, 0u};
     if (p < a.n_paths) {
         // t < 1024 = kTilePaths: the block's paths share p >> 32, a scalar here.
         const uint32_t plo = (uint32_t)p, phh = (uint32_t)((tile * kTilePaths) >> 32);
@@ -1264,18 +1266,22 @@ PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
 
 // Table (MFMA evaluation of the network on the grid), then the streaming policy kernel and
 // the two fixed-order merges. Results land in pn[tiles], pq[tiles] and ctr[0..2].
+// mode: 0 the network (QR outside the table range), 1 both policies (the judge), 2 the QR
+// policy alone (no table).
 int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_t grid,
-                   double lo, double hi, bool fp64, bool judge, hipStream_t s) {
+                   double lo, double hi, bool fp64, int mode, hipStream_t s) {
     const int m = r.a.m;
     if (m + 1 > kMaxLdsDates) return MCDX_EINVAL;
     const uint64_t pts = (uint64_t)(m - 1) * grid;
     const double h = (hi - lo) / (double)(grid - 1);
-    hipLaunchKernelGGL(lsm_grid_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, b.gx,
-                       b.gt, m, grid, lo, h);
-    int st = mlp_eval_device(w, b.gx, b.gt, pts, b.table, s);
-    if (st) return st;
+    if (mode != 2) {
+        hipLaunchKernelGGL(lsm_grid_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s,
+                           b.gx, b.gt, m, grid, lo, h);
+        int st = mlp_eval_device(w, b.gx, b.gt, pts, b.table, s);
+        if (st) return st;
+    }
     PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, 1.0 / r.a.strike, r.a.strike / kTargetScale,
-                 b.pn, b.pq, b.tile_counts, judge ? 1 : 0};
+                 b.pn, b.pq, b.tile_counts, mode};
     if (fp64)
         hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTilePaths),
                            has_lds_bytes(m), s, r.a, q);
@@ -1461,12 +1467,17 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
     if ((st = mlp_init_device(w, o->init_seed, 0.f, s))) return st;
     const float decay = (float)pow((double)o->final_learning_rate / (double)o->learning_rate,
                                    1.0 / (double)o->steps);
+    // the last average_steps iterates (all of them when the run is shorter)
+    const int avg_from = o->average_steps > 1
+                             ? (o->average_steps >= (uint32_t)o->steps ? 0 : o->steps - (int)o->average_steps)
+                             : -1;
     if ((st = mlp_train_device(w, r.a.cap_f1, r.a.cap_f2, r.a.cap_y, (uint64_t)(m - 1) * n,
-                               o->steps, o->learning_rate, decay, o->sample_seed, losses, nullptr, s)))
+                               o->steps, o->learning_rate, decay, o->sample_seed, losses, nullptr, s,
+                               avg_from)))
         return st;
     double lo, hi;
     table_range(spot, strike, rate, carry, vol, time, type, &lo, &hi);
-    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, fp64, true, s))) return st;
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, fp64, 1, s))) return st;
     char* h = (char*)ctx->pinned;
     PolicyHost* ph = (PolicyHost*)h;
     e = hipMemcpyAsync(&ph->qr_lsm, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
@@ -1513,7 +1524,8 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
                       int precision, const float* weights, uint32_t grid, double lo, double hi,
                       const double* coeffs, const uint8_t* has, uint8_t judge,
                       mcdx_lsm_learned_result* out) {
-    if (!out || !weights || !coeffs || !has || m < 2 || grid < 2 || grid > 4096 || !(hi > lo))
+    if (!out || (!weights && judge != 2) || !coeffs || !has || m < 2 || grid < 2 || grid > 4096 ||
+        !(hi > lo) || judge > 2)
         return MCDX_EINVAL;
     ThreadCtx* ctx;
     int st = thread_ctx(&ctx);
@@ -1542,11 +1554,13 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     char* h = (char*)ctx->pinned;
     for (int i = 0; i < (m + 1) * 4; ++i) ((double*)(h + h_coef))[i] = coeffs[i];
     for (int i = 0; i <= m; ++i) ((uint8_t*)(h + h_has))[i] = has[i] ? 1 : 0;
-    for (int i = 0; i < kMlpParams; ++i) ((float*)(h + h_w))[i] = weights[i];
+    if (judge != 2)
+        for (int i = 0; i < kMlpParams; ++i) ((float*)(h + h_w))[i] = weights[i];
     e = hipMemcpyAsync(r.a.coeffs, h + h_coef, pol_span, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && judge != 2)
+        e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return status_of(e);
-    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, judge != 0, s)))
+    if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, judge, s)))
         return st;
     PolicyHost* ph = (PolicyHost*)h;
     e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
@@ -1554,19 +1568,19 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
     // The table comes back only for the judge's boundary comparison: serving without the
     // judge skips the read-back and the host scan of (m - 1) x grid points.
-    if (e == hipSuccess && judge)
+    if (e == hipSuccess && judge == 1)
         e = hipMemcpyAsync(h + h_tbl, pb.table, tbl * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return status_of(e);
     mcdx_lsm_learned_result res{};
     res.qr_lsm = mcdx_result{NAN, NAN, 0};
-    res.learned = with_inception(ph->nn, spot, strike, type);
+    res.learned = judge != 2 ? with_inception(ph->nn, spot, strike, type) : mcdx_result{NAN, NAN, 0};
     res.qr_forward = judge ? with_inception(ph->qr, spot, strike, type) : mcdx_result{NAN, NAN, 0};
     res.itm_decisions = ph->ctr[0];
     res.disagreements = ph->ctr[1];
     res.fallbacks = ph->ctr[2];
     res.final_loss = NAN;
-    if (judge) {
+    if (judge == 1) {
         boundary_gap((const float*)(h + h_tbl), coeffs, has, m, grid, lo, hi, strike, type,
                      &res.boundary_gap, &res.boundary_dates);
     } else {
@@ -1636,6 +1650,8 @@ void mcdx_lsm_learn_default_options(mcdx_lsm_learn_options* o) {
     o->learning_rate = 3e-3f;
     o->final_learning_rate = 1e-4f;
     o->grid_points = 1024;
+    o->average_steps = 3000;
+    o->reserved = 0;
 }
 
 int mcdx_lsm_american_learned(double spot, double strike, double rate, double carry_yield,

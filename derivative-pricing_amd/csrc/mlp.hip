@@ -258,6 +258,9 @@ struct TrainArgs {
     uint64_t seed;
     float* loss_out;   // [steps] batch losses
     float* grad_out;   // when non-null: one batch (indices 0..127), gradients written, no update
+    int32_t avg_from;  // the weights returned are the mean of the iterates after steps
+                       // avg_from .. steps - 1 (Polyak tail averaging; avg_from >= steps - 1:
+                       // the last iterate)
 };
 
 constexpr int kTrainLanes = 128;
@@ -296,12 +299,14 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
     const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
     float bc1 = 1.f, bc2 = 1.f, lr = a.lr;
     float wr[kPer], am[kPer], av[kPer]; // parameters and Adam moments, lane-owned
+    float wa[kPer];                     // running mean of the iterates from avg_from on
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int i = tid + kTrainLanes * k;
         wr[k] = i < kParams ? a.w[i] : 0.f;
         am[k] = 0.f;
         av[k] = 0.f;
+        wa[k] = 0.f;
     }
     for (int step = 0; step < a.steps; ++step) {
 #pragma unroll
@@ -363,13 +368,19 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
                 wr[k] -= lr * mh * fast_rcp(fast_sqrt(vh) + eps);
             }
         }
+        if (step >= a.avg_from) { // running mean of the iterates (uniform branch)
+            const float inv = 1.f / (float)(step - a.avg_from + 1);
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) wa[k] += (wr[k] - wa[k]) * inv;
+        }
         lr *= a.lr_decay;
         __syncthreads(); // every lane's reads of the LDS weights and gradients are done
     }
+    const bool averaged = a.avg_from < a.steps - 1;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int i = tid + kTrainLanes * k;
-        if (i < kParams) a.w[i] = wr[k];
+        if (i < kParams) a.w[i] = averaged ? wa[k] : wr[k];
     }
 }
 
@@ -418,8 +429,9 @@ int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s) {
 
 int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
                      int steps, float lr, float lr_decay, uint64_t seed, float* loss, float* grad,
-                     hipStream_t s) {
-    TrainArgs a{w, xs, ts, ys, n, steps, lr, lr_decay, seed, loss, grad};
+                     hipStream_t s, int avg_from) {
+    TrainArgs a{w, xs, ts, ys, n, steps, lr, lr_decay, seed, loss, grad,
+                avg_from < 0 ? steps : avg_from};
     const int bytes = (int)sizeof(TrainLds);
     hipError_t e = hipFuncSetAttribute((const void*)mlp_train_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

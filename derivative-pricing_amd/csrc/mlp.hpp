@@ -16,9 +16,11 @@ int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s);
 // (xs, ts, ys)[0..n); samples whose target is NaN are redrawn (up to 8 times, then masked).
 // loss[step] receives the mean squared error of the step's first 64 samples when non-null.
 // With grad non-null: one pass over samples 0..127, gradients to grad, no update.
+// avg_from >= 0: w receives the mean of the iterates after steps avg_from .. steps - 1
+// (Polyak tail averaging) instead of the last iterate.
 int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
                      int steps, float lr, float lr_decay, uint64_t seed, float* loss, float* grad,
-                     hipStream_t s);
+                     hipStream_t s, int avg_from = -1);
 int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
                     hipStream_t s);
 

@@ -58,7 +58,8 @@ def contract(product, spot=100.0, strike=100.0, barrier=0.0, rebate=0.0, rate=0.
 class LearnOptions(C.Structure):
     _fields_ = [("init_seed", C.c_uint64), ("sample_seed", C.c_uint64), ("steps", C.c_int32),
                 ("learning_rate", C.c_float), ("final_learning_rate", C.c_float),
-                ("grid_points", C.c_uint32)]
+                ("grid_points", C.c_uint32), ("average_steps", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class LearnedResult(C.Structure):
@@ -397,13 +398,15 @@ def lsm_learned(spot, strike, rate, carry, vol, time, type, m, n, seed, precisio
 
 def lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed, precision, w,
                       grid, lo, hi, coeffs, has, judge=True):
+    """judge True/1: both policies on the same paths; False/0: the network only; 2: the QR
+    policy alone through the same streaming kernel."""
     r = LearnedResult()
-    w_, pw = _f32(w)
+    w_, pw = _f32(w) if w is not None else (None, None)
     co = np.ascontiguousarray(coeffs, dtype=np.float64).ravel()
     hs = np.ascontiguousarray(has, dtype=np.uint8)
     check(lib().mcdx_lsm_learned_price(spot, strike, rate, carry, vol, time, type, m, n, seed,
                                        precision, pw, grid, lo, hi, co.ctypes.data_as(P(dbl)),
-                                       hs.ctypes.data_as(P(u8)), 1 if judge else 0, C.byref(r)),
+                                       hs.ctypes.data_as(P(u8)), int(judge), C.byref(r)),
           "mcdx_lsm_learned_price")
     return _learned_dict(r)
 

@@ -317,6 +317,10 @@ typedef struct mcdx_lsm_learn_options {
     float learning_rate;       /* first step */
     float final_learning_rate; /* geometric decay to this by the last step */
     uint32_t grid_points;      /* continuation table points per exercise date, 2..4096 */
+    uint32_t average_steps;    /* the network used is the mean of the last average_steps
+                                  Adam iterates (Polyak tail averaging; all iterates when
+                                  steps is smaller); 0 or 1: the last iterate */
+    uint32_t reserved;
 } mcdx_lsm_learn_options;
 
 typedef struct mcdx_lsm_learned_result {
@@ -343,10 +347,11 @@ int mcdx_lsm_american_learned(double spot, double strike, double rate, double ca
                               const mcdx_lsm_learn_options* options, float* weights,
                               double* coefficients, uint8_t* has_coefficients,
                               mcdx_lsm_learned_result* out);
-/* Price with a trained network on fresh paths: streaming, no path store. With judge != 0 the
+/* Price with a trained network on fresh paths: streaming, no path store. With judge == 1 the
  * QR policy also runs on the same paths (qr_forward, decisions, disagreements) and the
  * boundary gap is measured; with judge == 0 it only decides outside the table range,
- * qr_forward and boundary_gap are NaN and boundary_dates is 0. */
+ * qr_forward and boundary_gap are NaN and boundary_dates is 0; judge == 2 serves the QR
+ * policy alone by the same streaming kernel (qr_forward; weights may be NULL, learned NaN). */
 int mcdx_lsm_learned_price(double spot, double strike, double rate, double carry_yield,
                            double vol, double time, uint8_t type, int32_t monitoring_points,
                            uint64_t path_count, uint64_t seed, int precision,

@@ -6,8 +6,9 @@ decisions and discounting against the backward pass that is itself checked again
 oracle; (2) the Adam step of the training kernel against the numpy oracle
 (oracle/mlp_oracle.py) on the batch the oracle's Philox picks; (3) the learned price at the
 American put configuration (S=100 K=100 r=0.06 sigma=0.2 T=1, 50 dates, 2^20 paths) within
-1 SE of QR-LSM on the same paths and within 3 SE + 0.05 of CRR (the reference's LSM bar,
-tests/lsm_test.cpp:76-98; 50 dates price a Bermudan, below the American value)."""
+1 SE of QR-LSM on the same paths and within 3 SE of CRR-10^4 and of the 50-date Bermudan tree
+(the reference's LSM bar, tests/lsm_test.cpp:76-98); (4) out of sample, the network's policy
+against the QR policy on the same fresh paths (what the network gains)."""
 import os
 import sys
 
@@ -87,29 +88,75 @@ def test_qr_forward_replay_reproduces_backward_induction(prec):
     assert s["boundary_gap"] == d["boundary_gap"] and s["boundary_dates"] == d["boundary_dates"]
 
 
+def crr_bermudan_put(spot, dates=50, steps=10_000):
+    return O.lib().orc_crr_bermudan(spot, 100, 0.06, 0, 0.2, 1, steps, 1, steps // dates)
+
+
 def test_learned_american_put_c5():
     n = 1 << 20
     d, w, co, has = mcdx.lsm_learned(*PUT, n, 42, FP32)
-    crr = O.lib().orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 10_000, 1)
+    L = O.lib()
+    crr = L.orc_crr_american(100, 100, 0.06, 0, 0.2, 1, 10_000, 1)
+    berm = crr_bermudan_put(100.0)
     qp, qse, _ = d["qr_lsm"]
     lp, lse, _ = d["learned"]
-    print("\nqr", qp, qse, "learned", lp, lse, "crr", crr, "loss", d["final_loss"],
+    print("\nqr", qp, qse, "learned", lp, lse, "crr", crr, "bermudan", berm, "loss", d["final_loss"],
           "disagree", d["disagreements"], "/", d["itm_decisions"], "fallbacks", d["fallbacks"],
           "gap", d["boundary_gap"], d["boundary_dates"])
+    # in sample (the paths the regression and the network were fitted on): within 1 SE of
+    # QR-LSM, and within 3 SE of CRR-10^4 and of the 50-date Bermudan tree
     assert abs(lp - qp) <= qse
-    assert abs(lp - crr) <= 3 * lse + 0.05
+    assert abs(lp - crr) < 3 * lse and abs(lp - berm) < 3 * lse
     assert d["disagreements"] <= 0.03 * d["itm_decisions"]
     assert d["fallbacks"] <= 1e-3 * d["itm_decisions"]
     # the trained network is what makes the policy: an untrained one prices far lower
     u, _, _, _ = mcdx.lsm_learned(*PUT, 1 << 16, 42, FP32, steps=1)
     assert u["learned"][0] < u["qr_lsm"][0] - 5 * u["qr_lsm"][1]
-    # out of sample (fresh paths, streaming, no path store) and at other spots (features are
-    # S/K): within the same bar of CRR
+    # Out of sample (fresh paths, streaming, no path store) and at other spots (features are
+    # S/K): an exercise policy run forward on fresh paths is a low-biased estimator of the
+    # option value (no policy beats the optimal one), so each price is judged against the QR
+    # policy on the same paths (within 1 SE) and may not exceed the Bermudan value by more than
+    # 3 SE; its low bias against that value is printed.
     for spot in (90.0, 100.0, 110.0):
         args = (spot,) + PUT[1:]
         s = mcdx.lsm_learned_price(*args, 1 << 19, 4242, FP32, w, 1024, d["table_lo"],
                                    d["table_hi"], co, has)
-        c = O.lib().orc_crr_american(spot, 100, 0.06, 0, 0.2, 1, 10_000, 1)
-        print(spot, s["learned"], s["qr_forward"], c, s["fallbacks"])
-        assert abs(s["learned"][0] - c) <= 3 * s["learned"][1] + 0.05
+        b = crr_bermudan_put(spot)
+        print(spot, s["learned"], s["qr_forward"], "bermudan", b,
+              "low bias (SE)", (s["learned"][0] - b) / s["learned"][1], s["fallbacks"])
         assert abs(s["learned"][0] - s["qr_forward"][0]) <= s["learned"][1]
+        assert s["learned"][0] < b + 3 * s["learned"][1]
+        assert s["qr_forward"][0] < b + 3 * s["qr_forward"][1]
+
+
+@pytest.mark.parametrize("n_train", [1 << 16, 1 << 20])
+def test_network_policy_beats_qr_policy_out_of_sample(n_train):
+    """What the network gains over the method without it: trained on the same n_train paths
+    (the QR-LSM fit's regression samples, 6000 Adam steps, the last 3000 iterates averaged),
+    the network's exercise policy earns more than the QR policy fitted on those paths when
+    both run on the same eight fresh sets of 2^20 paths (a forward policy value is low-biased,
+    so more is better). The paired differences share their paths, so their spread is far
+    below either price's SE. Measured: +0.0060 +- 0.0004 (2^16), +0.0015 +- 0.0002 (2^20)."""
+    d, w, co, has = mcdx.lsm_learned(*PUT, n_train, 42, FP32)
+    diffs = []
+    for k in range(8):
+        s = mcdx.lsm_learned_price(*PUT, 1 << 20, 5000 + k, FP32, w, 1024, d["table_lo"],
+                                   d["table_hi"], co, has)
+        diffs.append(s["learned"][0] - s["qr_forward"][0])
+    diffs = np.array(diffs)
+    se = diffs.std(ddof=1) / np.sqrt(len(diffs))
+    print(f"\nn_train {n_train}: network - QR = {diffs.mean():+.5f} +- {se:.5f}")
+    assert diffs.mean() > 3 * se > 0
+
+
+def test_qr_policy_streaming_matches_judged_serving():
+    # judge == 2 serves the QR policy alone through the streaming kernel: the same price as the
+    # QR policy run alongside the network (judge == 1) on the same paths, bit for bit.
+    n = 1 << 17
+    d, w, co, has = mcdx.lsm_learned(*PUT, n, 42, FP32, steps=100)
+    both = mcdx.lsm_learned_price(*PUT, n, 77, FP32, w, 1024, d["table_lo"], d["table_hi"], co,
+                                  has, judge=1)
+    qr = mcdx.lsm_learned_price(*PUT, n, 77, FP32, None, 1024, d["table_lo"], d["table_hi"], co,
+                                has, judge=2)
+    assert qr["qr_forward"] == both["qr_forward"]
+    assert qr["learned"][0] != qr["learned"][0]  # NaN: the network is not run
