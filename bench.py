@@ -53,6 +53,24 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector peak
 # 30 (erfc 20, exp 4, 6 others) = 136, plus the terminal spot, payoff and Welford update 8.
 OPS_C1_FP64_PER_PATH = 144
 
+# Secondary path kernels, the same rule (FP32 generation; a draw = OPS_DRAW = 113):
+# * bridge-corrected barrier, per path-step: draw 113 + log2-space step 3 + crossing exponent
+#   and tests 6 = 122 (the stream-1 uniform is drawn only where the crossing probability can
+#   exceed 2^-34; those draws are not counted);
+# * Sobol QMC with the Brownian bridge, per point-date: Sobol word (LDS part read, 10 masked
+#   XORs) 12 + inverse normal 33 + bridge point 2 FMA = 4 + exp2 with its FMA 3 + observe 1
+#   = 53;
+# * Heston QE, per path-step: two Philox words 160 + inverse normal of z_s 33 + moment
+#   matching m, s^2, m^2, psi with its Newton reciprocal and the switch 12 + quadratic branch
+#   (1/psi, b^2, a, b with two square roots, Phi^-1(U) 33, v') 53 + variance and log-spot
+#   update 12 = 270;
+# * batched book, per shared-draw group path-step: draw 113 + 4 per member evaluated on it
+#   (log2 step 3, observation 1); a plain (ungrouped) item 117 per path-step.
+OPS_BB = 122
+OPS_QMC = 53
+OPS_HESTON = 270
+OPS_MEMBER = 4
+
 
 def contracts(mcdx, n):
     common = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.2, time=1.0,
@@ -222,7 +240,11 @@ def extras(torch, mcdx, L, stream, handle, ref):
     enq(c)
     ms = timed_events(torch, stream, lambda: enq(c), reps=2)
     p, se, _ = mcdx.price(c, mcdx.PREC_FP32)
-    entry = {"paths_per_s": N_PATHS / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se}
+    tf = OPS_BB * N_PATHS * STEPS / (ms / 1e3) / 1e12
+    entry = {"paths_per_s": N_PATHS / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se,
+             "roofline": {"bound": "valu", "ops_per_path_step": OPS_BB, "achieved": tf,
+                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP32_TFLOPS,
+                          "timing": "pricing call (tile kernel + merge), HIP events"}}
     if ref:
         rp = fh(ref["c2_barrier_uo_call_bb"]["price"])
         entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
@@ -259,7 +281,13 @@ def extras(torch, mcdx, L, stream, handle, ref):
             rp, rse = float.fromhex(ref[key]["price"]), float.fromhex(ref[key]["se"])
             entry.update(reference_mc_fp64=rp, reference_mc_se=rse, dev_in_reference_se=(qv - rp) / rse)
         qpaths[name] = entry
+    tq = OPS_QMC * (1 << 20) * STEPS / wall / 1e12
     res["c2_qmc_sobol_bb_asian"] = {"paths_per_s_wall": (1 << 20) / wall, "ms_wall": wall * 1e3,
+                                    "roofline": {"bound": "valu", "ops_per_point_date": OPS_QMC,
+                                                 "achieved": tq, "peak": PEAK_FP32_TFLOPS,
+                                                 "unit": "TFLOP/s", "frac": tq / PEAK_FP32_TFLOPS,
+                                                 "timing": "synchronous call, wall clock "
+                                                           "(kernel time in profiles/)"},
                                     "other_payoffs_2p20x252": qpaths,
                                     "price_fp32": q32, "price_fp64": q64,
                                     "randomized_16x2p16": {"price": rq, "se": rq_se,
@@ -281,8 +309,12 @@ def extras(torch, mcdx, L, stream, handle, ref):
     # FP64 mode on the same paths: the reference's arithmetic (per-path parity with the
     # oracle is tested), so the FP32-generation bias is measured path for path.
     r64 = mcdx.heston_strikes(par, ks, 0, STEPS, 1 << 22, SEED, mcdx.PREC_FP64)
+    th = OPS_HESTON * (1 << 22) * STEPS / wall / 1e12
     res["c3_heston_qe_9_strikes"] = {
         "paths_per_s_wall": (1 << 22) / wall, "path_steps_per_s_wall": (1 << 22) * STEPS / wall,
+        "roofline": {"bound": "valu", "ops_per_path_step": OPS_HESTON, "achieved": th,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": th / PEAK_FP32_TFLOPS,
+                     "timing": "synchronous call, wall clock (kernel time in profiles/)"},
         "ms_wall": wall * 1e3,
         "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
         "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)],
@@ -300,6 +332,34 @@ def extras(torch, mcdx, L, stream, handle, ref):
     res["c5_american_put_lsm"] = c5_lsm(mcdx, mcd)
     progress("C5 done")
     return res
+
+
+def book_ops(mcdx, cs):
+    """Algorithmic ops of one price book (shared-draw groups as the engine forms them: FP32
+    contracts of one seed, payoff family and step count, path counts descending, up to 8 per
+    group; a lone contract is a plain item)."""
+    fam = {mcdx.EUROPEAN: 0, mcdx.DIGITAL: 0, mcdx.ASIAN: 1, mcdx.BARRIER: 2, mcdx.LOOKBACK: 4}
+    buckets = {}
+    ops = 0.0
+    for c in cs:
+        m = 1 if c.product in (mcdx.EUROPEAN, mcdx.DIGITAL) else c.monitoring_points
+        if c.antithetic or (c.product == mcdx.BARRIER and c.brownian_bridge):
+            ops += c.path_count * m * (OPS_DRAW + OPS_MEMBER)
+            continue
+        buckets.setdefault((c.seed, fam[c.product], m), []).append(c.path_count)
+    for (_, _, m), counts in buckets.items():
+        counts.sort(reverse=True)
+        for o in range(0, len(counts), 8):
+            grp = counts[o:o + 8]
+            if len(grp) < 2:
+                ops += grp[0] * m * (OPS_DRAW + OPS_MEMBER)
+                continue
+            tiles = (grp[0] + 1023) // 1024
+            for t in range(tiles):  # members active on tile t share its draws
+                paths = min(1024, grp[0] - 1024 * t)
+                k = sum(1 for n in grp if n > 1024 * t)
+                ops += paths * m * (OPS_DRAW + OPS_MEMBER * k)
+    return ops
 
 
 def c4_entry(torch, mcdx, L, stream, handle):
@@ -350,6 +410,12 @@ def c4_entry(torch, mcdx, L, stream, handle):
     out["terminal_contracts_within_3se_of_closed_form"] = [ok, tot]
     steps = sum(c.path_count * (c.monitoring_points if c.product in
                                 (mcdx.ASIAN, mcdx.BARRIER, mcdx.LOOKBACK) else 1) for c in crn)
+    bops = book_ops(mcdx, crn)
+    tb = bops / (b_ms / 1e3) / 1e12
+    out["roofline"] = {"bound": "valu", "ops": bops, "achieved": tb, "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": tb / PEAK_FP32_TFLOPS,
+                       "model": "draw 113 per shared-draw path-step + 4 per member evaluated",
+                       "timing": "graph replay, HIP events"}
     out.update(seeds="one seed (42) for the book", batched_ms=b_ms, per_contract_ms=p_ms,
                speedup=p_ms / b_ms, paths=total, batched_paths_per_s=total / (b_ms / 1e3),
                contract_path_steps=steps,
@@ -530,6 +596,37 @@ def c5_lsm(mcdx, mcd):
     }
 
 
+def one_launch_each(torch, mcdx, L, stream, handle):
+    """One launch of every kernel family at its configuration's size (the C2 contracts, C1 in
+    FP64, the bridge-corrected barrier, Sobol QMC, Heston, the C4 book replay, C5 QR-LSM), for
+    per-dispatch counter passes."""
+    dev = torch.zeros((4, 3), dtype=torch.float64, device="cuda")
+    ran = []
+    for k, c in contracts(mcdx, N_PATHS).items():
+        mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c), mcdx.PREC_FP32, handle,
+                                            C.c_void_p(dev[0].data_ptr())), "enqueue")
+        ran.append(f"c2 {k}")
+    c1 = mcdx.contract(mcdx.EUROPEAN, path_count=1 << 24, seed=42)
+    mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(c1), mcdx.PREC_FP64, handle,
+                                        C.c_void_p(dev[1].data_ptr())), "enqueue")
+    bb = mcdx.contract(mcdx.BARRIER, barrier=120.0, monitoring_points=STEPS, path_count=N_PATHS,
+                       seed=SEED, brownian_bridge=1)
+    mcdx.check(L.mcdx_price_gbm_enqueue(C.byref(bb), mcdx.PREC_FP32, handle,
+                                        C.c_void_p(dev[2].data_ptr())), "enqueue")
+    torch.cuda.synchronize()
+    ran += ["c1 fp64 2^24", "c2 bridge barrier"]
+    mcdx.qmc_asian(100, 100, .05, 0, .2, 1, 0, 0, STEPS, 1 << 20, mcdx.PREC_FP32)
+    mcdx.heston_strikes(mcdx.heston(), [80.0 + 5 * i for i in range(9)], 0, STEPS, 1 << 22, SEED,
+                        mcdx.PREC_FP32)
+    book = mcdx.Book(mcdx.c4_book(mc_seed=SEED), mcdx.PREC_FP32)
+    book.enqueue(handle.value)
+    book.results()
+    book.close()
+    mcdx.lsm(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50, 1 << 20, SEED, mcdx.PREC_FP32)
+    ran += ["c2 qmc", "c3 heston", "c4 book", "c5 qr-lsm"]
+    return {"ran": ran}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -539,8 +636,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary configs")
-    ap.add_argument("--only", choices=["c4", "c5"],
-                    help="run one secondary config alone (profiling)")
+    ap.add_argument("--only", choices=["c4", "c5", "kernels"],
+                    help="run one secondary config alone, or one launch of every kernel family "
+                         "(kernels: for counter passes under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -566,6 +664,9 @@ def main():
         return 0
     if args.only == "c4":
         print(json.dumps({"c4_book_4096": c4_entry(torch, mcdx, L, stream, handle)}), flush=True)
+        return 0
+    if args.only == "kernels":
+        print(json.dumps(one_launch_each(torch, mcdx, L, stream, handle)), flush=True)
         return 0
     out = torch.zeros((len(names), 3), dtype=torch.float64, device="cuda")
 
