@@ -71,6 +71,18 @@ OPS_QMC = 53
 OPS_HESTON = 270
 OPS_MEMBER = 4
 
+# Lane-issue roofline of the three path kernels. A SIMD issues one wave64 VALU instruction
+# per 4 cycles (the issue probe measures the same rate for v_fma_f32, v_mad_u64_u32,
+# v_mul_hi_u32, v_bitop3_b32 and v_fma_f64, and half of it for v_exp_f32), so the issue slots
+# a kernel needs are rocprofv3's SQ_ACTIVE_INST_VALU (quad-cycles of VALU work, a
+# transcendental counting 2) less SQ_ACTIVE_INST_VALU2 (quad-cycles in which gfx950 issued a
+# second VALU instruction beside the first). Per wave-path-step of each 2^24 x 252 FP32
+# launch (counters / (2^24 x 252 / 64)): (VALU quad-cycles, paired quad-cycles).
+ISSUE_QUADS = {"asian": (58.961, 14.827), "barrier_uo": (56.643, 14.812),
+               "lookback_float": (56.626, 14.854)}
+SIMDS = 1024               # MI355X: 256 CUs x 4 SIMDs
+PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
+
 
 def contracts(mcdx, n):
     common = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.2, time=1.0,
@@ -596,6 +608,28 @@ def c5_lsm(mcdx, mcd):
     }
 
 
+def issue_roofline(mcdx, per_kernel_ms, paths):
+    """Lane-issue roofline of each path launch: the issue slots it needs (counter-derived,
+    per wave-path-step) over the slots the SIMDs offer at the peak clock in its measured time;
+    with the issue probe's sustained rate per instruction class, measured now."""
+    wave_steps = paths * STEPS / 64
+    per = {}
+    for k, ms in per_kernel_ms.items():
+        quads, paired = ISSUE_QUADS[k]
+        need_ms = (quads - paired) * 4 * wave_steps / SIMDS / PEAK_CLOCK_HZ * 1e3
+        per[k] = {"issue_limited_ms": round(need_ms, 4), "launch_ms": round(ms, 4),
+                  "frac": round(need_ms / ms, 4),
+                  "single_issue_frac": round(quads * 4 * wave_steps / SIMDS / PEAK_CLOCK_HZ *
+                                             1e3 / ms, 4)}
+    rates = mcdx.issue_rates(1 << 14)
+    slots = SIMDS * PEAK_CLOCK_HZ / 4
+    return {"bound": "valu issue", "unit": "wave64 VALU issue slots",
+            "peak_slots_per_s": slots, "per_launch": per,
+            "quad_cycles_per_wave_path_step": {k: v[0] for k, v in ISSUE_QUADS.items()},
+            "paired_per_wave_path_step": {k: v[1] for k, v in ISSUE_QUADS.items()},
+            "probe_instructions_per_slot": {k: round(v / slots, 4) for k, v in rates.items()}}
+
+
 def one_launch_each(torch, mcdx, L, stream, handle):
     """One launch of every kernel family at its configuration's size (the C2 contracts, C1 in
     FP64, the bridge-corrected barrier, Sobol QMC, Heston, the C4 book replay, C5 QR-LSM), for
@@ -636,9 +670,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 bias leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary configs")
-    ap.add_argument("--only", choices=["c4", "c5", "kernels"],
-                    help="run one secondary config alone, or one launch of every kernel family "
-                         "(kernels: for counter passes under rocprofv3)")
+    ap.add_argument("--only", choices=["c4", "c5", "kernels", "issue"],
+                    help="run one secondary config alone, one launch of every kernel family "
+                         "(kernels: for counter passes under rocprofv3), or the VALU issue "
+                         "probe of every instruction class (issue)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -664,6 +699,10 @@ def main():
         return 0
     if args.only == "c4":
         print(json.dumps({"c4_book_4096": c4_entry(torch, mcdx, L, stream, handle)}), flush=True)
+        return 0
+    if args.only == "issue":
+        rates = mcdx.issue_rates(1 << 14)
+        print(json.dumps({"wave64_instructions_per_s": rates}), flush=True)
         return 0
     if args.only == "kernels":
         print(json.dumps(one_launch_each(torch, mcdx, L, stream, handle)), flush=True)
@@ -740,6 +779,7 @@ def main():
         "per_launch_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "traffic": TRAFFIC_BYTES[dom] if args.paths == N_PATHS else None,
         "traffic_unit": "bytes per launch",
+        "issue": issue_roofline(mcdx, per_kernel_ms, args.paths),
     }
 
     value = len(names) * args.paths / (ms_per_step / 1e3)

@@ -90,6 +90,7 @@ SIGNATURES = {
     "mcdx_normal_draws": (C.c_int, [u64, u64, u64, u32, C.c_int, P(dbl)]),
     "mcdx_inv_normal_f32_scan": (C.c_int, [u64, u64, P(dbl), P(u32)]),
     "mcdx_inv_normal_words": (C.c_int, [P(u32), u64, C.c_int, P(dbl)]),
+    "mcdx_issue_probe": (C.c_int, [C.c_int, u32, P(dbl), P(u64)]),
     "mcdx_monte_carlo_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Options), P(Result)]),
     "mcdx_monte_carlo_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Options),
                                                        P(Result)]),
@@ -200,6 +201,20 @@ def normal_draws(seed, path0, n, draw, precision):
     out = np.empty(n, dtype=np.float64)
     check(lib().mcdx_normal_draws(seed, path0, n, draw, precision, out.ctypes.data_as(P(dbl))),
           "mcdx_normal_draws")
+    return out
+
+
+ISSUE_CLASSES = {"fma_f32": 0, "mad_u64_u32": 1, "mul_hi_u32": 2, "xor3": 3, "exp2_f32": 4,
+                 "fma_f64": 5, "mix_fma_mad": 6, "mix_fma_exp2": 7}
+
+
+def issue_rates(iterations=1 << 14):
+    """Sustained wave64 instructions per second for each probed class, whole GPU."""
+    out = {}
+    for name, cls in ISSUE_CLASSES.items():
+        sec, wi = dbl(), u64()
+        check(lib().mcdx_issue_probe(cls, iterations, C.byref(sec), C.byref(wi)), "mcdx_issue_probe")
+        out[name] = wi.value / sec.value
     return out
 
 

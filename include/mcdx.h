@@ -101,6 +101,21 @@ int mcdx_inv_normal_words(const uint32_t* words, uint64_t n, int precision, doub
 int mcdx_inv_normal_f32_scan(uint64_t begin, uint64_t count, double max_err[2],
                              uint32_t argmax_bits[2]);
 
+/* ---- VALU issue-rate probe (no reference counterpart) ----
+ * Times `iterations` steps of eight independent chains of one instruction class in every lane
+ * of 8 waves per SIMD (all SIMDs) and reports the seconds and the wave64 instructions issued:
+ * the class's sustained issue rate, the peak of the path kernels' lane-issue roofline. */
+#define MCDX_ISSUE_FMA_F32 0     /* v_fma_f32 */
+#define MCDX_ISSUE_MAD_U64_U32 1 /* 32x32 -> 64-bit integer multiply-add (Philox products) */
+#define MCDX_ISSUE_MUL_HI_U32 2  /* high word of a 32x32 product */
+#define MCDX_ISSUE_XOR3 3        /* three-input XOR */
+#define MCDX_ISSUE_EXP2_F32 4    /* transcendental exp2 */
+#define MCDX_ISSUE_FMA_F64 5     /* v_fma_f64 */
+#define MCDX_ISSUE_MIX_FMA_MAD 6 /* alternate chains of classes 0 and 1 */
+#define MCDX_ISSUE_MIX_FMA_EXP2 7 /* alternate chains of classes 0 and 4 */
+int mcdx_issue_probe(int instruction_class, uint32_t iterations, double* seconds,
+                     uint64_t* wave_instructions);
+
 /* ---- GBM Monte Carlo pricers (src/pricers/monte_carlo.cpp:234-338) ---- */
 int mcdx_monte_carlo_european(double spot, double strike, double rate, double carry_yield,
                               double vol, double time, uint8_t type, uint64_t path_count,
