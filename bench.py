@@ -100,7 +100,8 @@ def cpu_baseline(budget_s=12.0):
     the C oracle where the reference build is absent, on a bounded sample of the workload."""
     import oracle_lib as O
     cores = len(os.sched_getaffinity(0))
-    R = O.ref()
+    R = O.ref_o3()
+    flags = "-O3 -march=x86-64-v3" if R is not None and R is not O.ref() else "-O2"
     res = O.Result()
 
     def run(n):
@@ -124,18 +125,22 @@ def cpu_baseline(budget_s=12.0):
     return {"value": 3 * n2 / dt2, "unit": "paths/s", "cores": cores,
             "kind": "reference" if R is not None else "port",
             "sample": f"3 contracts x {n2} paths x {STEPS} steps (same contracts, seed {SEED}), "
-                      f"{dt2:.1f} s, num_threads={cores}"}
+                      f"{dt2:.1f} s, num_threads={cores}, built {flags}"}
 
 
 def cpu_reference_configs(ex, budget_s=2.5):
-    """The reference's own CPU pricers (oracle/_ref) on bounded samples of the C1, C3 and C5
-    workloads, beside the GPU figures: C1 multithreaded (the reference's thread pool), Heston
-    QE and LSM single-threaded (the reference's pricers for those are serial). Each sample is
-    scaled from a probe to about budget_s seconds. Returns None without the reference build."""
+    """CPU figures beside the GPU ones for C1, C3 and C5 on bounded samples, each scaled from
+    a probe to about budget_s seconds: the reference's own pricers (oracle/_ref, release -O3
+    build) -- C1 on its thread pool, Heston QE and LSM single-threaded as the reference runs
+    them -- and, for Heston and LSM, the oracle's path-parallel restatements on all cores
+    (Heston: the 9 strikes from one path pass, as the GPU prices them; LSM: per-chunk QR blocks
+    stacked in a fixed order). Returns None without the reference build."""
     import oracle_lib as O
-    R = O.ref()
+    R = O.ref_o3()
     if R is None:
         return None
+    OL = O.lib_o3()
+    flags = "-O3 -march=x86-64-v3" if R is not O.ref() else "-O2"
     cores = len(os.sched_getaffinity(0))
     res = O.Result()
 
@@ -151,26 +156,45 @@ def cpu_reference_configs(ex, budget_s=2.5):
     out = {}
     n, dt = scaled(lambda n: R.ref_mc_european(100, 100, .05, 0, .2, 1, 0, n, SEED, 0, cores,
                                                C.byref(res)), 1 << 16, 1 << 30)
-    out["c1_european"] = {"value": n / dt, "unit": "paths/s", "cores": cores,
-                          "sample": f"{n} paths, num_threads={cores}, {dt:.2f} s"}
+    out["c1_european"] = {"value": n / dt, "unit": "paths/s", "cores": cores, "kind": "reference",
+                          "sample": f"{n} paths, num_threads={cores}, built {flags}, {dt:.2f} s"}
     p9 = (C.c_double * 9)(100.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
     n, dt = scaled(lambda n: R.ref_heston_qe(p9, 100.0, 0, STEPS, n, SEED, C.byref(res)),
                    1 << 10, 1 << 22)
     out["c3_heston_qe"] = {"value": n / dt, "unit": "paths/s (one strike)", "cores": 1,
+                           "kind": "reference",
                            "sample": f"{n} paths x {STEPS} steps, strike 100, single-threaded "
-                                     f"(the reference's Heston pricer), {dt:.2f} s"}
+                                     f"(the reference's Heston pricer), built {flags}, {dt:.2f} s"}
+    h = O.Heston(100.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7, 1.0)
+    ks = (C.c_double * 9)(*[80.0 + 5 * i for i in range(9)])
+    res9 = (O.Result * 9)()
+    n, dt = scaled(lambda n: OL.orc_heston_qe_multi(C.byref(h), ks, 9, 0, STEPS, n, SEED, cores,
+                                                    res9), 1 << 12, 1 << 22)
+    out["c3_heston_qe_path_parallel"] = {
+        "value": n / dt, "unit": "paths/s (9 strikes per path)", "cores": cores, "kind": "port",
+        "sample": f"{n} paths x {STEPS} steps x 9 strikes, {cores} threads (the oracle's "
+                  f"path-parallel Heston QE), built {flags}, {dt:.2f} s"}
     co = (C.c_double * (51 * 4))()
     has = (C.c_uint8 * 51)()
     inc = C.c_uint8()
     n, dt = scaled(lambda n: R.ref_lsm(100, 100, .06, 0, .2, 1, 1, 50, n, SEED, C.byref(res),
                                        co, has, C.byref(inc)), 1 << 12, 1 << 20)
-    out["c5_lsm"] = {"value": n / dt, "unit": "paths/s", "cores": 1,
+    out["c5_lsm"] = {"value": n / dt, "unit": "paths/s", "cores": 1, "kind": "reference",
                      "sample": f"{n} paths x 50 dates, single-threaded (the reference's LSM "
-                               f"pricer), {dt:.2f} s"}
+                               f"pricer), built {flags}, {dt:.2f} s"}
+    n, dt = scaled(lambda n: OL.orc_lsm_american_mt(100, 100, .06, 0, .2, 1, 1, 50, n, SEED,
+                                                    cores, co, has, C.byref(inc)),
+                   1 << 12, 1 << 20)
+    out["c5_lsm_path_parallel"] = {
+        "value": n / dt, "unit": "paths/s", "cores": cores, "kind": "port",
+        "sample": f"{n} paths x 50 dates, {cores} threads (the oracle's path-parallel LSM: "
+                  f"per-chunk QR blocks stacked in thread order), built {flags}, {dt:.2f} s"}
     if ex:
+        c3 = ex.get("c3_heston_qe_9_strikes", {}).get("paths_per_s_wall")
+        c5 = ex.get("c5_american_put_lsm", {}).get("qr_lsm", {}).get("paths_per_s")
         gpu = {"c1_european": ex.get("c1_european_1000000", {}).get("paths_per_s"),
-               "c3_heston_qe": ex.get("c3_heston_qe_9_strikes", {}).get("paths_per_s_wall"),
-               "c5_lsm": ex.get("c5_american_put_lsm", {}).get("qr_lsm", {}).get("paths_per_s")}
+               "c3_heston_qe": c3, "c3_heston_qe_path_parallel": c3,
+               "c5_lsm": c5, "c5_lsm_path_parallel": c5}
         for k, g in gpu.items():
             if g:
                 out[k]["gpu_paths_per_s"] = g

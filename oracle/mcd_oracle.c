@@ -1058,6 +1058,192 @@ double orc_lsm_reprice_frozen(double spot, double strike, double rate, double ca
     return acc.mean;
 }
 
+/* Path-parallel LSM (a CPU baseline on all cores; the reference's pricer is serial). The same
+ * backward induction as orc_lsm_american with the paths split into `threads` contiguous chunks:
+ * each thread generates its chunk, and at each date factors its in-the-money rows by
+ * Householder QR into R (4 x 4) and the top four entries of Q^T y; thread 0 stacks those
+ * blocks in thread order (raw rows for a chunk with fewer than four) and solves the stacked
+ * least squares. That is the same least-squares solution up to rounding (a tall-skinny QR),
+ * so the price agrees with the serial one except where rounding flips an exercise decision. */
+typedef struct {
+    double spot, strike, rate, carry, vol, time;
+    int type, steps;
+    uint64_t seed;
+    unsigned threads;
+    uint64_t* begin;
+    uint64_t* end;
+    double* paths;       /* chunk rows in path order: n x (steps + 1) */
+    double* cf;
+    double* blocks;      /* per thread: up to 4 rows x 4 + 4 targets (R, Q^T y or raw rows) */
+    int* block_rows;
+    uint64_t* itm_count;
+    double* beta;        /* (steps + 1) x 4 */
+    uint8_t* has;
+    orc_welford* acc;
+    pthread_barrier_t bar;
+} lsm_mt_job;
+
+typedef struct {
+    lsm_mt_job* job;
+    unsigned t;
+} lsm_mt_arg;
+
+static void* lsm_mt_worker(void* p) {
+    const lsm_mt_arg* a = (const lsm_mt_arg*)p;
+    lsm_mt_job* j = a->job;
+    const unsigned t = a->t;
+    const int steps = j->steps;
+    const size_t stride = (size_t)steps + 1;
+    const double dt = j->time / (double)steps;
+    const double disc = exp(-j->rate * dt);
+    const uint64_t b = j->begin[t], e = j->end[t], m = e - b;
+    for (uint64_t i = b; i < e; ++i) { /* lsm.cpp:35-58 on this chunk */
+        double s = j->spot;
+        double* row = &j->paths[i * stride];
+        row[0] = s;
+        for (int k = 0; k < steps; ++k) {
+            s = orc_gbm_terminal_spot(s, j->rate, j->carry, j->vol, dt,
+                                      orc_standard_normal_variate(j->seed, i, (uint32_t)k));
+            row[k + 1] = s;
+        }
+        j->cf[i] = intrinsic(s, j->strike, j->type);
+    }
+    double* design = (double*)malloc(sizeof(double) * (size_t)(m ? m : 1) * 4);
+    double* targets = (double*)malloc(sizeof(double) * (size_t)(m ? m : 1));
+    uint64_t* itm = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(m ? m : 1));
+    for (int step = steps - 1; step >= 1; --step) {
+        uint64_t cnt = 0;
+        for (uint64_t i = b; i < e; ++i) {
+            j->cf[i] *= disc;
+            const double s = j->paths[i * stride + (size_t)step];
+            if (intrinsic(s, j->strike, j->type) > 0.0) {
+                laguerre(s / j->strike, &design[cnt * 4]);
+                targets[cnt] = j->cf[i];
+                itm[cnt++] = i;
+            }
+        }
+        double* blk = &j->blocks[(size_t)t * 20];
+        if (cnt >= 4) {
+            double* a2 = (double*)malloc(sizeof(double) * (size_t)cnt * 4);
+            double* y2 = (double*)malloc(sizeof(double) * (size_t)cnt);
+            memcpy(a2, design, sizeof(double) * (size_t)cnt * 4);
+            memcpy(y2, targets, sizeof(double) * (size_t)cnt);
+            double unused[4];
+            orc_householder_ls(a2, (int)cnt, 4, y2, unused);
+            for (int r = 0; r < 4; ++r) {
+                for (int c = 0; c < 4; ++c) blk[r * 4 + c] = c >= r ? a2[r * 4 + c] : 0.0;
+                blk[16 + r] = y2[r];
+            }
+            j->block_rows[t] = 4;
+            free(a2);
+            free(y2);
+        } else {
+            for (uint64_t r = 0; r < cnt; ++r) {
+                for (int c = 0; c < 4; ++c) blk[r * 4 + c] = design[r * 4 + c];
+                blk[16 + r] = targets[r];
+            }
+            j->block_rows[t] = (int)cnt;
+        }
+        j->itm_count[t] = cnt;
+        pthread_barrier_wait(&j->bar);
+        if (t == 0) {
+            uint64_t total = 0;
+            int rows = 0;
+            for (unsigned u = 0; u < j->threads; ++u) {
+                total += j->itm_count[u];
+                rows += j->block_rows[u];
+            }
+            j->has[step] = total >= 4;
+            if (j->has[step]) {
+                double* a2 = (double*)malloc(sizeof(double) * (size_t)rows * 4);
+                double* y2 = (double*)malloc(sizeof(double) * (size_t)rows);
+                int r0 = 0;
+                for (unsigned u = 0; u < j->threads; ++u) {
+                    const double* bu = &j->blocks[(size_t)u * 20];
+                    for (int r = 0; r < j->block_rows[u]; ++r, ++r0) {
+                        for (int c = 0; c < 4; ++c) a2[r0 * 4 + c] = bu[r * 4 + c];
+                        y2[r0] = bu[16 + r];
+                    }
+                }
+                orc_householder_ls(a2, rows, 4, y2, &j->beta[(size_t)step * 4]);
+                free(a2);
+                free(y2);
+            }
+        }
+        pthread_barrier_wait(&j->bar);
+        if (!j->has[step]) continue;
+        const double* beta = &j->beta[(size_t)step * 4];
+        for (uint64_t q = 0; q < cnt; ++q) {
+            const uint64_t i = itm[q];
+            const double s = j->paths[i * stride + (size_t)step];
+            double basis[4];
+            laguerre(s / j->strike, basis);
+            double cont = 0.0;
+            for (int c = 0; c < 4; ++c) cont += beta[c] * basis[c];
+            const double ex = intrinsic(s, j->strike, j->type);
+            if (ex > cont) j->cf[i] = ex;
+        }
+    }
+    orc_welford w = {0, 0.0, 0.0};
+    for (uint64_t i = b; i < e; ++i) orc_welford_add(&w, j->cf[i] * disc);
+    j->acc[t] = w;
+    free(design);
+    free(targets);
+    free(itm);
+    return NULL;
+}
+
+orc_result orc_lsm_american_mt(double spot, double strike, double rate, double carry,
+                               double vol, double time, int type, int steps, uint64_t n,
+                               uint64_t seed, unsigned threads, double* coeffs,
+                               uint8_t* has_coeffs, uint8_t* exercise_at_inception) {
+    if (threads == 0) threads = 1;
+    const size_t stride = (size_t)steps + 1;
+    lsm_mt_job j;
+    memset(&j, 0, sizeof j);
+    j.spot = spot; j.strike = strike; j.rate = rate; j.carry = carry; j.vol = vol;
+    j.time = time; j.type = type; j.steps = steps; j.seed = seed; j.threads = threads;
+    j.begin = (uint64_t*)calloc(threads, sizeof(uint64_t));
+    j.end = (uint64_t*)calloc(threads, sizeof(uint64_t));
+    j.paths = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1) * stride);
+    j.cf = (double*)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    j.blocks = (double*)calloc((size_t)threads * 20, sizeof(double));
+    j.block_rows = (int*)calloc(threads, sizeof(int));
+    j.itm_count = (uint64_t*)calloc(threads, sizeof(uint64_t));
+    j.beta = coeffs;
+    j.has = has_coeffs;
+    j.acc = (orc_welford*)calloc(threads, sizeof(orc_welford));
+    for (size_t s = 0; s < stride; ++s) has_coeffs[s] = 0;
+    for (unsigned t = 0; t < threads; ++t) orc_chunk_bounds(t, n, threads, &j.begin[t], &j.end[t]);
+    pthread_barrier_init(&j.bar, NULL, threads);
+    pthread_t* tids = (pthread_t*)calloc(threads, sizeof(pthread_t));
+    lsm_mt_arg* args = (lsm_mt_arg*)calloc(threads, sizeof(lsm_mt_arg));
+    for (unsigned t = 0; t < threads; ++t) {
+        args[t] = (lsm_mt_arg){&j, t};
+        if (t > 0) pthread_create(&tids[t], NULL, lsm_mt_worker, &args[t]);
+    }
+    lsm_mt_worker(&args[0]);
+    for (unsigned t = 1; t < threads; ++t) pthread_join(tids[t], NULL);
+    pthread_barrier_destroy(&j.bar);
+    orc_welford acc = {0, 0.0, 0.0};
+    for (unsigned t = 0; t < threads; ++t) orc_welford_merge(&acc, &j.acc[t]);
+    free(tids); free(args); free(j.begin); free(j.end); free(j.paths); free(j.cf);
+    free(j.blocks); free(j.block_rows); free(j.itm_count); free(j.acc);
+    const double inc = intrinsic(spot, strike, type);
+    orc_result r;
+    r.path_count = n;
+    if (inc > acc.mean) {
+        *exercise_at_inception = 1;
+        r.price = inc;
+        r.standard_error = 0.0;
+        return r;
+    }
+    *exercise_at_inception = 0;
+    r.price = acc.mean;
+    r.standard_error = orc_welford_se(&acc);
+    return r;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Analytic: src/pricers/analytic.cpp:12-235                                 */
 /* ------------------------------------------------------------------------ */

@@ -136,6 +136,13 @@ orc_result orc_lsm_american_qmc(double spot, double strike, double rate, double 
                                 uint64_t path_count, int shifted, uint64_t shift_seed,
                                 uint32_t replicate, double* coeffs, uint8_t* has_coeffs,
                                 uint8_t* exercise_at_inception);
+/* orc_lsm_american with the paths split over `threads` (a CPU baseline): per-chunk QR blocks
+ * stacked in thread order; equal to the serial price up to rounding-flipped decisions. */
+orc_result orc_lsm_american_mt(double spot, double strike, double rate, double carry,
+                               double vol, double time, int type, int steps,
+                               uint64_t path_count, uint64_t seed, unsigned threads,
+                               double* coeffs, uint8_t* has_coeffs,
+                               uint8_t* exercise_at_inception);
 double orc_lsm_reprice_frozen(double spot, double strike, double rate, double carry, double vol,
                               double time, int type, int steps, uint64_t path_count,
                               uint64_t seed, const double* coeffs, const uint8_t* has_coeffs,

@@ -399,6 +399,26 @@ def test_lsm_matches_reference_fixture_bits():
     assert all(x == fh(y) for x, y in zip(co.ravel(), g["coeffs"]))
 
 
+def test_path_parallel_lsm_agrees_with_the_serial_oracle():
+    # one thread is the serial algorithm with one extra QR of R: the same coefficients to
+    # rounding; several threads stack per-chunk R factors (tall-skinny QR). The least-squares
+    # solution is unchanged up to rounding times the design's condition number (about 2e6
+    # at the first dates, where coefficients near 2e4 cancel), so coefficients agree to 1e-7
+    # relative and the price to the exercise decisions rounding flips (none at these sizes).
+    g = GOLD["lsm"]["args"]
+    p, se, co, has, inc = O.lsm(*g)
+    for threads in (1, 3, 8):
+        p2, se2, co2, has2, inc2 = O.lsm_mt(*g, threads)
+        assert has2.tolist() == has.tolist() and inc2 == inc
+        assert np.allclose(co2, co, rtol=1e-7, atol=1e-9)
+        assert abs(p2 - p) < 1e-12 * abs(p) + 1e-14 and abs(se2 - se) < 1e-9 * se
+    # a chunk can hold fewer than four in-the-money rows (deep out of the money)
+    p, _, co, has, _ = O.lsm(150, 100, .06, 0, .2, 1, 1, 10, 64, 5)
+    p2, _, co2, has2, _ = O.lsm_mt(150, 100, .06, 0, .2, 1, 1, 10, 64, 5, 8)
+    assert has2.tolist() == has.tolist() and np.allclose(co2, co, rtol=1e-7, atol=1e-9)
+    assert abs(p2 - p) < 1e-12
+
+
 def test_crr_american_fixture_and_european_limit():
     for n, v in GOLD["crr_american"].items():
         assert L.orc_crr_american(100, 100, .06, 0, .2, 1, int(n), 1) == fh(v)
@@ -533,3 +553,31 @@ def test_oracle_equals_live_reference_on_random_cases():
             for path in rng.integers(0, 1 << 40, 50):
                 assert L.orc_standard_normal_variate(3, int(path), draw) == \
                     R.ref_standard_normal_variate(3, int(path), draw)
+
+
+def test_o3_timing_builds_compute_the_same_bits():
+    # The CPU baseline times -O3 x86-64-v3 builds; contraction stays off, so they must
+    # reproduce the -O2 builds' results exactly (else the baseline would time other work).
+    if not O.fast_isa():
+        pytest.skip("host lacks the x86-64-v3 features the timing builds need")
+    F = O.lib_o3()
+    assert F is not L
+    for args in [(100, 100, .05, 0, .2, 1, 0, 4096, 42), (90, 100, .03, .01, .3, .5, 1, 999, 7)]:
+        prod = O.product(O.ASIAN, spot=args[0], strike=args[1], rate=args[2],
+                         carry_yield=args[3], vol=args[4], time=args[5], type=args[6],
+                         monitoring_points=13)
+        a = [lib_.orc_price(C.byref(prod), args[7], args[8], 4, 4) for lib_ in (L, F)]
+        assert (a[0].price, a[0].standard_error) == (a[1].price, a[1].standard_error)
+    g = GOLD["lsm"]["args"]
+    co = np.zeros(((g[7] + 1) * 4,))
+    has = np.zeros(g[7] + 1, dtype=np.uint8)
+    inc = C.c_uint8()
+    r = F.orc_lsm_american(*g, co.ctypes.data_as(C.POINTER(C.c_double)),
+                           has.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(inc))
+    assert r.price == fh(GOLD["lsm"]["price"])
+    R, R3 = O.ref(), O.ref_o3()
+    if R is not None and R3 is not R:
+        res = [O.Result(), O.Result()]
+        for lib_, rr in zip((R, R3), res):
+            lib_.ref_mc_asian(100, 100, .05, 0, .2, 1, 0, 0, 0, 52, 5000, 42, 0, 0, 4, C.byref(rr))
+        assert (res[0].price, res[0].standard_error) == (res[1].price, res[1].standard_error)
