@@ -702,7 +702,7 @@ hipError_t enqueue_raw(mcdx_book& b, hipStream_t s) {
         if (le != hipSuccess) return le;
         off += b.n_fused[f];
     }
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e != hipSuccess) return e;
     return launch_final_merge(b.d_partials, b.d_toff, 0, b.slots, b.d_results, s);
 }
@@ -859,13 +859,13 @@ int mcdx_book_create_ex(const mcdx_gbm_contract* contracts, uint32_t n, int prec
     const size_t ng = std::max<size_t>(1, greeks ? n : n / 2 + 1);
 
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&b->d_args, slots * sizeof(GbmArgs));
-    if (e == hipSuccess) e = hipMalloc(&b->d_kind, slots);
-    if (e == hipSuccess) e = hipMalloc(&b->d_toff, (slots + 1) * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&b->d_items64, acc * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&b->d_groups, ng * sizeof(FusedGroup));
-    if (e == hipSuccess) e = hipMalloc(&b->d_partials, acc * sizeof(Welford));
-    if (e == hipSuccess) e = hipMalloc(&b->d_results, slots * sizeof(Welford));
+    if (e == hipSuccess) e = dev_malloc(&b->d_args, slots * sizeof(GbmArgs));
+    if (e == hipSuccess) e = dev_malloc(&b->d_kind, slots);
+    if (e == hipSuccess) e = dev_malloc(&b->d_toff, (slots + 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = dev_malloc(&b->d_items64, acc * sizeof(uint64_t));
+    if (e == hipSuccess) e = dev_malloc(&b->d_groups, ng * sizeof(FusedGroup));
+    if (e == hipSuccess) e = dev_malloc(&b->d_partials, acc * sizeof(Welford));
+    if (e == hipSuccess) e = dev_malloc(&b->d_results, slots * sizeof(Welford));
     if (e == hipSuccess) e = hipMemcpy(b->d_args, args.data(), slots * sizeof(GbmArgs), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_kind, kinds.data(), slots, hipMemcpyHostToDevice);
     if (e == hipSuccess)

@@ -107,7 +107,7 @@ hipError_t enqueue_contract(const mcdx_gbm_contract& c, int precision, Welford* 
         if (anti) launch_tiles_k<false, true>(kind, a, (unsigned)tiles, s);
         else launch_tiles_k<false, false>(kind, a, (unsigned)tiles, s);
     }
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e != hipSuccess) return e;
     return launch_final_merge(partials, nullptr, tiles, 1, result, s);
 }
@@ -217,8 +217,11 @@ int mcdx_price_gbm_enqueue(const mcdx_gbm_contract* c, int precision, void* stre
     void* scratch;
     if ((st = enqueue_scratch(ctx, s, tiles * sizeof(Welford), &scratch))) return st;
     hipError_t e = enqueue_contract(*c, precision, (Welford*)scratch, (Welford*)dev_welford, s);
+    // recorded whatever was enqueued, so a later user of the scratch waits for it even when a
+    // launch reported an error after an earlier kernel of this call was queued
+    const int done = enqueue_scratch_done(ctx, s);
     if (e != hipSuccess) return status_of(e);
-    return enqueue_scratch_done(ctx, s);
+    return done;
 }
 
 int mcdx_price_gbm_enqueue_noalloc(const mcdx_gbm_contract* c, int precision, void* stream,
@@ -261,7 +264,7 @@ int mcdx_gbm_path_samples(const mcdx_gbm_contract* c, int precision, uint64_t pa
         if (anti) launch_samples_k<false, true>(kind, a, n, ctx->stream);
         else launch_samples_k<false, false>(kind, a, n, ctx->stream);
     }
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, ctx->dev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

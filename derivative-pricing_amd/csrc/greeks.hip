@@ -240,7 +240,7 @@ int run_greeks(int kind, GreekArgs a, mcdx_result* out) {
         case kLrBarrier: hipLaunchKernelGGL((greek_tile_kernel<kLrBarrier, 4>), g, b, 0, ctx->stream, a); break;
         default: hipLaunchKernelGGL((greek_tile_kernel<kFdEuropean, 9>), g, b, 0, ctx->stream, a); break;
         }
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess) e = launch_final_merge(partials, offs, tiles, (uint32_t)nacc, results, ctx->stream);
     Welford* h_res = (Welford*)((char*)ctx->pinned + 64 + offs_bytes);

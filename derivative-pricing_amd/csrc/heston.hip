@@ -251,7 +251,7 @@ int heston_price(const mcdx_heston_params* p, const double* strikes, int32_t n_s
         else
             hipLaunchKernelGGL(heston_tile_kernel<false>, dim3((unsigned)tiles), dim3(kTileLanes), 0,
                                ctx->stream, a);
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess)
         e = launch_final_merge(partials, offs, tiles, (uint32_t)n_strikes, results, ctx->stream);
@@ -311,7 +311,7 @@ int mcdx_heston_path_samples(const mcdx_heston_params* params, double strike, ui
             hipLaunchKernelGGL(heston_samples_kernel<true>, dim3(grid), dim3(256), 0, ctx->stream, a, n);
         else
             hipLaunchKernelGGL(heston_samples_kernel<false>, dim3(grid), dim3(256), 0, ctx->stream, a, n);
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, a.samples, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);

@@ -133,7 +133,7 @@ int mcdx_issue_probe(int instruction_class, uint32_t iterations, double* seconds
     e = hipEventRecord(a, ctx->stream);
     if (e == hipSuccess) {
         run(iterations);
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);

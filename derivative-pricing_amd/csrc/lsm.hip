@@ -1028,7 +1028,7 @@ template <bool FP64>
 hipError_t enqueue_init(const LsmRun& r, hipStream_t s) {
     const unsigned g = (unsigned)r.blocks;
     hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g, r.runs), dim3(256), 0, s, r.a);
-    return hipGetLastError();
+    return launch_check();
 }
 
 template <bool FP64>
@@ -1036,7 +1036,7 @@ hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
     if (r.runs != 1) return hipErrorInvalidValue; // Philox paths: one run at a time
     const unsigned g = (unsigned)r.blocks;
     hipLaunchKernelGGL(lsm_paths_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     return e == hipSuccess ? enqueue_init<FP64>(r, s) : e;
 }
 
@@ -1061,7 +1061,7 @@ hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
                        fit ? 1 : 0);
     hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles, r.runs), dim3(kTileLanes), 0,
                        s, r.a);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e != hipSuccess) return e;
     // run k's result at partials[runs x tiles + k]
     return launch_final_merge(r.a.partials, nullptr, r.tiles, r.runs,
@@ -1288,10 +1288,10 @@ int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_
     else
         hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTilePaths),
                            has_lds_bytes(m), s, r.a, q);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(lsm_count_kernel, dim3(1), dim3(256), 0, s, b.tile_counts, r.tiles, b.ctr);
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess) e = launch_final_merge(b.pn, nullptr, r.tiles, 1, b.pn + r.tiles, s);
     if (e == hipSuccess) e = launch_final_merge(b.pq, nullptr, r.tiles, 1, b.pq + r.tiles, s);
@@ -1400,7 +1400,7 @@ int lsm_am_greeks(double spot, double strike, double rate, double carry, double 
     else
         hipLaunchKernelGGL(lsm_am_greeks_kernel<false>, dim3((unsigned)tiles), dim3(kTilePaths),
                            has_lds_bytes(m), s, r.a, g);
-    e = hipGetLastError();
+    e = launch_check();
     for (int q = 0; q < 4 && e == hipSuccess; ++q) {
         Welford* part = g.parts + (uint64_t)q * (tiles + 1);
         e = launch_final_merge(part, nullptr, tiles, 1, part + tiles, s);

@@ -424,7 +424,7 @@ size_t lds_bytes() { return sizeof(MlpLds); }
 int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s) {
     hipLaunchKernelGGL(mlp_init_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, w, seed,
                        y_mean);
-    return status_of(hipGetLastError());
+    return status_of(launch_check());
 }
 
 int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
@@ -437,7 +437,7 @@ int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(mlp_train_kernel, dim3(1), dim3(kTrainLanes), bytes, s, a);
-    return status_of(hipGetLastError());
+    return status_of(launch_check());
 }
 
 int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
@@ -448,7 +448,7 @@ int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n
     if (e != hipSuccess) return status_of(e);
     hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), lds_bytes(),
                        s, w, xs, ts, n, out);
-    return status_of(hipGetLastError());
+    return status_of(launch_check());
 }
 
 } // namespace mcdx

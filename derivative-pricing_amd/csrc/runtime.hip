@@ -30,8 +30,26 @@ struct CtxReaper {
 thread_local CtxReaper t_reaper;
 } // namespace
 
+namespace {
+thread_local int32_t t_fail_alloc = 0;  // countdown to an injected allocation failure (0: off)
+thread_local int32_t t_fail_launch = 0; // countdown to an injected launch failure (0: off)
+// Set when a call on this thread returned a HIP error: work it queued before the failure may
+// still be in flight on the thread's stream (and its pinned staging), so the next call
+// drains that stream before reusing anything.
+thread_local bool t_unsettled = false;
+} // namespace
+
+bool inject_alloc_failure() { return t_fail_alloc > 0 && --t_fail_alloc == 0; }
+
+hipError_t launch_check() {
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess && t_fail_launch > 0 && --t_fail_launch == 0) return hipErrorLaunchFailure;
+    return e;
+}
+
 int status_of(hipError_t e) {
     if (e == hipSuccess) return MCDX_OK;
+    t_unsettled = true;
     if (e == hipErrorOutOfMemory) return MCDX_ENOMEM;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return MCDX_ENODEV;
     return MCDX_EDEVICE;
@@ -45,6 +63,10 @@ int thread_ctx(ThreadCtx** ctx) {
         hipError_t e = hipStreamCreateWithFlags(&t_ctx.stream, hipStreamNonBlocking);
         if (e != hipSuccess) return status_of(e);
         t_ready = true;
+    }
+    if (t_unsettled) {
+        t_unsettled = false;
+        (void)hipStreamSynchronize(t_ctx.stream);
     }
     *ctx = &t_ctx;
     return MCDX_OK;
@@ -60,7 +82,7 @@ int ensure_dev(ThreadCtx* ctx, size_t bytes) {
         ctx->dev_bytes = 0;
     }
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
-    hipError_t e = hipMalloc(&ctx->dev, want);
+    hipError_t e = dev_malloc(&ctx->dev, want);
     if (e != hipSuccess) return status_of(e);
     ctx->dev_bytes = want;
     return MCDX_OK;
@@ -76,7 +98,8 @@ int ensure_pinned(ThreadCtx* ctx, size_t bytes) {
         ctx->pinned_bytes = 0;
     }
     size_t want = bytes < 4096 ? 4096 : bytes;
-    hipError_t e = hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault);
+    hipError_t e = inject_alloc_failure() ? hipErrorOutOfMemory
+                                          : hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault);
     if (e != hipSuccess) return status_of(e);
     ctx->pinned_bytes = want;
     return MCDX_OK;
@@ -98,7 +121,7 @@ int enqueue_scratch(ThreadCtx* ctx, hipStream_t s, size_t bytes, void** out) {
             ctx->enq_bytes = 0;
         }
         const size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
-        if ((e = hipMalloc(&ctx->enq_dev, want)) != hipSuccess) return status_of(e);
+        if ((e = dev_malloc(&ctx->enq_dev, want)) != hipSuccess) return status_of(e);
         ctx->enq_bytes = want;
         ctx->enq_used = false;
     } else if (ctx->enq_used && ctx->enq_last != s) {
@@ -159,7 +182,7 @@ hipError_t launch_final_merge(const Welford* partials, const uint64_t* tile_offs
                               hipStream_t stream) {
     hipLaunchKernelGGL(final_merge_kernel, dim3(n_contracts), dim3(kMergeGroups), 0, stream,
                        partials, tile_offsets, n_tiles, out);
-    return hipGetLastError();
+    return launch_check();
 }
 
 // ---- RNG parity kernels ----
@@ -244,6 +267,20 @@ using namespace mcdx;
 
 extern "C" {
 
+int mcdx_fault_inject(int site, int32_t countdown) {
+    if (countdown < 0) return MCDX_EINVAL;
+    if (site == MCDX_FAULT_ALLOC) t_fail_alloc = countdown;
+    else if (site == MCDX_FAULT_LAUNCH) t_fail_launch = countdown;
+    else return MCDX_EINVAL;
+    return MCDX_OK;
+}
+
+int mcdx_fault_pending(int site) {
+    if (site == MCDX_FAULT_ALLOC) return t_fail_alloc;
+    if (site == MCDX_FAULT_LAUNCH) return t_fail_launch;
+    return -MCDX_EINVAL;
+}
+
 int mcdx_inv_normal_words(const uint32_t* words, uint64_t n, int precision, double* out) {
     if ((!words || !out) && n) return MCDX_EINVAL;
     if (precision != MCDX_PREC_FP32 && precision != MCDX_PREC_FP64) return MCDX_EINVAL;
@@ -260,7 +297,7 @@ int mcdx_inv_normal_words(const uint32_t* words, uint64_t n, int precision, doub
     if (e == hipSuccess) {
         hipLaunchKernelGGL(inv_normal_words_kernel, dim3(blocks), dim3(256), 0, ctx->stream, dw,
                            n, precision == MCDX_PREC_FP64 ? 1 : 0, dout);
-        e = hipGetLastError();
+        e = launch_check();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -285,7 +322,7 @@ int mcdx_inv_normal_f32_scan(uint64_t begin, uint64_t count, double* max_err,
     uint32_t* dbits = (uint32_t*)(derr + slots);
     hipLaunchKernelGGL(inv_normal_scan_kernel, dim3(kScanBlocks), dim3(256), 0, ctx->stream,
                        begin, count, derr, dbits);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess)
         e = hipMemcpyAsync(ctx->pinned, ctx->dev, bytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -335,7 +372,7 @@ int mcdx_philox_bits(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw, u
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(philox_bits_kernel, dim3(blocks), dim3(256), 0, ctx->stream, seed, path0,
                        n, draw, stream, (uint32_t*)ctx->dev);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->dev, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return status_of(e);
@@ -354,7 +391,7 @@ int mcdx_normal_draws(uint64_t seed, uint64_t path0, uint64_t n, uint32_t draw, 
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(normal_draws_kernel, dim3(blocks), dim3(256), 0, ctx->stream, seed, path0,
                        n, draw, precision == MCDX_PREC_FP64 ? 1 : 0, (double*)ctx->dev);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->dev, n * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return status_of(e);

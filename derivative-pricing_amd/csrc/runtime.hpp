@@ -42,6 +42,21 @@ int enqueue_scratch(ThreadCtx* ctx, hipStream_t s, size_t bytes, void** out);
 int enqueue_scratch_done(ThreadCtx* ctx, hipStream_t s);
 int status_of(hipError_t e);
 
+// Device allocation and launch-status checks behind a test seam (mcdx_fault_inject): when the
+// calling thread is armed, its Nth allocation reports out-of-memory and allocates nothing, or
+// its Nth launch check reports a device error after consuming the real status, so every error
+// path can run without a real fault.
+bool inject_alloc_failure();
+template <class T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    if (inject_alloc_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc((void**)p, bytes);
+}
+hipError_t launch_check();
+
 // Stream to launch on: the caller's stream if given, else the thread's own.
 inline hipStream_t pick_stream(ThreadCtx* ctx, void* stream) {
     return stream ? (hipStream_t)stream : ctx->stream;

@@ -122,7 +122,7 @@ int device_tables(const uint32_t** out) {
     if (!g_dev_tables[dev]) {
         const std::vector<uint32_t> host = build_tables();
         uint32_t* d = nullptr;
-        e = hipMalloc(&d, host.size() * sizeof(uint32_t));
+        e = dev_malloc(&d, host.size() * sizeof(uint32_t));
         if (e != hipSuccess) return status_of(e);
         e = hipMemcpy(d, host.data(), host.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -538,7 +538,7 @@ hipError_t launch_qmc_as(unsigned tiles, unsigned reps, size_t shmem, hipStream_
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((qmc_tile_kernel<FP64, PROD, SHIFTED>), dim3(tiles, reps), dim3(kTilePaths),
                        shmem, s, a);
-    return hipGetLastError();
+    return launch_check();
 }
 
 template <bool FP64, bool SHIFTED>
@@ -884,7 +884,7 @@ int qmc_fill_store(double spot, double rate, double carry, double vol, double ti
             hipLaunchKernelGGL((qmc_store_kernel<false, false>), dim3(grid), dim3(256), shmem, s, v, dops,
                                (int32_t)ops.size(), dshift, spot, vol, vol2, n, store);
     }
-    return status_of(hipGetLastError());
+    return status_of(launch_check());
 }
 
 } // namespace mcdx
@@ -917,7 +917,7 @@ int mcdx_sobol_ints(uint32_t dim, uint64_t index0, uint64_t n, uint32_t* out) {
     if ((st = ensure_dev(ctx, n * 4))) return st;
     hipLaunchKernelGGL(sobol_ints_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        ctx->stream, v + 32 * dim, index0, n, (uint32_t*)ctx->dev);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_check();
     if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->dev, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return status_of(e);

@@ -8,6 +8,7 @@
 #include "mcd/engine.hpp"
 #include <pybind11/stl.h>
 
+#include <new>
 #include <string>
 
 #include "mcd/core/sobol.hpp"
@@ -52,6 +53,7 @@ auto engine(F&& f) {
     const int st = mcd::last_engine_status();
     if (st != MCDX_OK) {
         if (st == MCDX_EINVAL) throw py::value_error(status_text(st));
+        if (st == MCDX_ENOMEM) throw std::bad_alloc(); // MemoryError, as a host allocation's
         throw std::runtime_error(status_text(st));
     }
     return out;

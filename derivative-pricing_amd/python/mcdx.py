@@ -91,6 +91,8 @@ SIGNATURES = {
     "mcdx_inv_normal_f32_scan": (C.c_int, [u64, u64, P(dbl), P(u32)]),
     "mcdx_inv_normal_words": (C.c_int, [P(u32), u64, C.c_int, P(dbl)]),
     "mcdx_issue_probe": (C.c_int, [C.c_int, u32, P(dbl), P(u64)]),
+    "mcdx_fault_inject": (C.c_int, [C.c_int, C.c_int32]),
+    "mcdx_fault_pending": (C.c_int, [C.c_int]),
     "mcdx_monte_carlo_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Options), P(Result)]),
     "mcdx_monte_carlo_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Options),
                                                        P(Result)]),
@@ -167,7 +169,9 @@ _lib = None
 
 
 class McdxError(RuntimeError):
-    pass
+    def __init__(self, message, status=None):
+        super().__init__(message)
+        self.status = status
 
 
 def lib():
@@ -187,7 +191,7 @@ def lib():
 
 def check(status, what=""):
     if status != OK:
-        raise McdxError(f"{what} failed with mcdx status {status}")
+        raise McdxError(f"{what} failed with mcdx status {status}", status)
 
 
 def philox_bits(seed, path0, n, draw, stream):
@@ -202,6 +206,15 @@ def normal_draws(seed, path0, n, draw, precision):
     check(lib().mcdx_normal_draws(seed, path0, n, draw, precision, out.ctypes.data_as(P(dbl))),
           "mcdx_normal_draws")
     return out
+
+
+FAULT_ALLOC, FAULT_LAUNCH = 0, 1
+
+
+def fault_inject(site, countdown):
+    """Arm (countdown >= 1) or disarm (0) an injected failure of this thread's
+    countdown-th device allocation (FAULT_ALLOC) or launch check (FAULT_LAUNCH)."""
+    check(lib().mcdx_fault_inject(site, countdown), "mcdx_fault_inject")
 
 
 ISSUE_CLASSES = {"fma_f32": 0, "mad_u64_u32": 1, "mul_hi_u32": 2, "xor3": 3, "exp2_f32": 4,

@@ -101,6 +101,19 @@ int mcdx_inv_normal_words(const uint32_t* words, uint64_t n, int precision, doub
 int mcdx_inv_normal_f32_scan(uint64_t begin, uint64_t count, double max_err[2],
                              uint32_t argmax_bits[2]);
 
+/* ---- Error-path test seam (no reference counterpart) ----
+ * Arms the calling thread so that its `countdown`-th device allocation (site
+ * MCDX_FAULT_ALLOC: scratch growth, book buffers, table uploads, pinned staging) reports
+ * MCDX_ENOMEM without allocating, or its `countdown`-th launch check (MCDX_FAULT_LAUNCH, made
+ * after every kernel launch) reports MCDX_EDEVICE after consuming the real launch status.
+ * countdown 0 disarms; the armed state is per thread and fires once. Kernels queued before
+ * an injected launch failure still run; the entry point reports the error and its state
+ * stays usable. mcdx_fault_pending returns the remaining countdown of a site. */
+#define MCDX_FAULT_ALLOC 0
+#define MCDX_FAULT_LAUNCH 1
+int mcdx_fault_inject(int site, int32_t countdown);
+int mcdx_fault_pending(int site);
+
 /* ---- VALU issue-rate probe (no reference counterpart) ----
  * Times `iterations` steps of eight independent chains of one instruction class in every lane
  * of 8 waves per SIMD (all SIMDs) and reports the seconds and the wave64 instructions issued:

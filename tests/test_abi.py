@@ -44,3 +44,26 @@ def test_engine_calls_without_device_report_status_not_crash():
     assert L.mcdx_set_precision(9) == mcdx.EINVAL
     assert L.mcdx_get_precision() in (0, 1)
     assert L.mcdx_version() == b"0.1.0"
+
+
+def test_fault_injection_seam_arms_per_thread():
+    # The seam only sets thread-local countdowns: it validates its arguments and reports what
+    # is pending without touching a device.
+    import threading
+    L = mcdx.lib()
+    assert L.mcdx_fault_inject(7, 1) == mcdx.EINVAL
+    assert L.mcdx_fault_inject(mcdx.FAULT_ALLOC, -1) == mcdx.EINVAL
+    assert L.mcdx_fault_pending(7) < 0
+    mcdx.fault_inject(mcdx.FAULT_ALLOC, 3)
+    mcdx.fault_inject(mcdx.FAULT_LAUNCH, 2)
+    seen = []
+    t = threading.Thread(target=lambda: seen.append((L.mcdx_fault_pending(mcdx.FAULT_ALLOC),
+                                                     L.mcdx_fault_pending(mcdx.FAULT_LAUNCH))))
+    t.start()
+    t.join()
+    assert seen == [(0, 0)]  # another thread is not armed
+    assert L.mcdx_fault_pending(mcdx.FAULT_ALLOC) == 3
+    assert L.mcdx_fault_pending(mcdx.FAULT_LAUNCH) == 2
+    mcdx.fault_inject(mcdx.FAULT_ALLOC, 0)
+    mcdx.fault_inject(mcdx.FAULT_LAUNCH, 0)
+    assert L.mcdx_fault_pending(mcdx.FAULT_ALLOC) == 0 == L.mcdx_fault_pending(mcdx.FAULT_LAUNCH)
