@@ -4,6 +4,7 @@ Produces, in-tree:
   derivative-pricing_amd/lib/libmcd_hip.so   HIP kernels + the mcdx C ABI (include/mcdx.h)
   derivative-pricing_amd/python/mcd*.so       pybind11 `mcd` module (drop-in Python API)
   derivative-pricing_amd/bin/mcd_cli          JSON command-line front end
+  derivative-pricing_amd/_san/                host-only sanitizer builds for the CPU tests
 and asks oracle/Makefile for the CPU oracle (test infrastructure) and, where the reference
 tree exists, oracle/_ref.
 """
@@ -113,6 +114,37 @@ def build_cpp_tests():
         os.replace(out + ".tmp", out)
 
 
+def build_sanitized():
+    """_san/: host-only sanitizer builds for the CPU tests (g++ and gcc, no HIP source on these
+    lines): the C++ host layer compiled into the reference-style caller, the API test and the
+    CLI under AddressSanitizer + UndefinedBehaviorSanitizer (linked against the normal
+    libmcd_hip.so), and the oracle's driver under ASan + UBSan and under ThreadSanitizer."""
+    san = os.path.join(PKG, "_san")
+    os.makedirs(san, exist_ok=True)
+    asan = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+            "-fno-sanitize-recover=all", "-ffp-contract=off"]
+    hosts = [s for s in sorted(glob.glob(os.path.join(HOST, "*.cpp")))
+             if not s.endswith("module.cpp")]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    link = ["-L", LIBDIR, "-lmcd_hip", "-Wl,-rpath,$ORIGIN/../lib"]
+    mains = {"ref_caller_test": os.path.join(ROOT, "tests", "cpp", "ref_caller_test.cpp"),
+             "api_test": os.path.join(ROOT, "tests", "cpp", "api_test.cpp"),
+             "mcd_cli": os.path.join(PKG, "cli", "mcd_cli.cpp")}
+    for name, main_src in mains.items():
+        out = os.path.join(san, name)
+        run(["g++", "-std=c++20"] + asan + inc + hosts + [main_src] + link + ["-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+    odir = os.path.join(ROOT, "oracle")
+    drv = os.path.join(ROOT, "tests", "cpp", "oracle_san_driver.c")
+    for name, flags in (("oracle_asan", asan),
+                        ("oracle_tsan", ["-O1", "-g", "-fsanitize=thread",
+                                         "-fno-omit-frame-pointer", "-ffp-contract=off"])):
+        out = os.path.join(san, name)
+        run(["gcc", "-std=c11"] + flags + ["-I", odir, drv, os.path.join(odir, "mcd_oracle.c"),
+                                           "-lm", "-lpthread", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     ref_root = os.environ.get("MCDX_REFERENCE_ROOT", "/root/reference")
@@ -127,6 +159,7 @@ def main():
     build_cli()
     build_cpp_tests()
     build_oracle()
+    build_sanitized()
 
 
 if __name__ == "__main__":
