@@ -14,8 +14,11 @@
 // dimensions: primitive polynomials (LFSR maximal-period test, the reference's own method)
 // in order of degree and coefficient bits, skipping the six the reference uses, with
 // initial direction integers m_1..m_s odd and < 2^k drawn from Philox key 0x50B01 at
-// counter {dim, 0, k, 3 + attempt}; an attempt is kept only when the dimension's first
-// eight direction integers differ from every earlier dimension's.
+// counter {dim, 0, k, 3 + attempt}. The attempt per dimension (sobol_attempts.inc) was chosen
+// offline by sobol/search_initial_numbers.cpp among 256, keeping only attempts whose first
+// eight direction integers differ from every earlier dimension's: the one with the smallest
+// worst t-value of the two-dimensional projections onto (earlier dimension, this one) at
+// 2^12 points, then the smallest sum of 4^t over earlier dimensions and 2^6, 2^9, 2^12 points.
 //
 // Kernel layout: one lane per Sobol index. A wave covers 64 consecutive indices, so the
 // Gray code's bits >= 6 are wave-uniform: x = Xhi(dim) ^ Xlo(dim, lane), where Xhi folds
@@ -78,6 +81,13 @@ void fill_directions(const Poly& p, const uint32_t* m_init, uint32_t* v /* [32],
     for (unsigned k = 1; k <= 32; ++k) v[k - 1] = m[k] << (32 - k);
 }
 
+// Per dimension d >= 7, the Philox attempt of its initial direction integers, chosen offline
+// (sobol/search_initial_numbers.cpp) for the best two-dimensional projections with every
+// earlier dimension.
+const uint8_t kSobolAttempt[kSobolDims] = {
+#include "sobol_attempts.inc"
+};
+
 std::vector<uint32_t> build_tables() {
     std::vector<uint32_t> v(kSobolDims * 32);
     uint32_t ones[32];
@@ -92,18 +102,13 @@ std::vector<uint32_t> build_tables() {
             for (unsigned r = 1; r < kSobolRefDims; ++r)
                 used = used || (kRefPolys[r].degree == s && kRefPolys[r].interior == c);
             if (used) continue;
-            for (uint32_t attempt = 0;; ++attempt) {
-                uint32_t m_init[32];
-                for (unsigned k = 1; k <= s; ++k) {
-                    const uint32_t bits = philox_bits(0x50B01ull, d, k, 3 + attempt);
-                    m_init[k - 1] = (k >= 32 ? bits : (bits & ((1u << k) - 1u))) | 1u;
-                }
-                fill_directions(Poly{s, c}, m_init, &v[d * 32]);
-                bool clash = false;
-                for (unsigned e = 0; e < d && !clash; ++e)
-                    clash = memcmp(&v[e * 32], &v[d * 32], 8 * sizeof(uint32_t)) == 0;
-                if (!clash) break;
+            const uint32_t attempt = kSobolAttempt[d];
+            uint32_t m_init[32];
+            for (unsigned k = 1; k <= s; ++k) {
+                const uint32_t bits = philox_bits(0x50B01ull, d, k, 3 + attempt);
+                m_init[k - 1] = (k >= 32 ? bits : (bits & ((1u << k) - 1u))) | 1u;
             }
+            fill_directions(Poly{s, c}, m_init, &v[d * 32]);
             ++d;
         }
     }

@@ -163,8 +163,12 @@ def test_rqmc_fp32_close_to_fp64_on_the_same_points(product, kw):
 @pytest.mark.parametrize("product,kw", RQMC_CASES)
 def test_rqmc_unbiased_against_mc_and_tighter(product, kw):
     # 16 replicates of 2^16 points against 2^22 Monte Carlo paths: the estimates agree within
-    # the combined error, and the randomized QMC error is well below Monte Carlo's at the
-    # same number of points (the shifts keep the net structure).
+    # the combined error, and the randomized QMC error is below Monte Carlo's at the same
+    # number of points (the shifts keep the net structure). The gain depends on the payoff:
+    # smooth path functionals (Asian, lookbacks) reach 0.06-0.2 of the Monte Carlo error, a
+    # discretely monitored knock-out only about 0.5 (128 replicates of 2^14 points on the CPU
+    # oracle: 0.50-0.53). A 16-replicate error estimate scatters by about 18% (1/sqrt(30)), so
+    # the barrier bound sits three of those above its expected ratio.
     c = qmc_contract(product, 252, 1 << 16, **kw)
     q, qse, total, _ = mcdx.rqmc_path(c, FP32, 16, 2024)
     m = mcdx.GbmContract.from_buffer_copy(c)
@@ -173,7 +177,8 @@ def test_rqmc_unbiased_against_mc_and_tighter(product, kw):
     p, se, _ = mcdx.price(m, FP32)
     assert abs(q - p) <= 4 * np.hypot(qse, se) + 1e-4, (q, qse, p, se)
     se_mc_same_points = se * np.sqrt((1 << 22) / total)
-    assert qse < 0.5 * se_mc_same_points, (qse, se_mc_same_points)
+    bound = 0.85 if product == mcdx.BARRIER else 0.5
+    assert qse < bound * se_mc_same_points, (qse, se_mc_same_points)
 
 
 def test_rqmc_european_within_error_of_black_scholes():

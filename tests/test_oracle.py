@@ -251,6 +251,72 @@ def test_sobol_direction_numbers_valid():
     assert len(firsts) == 1024
 
 
+def _gen_rows(dim, m):
+    # row r (r = 0 the most significant bit) of the m x m generator matrix over GF(2): bit
+    # 31 - r of the direction integers v_1..v_m, v_k in bit k - 1
+    v = O.sobol_table(dim)
+    return [sum(((int(v[k]) >> (31 - r)) & 1) << k for k in range(m)) for r in range(m)]
+
+
+def _independent(vecs):
+    basis = []
+    for x in vecs:
+        for b in basis:
+            x = min(x, x ^ b)
+        if x == 0:
+            return False
+        basis.append(x)
+    return True
+
+
+def _t_value(ra, rb, m):
+    # points 0..2^m-1 of the projection form a (t, m, 2)-net with t = m - rho, rho the largest
+    # value with every first-d1-rows-of-a plus first-d2-rows-of-b (d1 + d2 = rho) independent
+    for rho in range(m, -1, -1):
+        if all(_independent(ra[:d1] + rb[:rho - d1]) for d1 in range(rho + 1)):
+            return m - rho
+    return m
+
+
+def _t_value_brute(da, db, m):
+    # the definition: the smallest t such that every elementary box [a 2^-d1, (a+1) 2^-d1) x
+    # [b 2^-d2, (b+1) 2^-d2) with d1 + d2 = m - t holds exactly 2^t of the 2^m points
+    xa = np.array([L.orc_sobol_int(da, i) for i in range(1 << m)], dtype=np.uint64)
+    xb = np.array([L.orc_sobol_int(db, i) for i in range(1 << m)], dtype=np.uint64)
+    for t in range(m + 1):
+        ok = True
+        for d1 in range(m - t + 1):
+            d2 = m - t - d1
+            cell = (xa >> np.uint64(32 - d1)) * np.uint64(1 << d2) + (xb >> np.uint64(32 - d2))
+            counts = np.bincount(cell.astype(np.int64), minlength=1 << (m - t))
+            if not np.all(counts == 1 << t):
+                ok = False
+                break
+        if ok:
+            return t
+    return m
+
+
+def test_sobol_t_value_computation_matches_the_net_definition():
+    for da, db in [(0, 1), (1, 2), (3, 5), (2, 9), (7, 12), (12, 20), (40, 41), (100, 251)]:
+        for m in (4, 6, 8):
+            assert _t_value(_gen_rows(da, m), _gen_rows(db, m), m) == \
+                _t_value_brute(da, db, m), (da, db, m)
+
+
+def test_sobol_two_dimensional_projections():
+    # The extension's initial direction integers were searched for good two-dimensional
+    # projections (smallest worst t-value over 2^5..2^12 points against every earlier
+    # dimension). Measured over dimension pairs, which the first choice that merely avoided
+    # duplicates left at worst t = 11 (of 12) among the first 64 dimensions at 2^12 points
+    # and t = 7 among the first 32 at 2^8: the searched table holds them to 6 and 5.
+    bounds = {(8, 32): (5, 2.2), (10, 32): (6, 2.35), (12, 64): (6, 2.85)}
+    for (m, dims), (t_max, t_mean) in bounds.items():
+        rows = [_gen_rows(d, m) for d in range(dims)]
+        ts = [_t_value(rows[i], rows[j], m) for i in range(dims) for j in range(i + 1, dims)]
+        assert max(ts) <= t_max and np.mean(ts) <= t_mean, (m, dims, max(ts), np.mean(ts))
+
+
 def test_brownian_bridge_is_the_exact_linear_map():
     # W(T) = sqrt(T) z0 and Cov(W_i, W_j) = min(t_i, t_j): the bridge is a linear map A
     # with A A^T = min(t_i, t_j); build A column by column from unit normals.
