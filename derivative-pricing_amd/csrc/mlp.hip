@@ -86,7 +86,6 @@ struct MlpLds {
     unsigned short w2[kH * kLd];  // W2[o][i]
     unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
     float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
-    float grad[2][kParams]; // per wave, summed in wave order by the update
     struct Wave {
         unsigned short h1[kH * kLd];  // H1[s][i]
         unsigned short h1t[kH * kLd]; // [i][s]
@@ -151,102 +150,6 @@ __device__ float forward64(MlpLds& L, MlpLds::Wave& W, float x, float t, f16v ac
     return y;
 }
 
-// Backward of this wave's samples given dL/dy per lane; writes the wave's parameter
-// gradients into L.grad[wave].
-__device__ void backward64(MlpLds& L, MlpLds::Wave& W, float g3, f16v acc[2][2], int wave) {
-    const int lane = threadIdx.x & 63;
-    W.g3[lane] = g3;
-    wave_sync();
-    // G2 = g3 * w3 * silu'(a2); dW3 = sum_s g3 * h2; db2 = sum_s G2
-    float dw3[2] = {0.f, 0.f}, db2[2] = {0.f, 0.f};
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int s = acc_row(tm, r), o = acc_col(tn);
-                const float a2 = acc[tm][tn][r];
-                const float gs = W.g3[s];
-                dw3[tn] += gs * silu(a2);
-                const float g2 = gs * L.w3[o] * silu_grad(a2);
-                db2[tn] += g2;
-                const unsigned short gb = f2bf(g2);
-                W.g2[s * kLd + o] = gb;
-                W.g2t[o * kLd + s] = gb;
-            }
-    // combine the two row halves (lanes l and l+32 share a column)
-    float dw3o[2], db2o[2];
-#pragma unroll
-    for (int tn = 0; tn < 2; ++tn) {
-        dw3o[tn] = dw3[tn] + __shfl_xor(dw3[tn], 32, 64);
-        db2o[tn] = db2[tn] + __shfl_xor(db2[tn], 32, 64);
-    }
-    wave_sync();
-    // dW2[o][i] = sum_s G2[s][o] H1[s][i]
-    f16v dw2[2][2];
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) dw2[tm][tn][r] = 0.f;
-    wave_gemm64(W.g2t, W.h1t, dw2);
-    // G1[s][i] = sum_o G2[s][o] W2[o][i]
-    f16v g1[2][2];
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) g1[tm][tn][r] = 0.f;
-    wave_gemm64(W.g2, L.w2t, g1);
-    float dw1a[2] = {0.f, 0.f}, dw1b[2] = {0.f, 0.f}, db1[2] = {0.f, 0.f};
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int s = acc_row(tm, r), i = acc_col(tn);
-                const float x = W.xs[s], t = W.ts[s];
-                const float a1 = __builtin_fmaf(L.w1[2 * i], x, __builtin_fmaf(L.w1[2 * i + 1], t, L.b1[i]));
-                const float g = g1[tm][tn][r] * silu_grad(a1);
-                dw1a[tn] += g * x;
-                dw1b[tn] += g * t;
-                db1[tn] += g;
-            }
-#pragma unroll
-    for (int tn = 0; tn < 2; ++tn) {
-        dw1a[tn] += __shfl_xor(dw1a[tn], 32, 64);
-        dw1b[tn] += __shfl_xor(dw1b[tn], 32, 64);
-        db1[tn] += __shfl_xor(db1[tn], 32, 64);
-    }
-    float gb3 = g3;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) gb3 += __shfl_xor(gb3, off, 64);
-    float* gw = L.grad[wave];
-    auto put = [&](int idx, float v) { gw[idx] = v; };
-    if (lane < 32) {
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn) {
-            const int c = tn * 32 + lane;
-            put(oW3 + c, dw3o[tn]);
-            put(oB2 + c, db2o[tn]);
-            put(oW1 + 2 * c, dw1a[tn]);
-            put(oW1 + 2 * c + 1, dw1b[tn]);
-            put(oB1 + c, db1[tn]);
-        }
-    }
-    if (lane == 0) put(oB3, gb3);
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) put(oW2 + acc_row(tm, r) * kH + acc_col(tn), dw2[tm][tn][r]);
-}
-
 struct TrainArgs {
     float* w;          // [kParams] weights (in/out)
     const float* xs;   // [n] first feature
@@ -263,11 +166,49 @@ struct TrainArgs {
                        // the last iterate)
 };
 
-constexpr int kTrainLanes = 128;
-constexpr int kPer = (kParams + kTrainLanes - 1) / kTrainLanes; // parameters per lane (35)
+// Training layout: one workgroup of 8 waves. A step's batch is two blocks of 64 samples (lane
+// = sample); each block's four waves split its work by accumulator tile: wave q of a block
+// owns tile (tm, tn) = (q / 2, q % 2) of every 64x64 product (one 32x32x8 MFMA chain instead
+// of four) and the elementwise work on that tile, and hidden units 16q .. 16q+15 of the first
+// layer. Column sums over a tile's rows are combined across the two row tiles through LDS.
+constexpr int kSampleBlocks = 2;
+constexpr int kWavesPerBlock = 4;
+constexpr int kTrainLanes = 64 * kSampleBlocks * kWavesPerBlock; // 512
+constexpr int kPer = (kParams + kTrainLanes - 1) / kTrainLanes;  // parameters per lane (9)
+constexpr int kSamplerWave = 3; // the wave of each block that draws its samples
+constexpr int kAttempts = 8;    // rejection-sampling attempts per sample
 
-// Lane-owned parameter i = lane + 128 k into its LDS places (BF16 W2 in both layouts).
-__device__ __forceinline__ void scatter_param(MlpLds& L, int i, float v) {
+struct TrainLds {
+    unsigned short w2[kH * kLd];  // W2[o][i]
+    unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
+    float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
+    float grad[kSampleBlocks][kParams]; // per sample block, summed in block order by Adam
+    struct Block {
+        unsigned short h1[kH * kLd];  // H1[s][i]
+        unsigned short h1t[kH * kLd]; // [i][s]
+        unsigned short g2[kH * kLd];  // G2[s][o]; FP32 h2 * w3 staging before the backward
+        unsigned short g2t[kH * kLd]; // [o][s]
+        float xs[kH], ts[kH], ys[kH], g3[kH];
+        float part[5][2][kH]; // dW3, db2, dW1[:,0], dW1[:,1], db1 column sums per row tile
+    } blk[kSampleBlocks];
+};
+static_assert(sizeof(TrainLds) <= 160 * 1024, "LDS budget");
+
+// acc += A(rows tm*32.., k) * Bt(rows tn*32.., k) over k < 64: one 32x32 tile.
+__device__ __forceinline__ void tile_gemm64(const unsigned short* a, const unsigned short* bt,
+                                            int tm, int tn, f16v& acc) {
+    const int lane = threadIdx.x & 63;
+    const int row = lane & 31, kg = lane >> 5;
+#pragma unroll
+    for (int kk = 0; kk < kH; kk += 8) {
+        const s4v af = *(const s4v*)(a + (tm * 32 + row) * kLd + kk + 4 * kg);
+        const s4v bf = *(const s4v*)(bt + (tn * 32 + row) * kLd + kk + 4 * kg);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af, bf, acc, 0, 0, 0);
+    }
+}
+
+// Lane-owned parameter i = lane + kTrainLanes k into its LDS places (BF16 W2 in both layouts).
+__device__ __forceinline__ void scatter_param(TrainLds& L, int i, float v) {
     if (i < oB1) L.w1[i] = v;
     else if (i < oW2) L.b1[i - oB1] = v;
     else if (i < oB2) {
@@ -280,22 +221,18 @@ __device__ __forceinline__ void scatter_param(MlpLds& L, int i, float v) {
     else L.b3 = v;
 }
 
-// Single workgroup of 128 lanes (2 waves x 64 samples) running every Adam step in one
-// launch. Each lane keeps its 35 parameters and their Adam moments in registers; the two
-// waves' backward passes run concurrently into separate gradient arrays.
-static_assert(sizeof(MlpLds) <= 160 * 1024, "LDS budget");
-
-struct TrainLds {
-    MlpLds net;
-};
-
+// Every Adam step in one launch. Each lane keeps its kPer parameters and their Adam moments
+// in registers; per step the batch's gradient is formed in LDS by the eight waves.
 __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
     extern __shared__ unsigned char smem[];
-    TrainLds& T = *(TrainLds*)smem;
-    MlpLds& L = T.net;
+    TrainLds& L = *(TrainLds*)smem;
     const int tid = threadIdx.x;
-    const int wave = tid >> 6;
-    MlpLds::Wave& W = L.wave[wave];
+    const int lane = tid & 63;
+    const int bi = tid >> 8;                       // sample block
+    const int q = (tid >> 6) & (kWavesPerBlock - 1); // wave within the block
+    const int tm = q >> 1, tn = q & 1;
+    TrainLds::Block& B = L.blk[bi];
+    float* gw = L.grad[bi];
     const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
     float bc1 = 1.f, bc2 = 1.f, lr = a.lr;
     float wr[kPer], am[kPer], av[kPer]; // parameters and Adam moments, lane-owned
@@ -308,49 +245,195 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         av[k] = 0.f;
         wa[k] = 0.f;
     }
+    // Sampling: rejection sampling over the valid (non-NaN target) samples, up to kAttempts
+    // attempts, the draw index being the attempt, so every sample's choice is a fixed function
+    // of (seed, step, sample); after kAttempts invalid draws the sample is masked. The
+    // sampler wave of each block prefetches: it issues all attempts' target loads for the next
+    // step while the output wave works, picks the first valid attempt and issues its feature
+    // loads while another wave combines column sums, and stores the sample at the next step's
+    // start, so no HBM round trip sits on the step's critical path.
+    const int sample = bi * 64 + lane;
+    uint64_t pidx[kAttempts];
+    float pty[kAttempts];
+    float nx = 0.f, nt = 0.f, ny = 0.f;
+    auto issue = [&](int st) {
+        const uint64_t ctr = (uint64_t)st * 128 + sample;
+#pragma unroll
+        for (uint32_t tr = 0; tr < kAttempts; ++tr) {
+            uint64_t r = philox_bits(a.seed, ctr, tr, 7);
+            if (a.n > 0xFFFFFFFFull) r |= (uint64_t)philox_bits(a.seed, ctr, tr + 8, 7) << 32;
+            pidx[tr] = r % a.n;
+            pty[tr] = a.ys[pidx[tr]];
+        }
+    };
+    auto choose = [&]() {
+        uint64_t idx = pidx[kAttempts - 1];
+        float y = pty[kAttempts - 1];
+#pragma unroll
+        for (int tr = kAttempts - 1; tr >= 0; --tr)
+            if (pty[tr] == pty[tr]) {
+                idx = pidx[tr];
+                y = pty[tr];
+            }
+        const bool valid = y == y;
+        nx = valid ? a.xs[idx] : 0.f;
+        nt = valid ? a.ts[idx] : 0.f;
+        ny = y;
+    };
+    if (!a.grad_out && q == kSamplerWave) {
+        issue(0);
+        choose();
+    }
     for (int step = 0; step < a.steps; ++step) {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int i = tid + kTrainLanes * k;
             if (i < kParams) scatter_param(L, i, wr[k]);
         }
-        __syncthreads();
-        uint64_t idx = 0;
-        float y = 0.f;
+        // sample s of block bi is the batch's sample bi * 64 + s
         if (a.grad_out) {
-            idx = (uint64_t)tid < a.n ? (uint64_t)tid : 0;
-            y = a.ys[idx];
-        } else {
-            // Rejection sampling over the valid (non-NaN target) samples; the draw index is
-            // the attempt, so every lane's choice is a fixed function of (seed, step, lane).
-            const uint64_t ctr = (uint64_t)step * kTrainLanes + tid;
-            for (uint32_t tr = 0; tr < 8; ++tr) {
-                uint64_t r = philox_bits(a.seed, ctr, tr, 7);
-                if (a.n > 0xFFFFFFFFull) r |= (uint64_t)philox_bits(a.seed, ctr, tr + 8, 7) << 32;
-                idx = r % a.n;
-                y = a.ys[idx];
-                if (y == y) break;
+            if (q == 0) {
+                const uint64_t idx = (uint64_t)sample < a.n ? (uint64_t)sample : 0;
+                const float y = a.ys[idx];
+                const bool valid = y == y;
+                B.xs[lane] = valid ? a.xs[idx] : 0.f;
+                B.ts[lane] = valid ? a.ts[idx] : 0.f;
+                B.ys[lane] = y;
+            }
+        } else if (q == kSamplerWave) { // chosen and loaded during the previous step
+            B.xs[lane] = nx;
+            B.ts[lane] = nt;
+            B.ys[lane] = ny;
+        }
+        __syncthreads();
+        // first layer: hidden units 16q .. 16q+15 of this lane's sample
+        {
+            const float x = B.xs[lane], t = B.ts[lane];
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int i = q * 16 + u;
+                const float a1 = __builtin_fmaf(L.w1[2 * i], x, __builtin_fmaf(L.w1[2 * i + 1], t, L.b1[i]));
+                const unsigned short h = f2bf(silu(a1));
+                B.h1[lane * kLd + i] = h;
+                B.h1t[i * kLd + lane] = h;
             }
         }
-        const float valid = (y == y) ? 1.f : 0.f;
-        const float x = valid != 0.f ? a.xs[idx] : 0.f, t = valid != 0.f ? a.ts[idx] : 0.f;
-        if (valid == 0.f) y = 0.f;
-        f16v acc[2][2];
-        const float yhat = forward64(L, W, x, t, acc);
-        const float diff = (yhat - y) * valid;
-        const float g3 = 2.0f * diff / 128.0f;
-        float l2 = diff * diff;
+        __syncthreads();
+        // second layer on this wave's tile; stage h2 * w3 per (s, o) in FP32
+        f16v acc;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) l2 += __shfl_xor(l2, off, 64);
-        backward64(L, W, g3, acc, wave);
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        tile_gemm64(B.h1, L.w2, tm, tn, acc);
+        float* h2w = (float*)B.g2; // [64][65] floats fits in g2 + g2t
+        {
+            const int o = acc_col(tn);
+            const float b2o = L.b2[o], w3o = L.w3[o];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                acc[r] += b2o;
+                h2w[acc_row(tm, r) * 65 + o] = silu(acc[r]) * w3o;
+            }
+        }
+        __syncthreads();
+        float l2 = 0.f;
+        if (!a.grad_out && q == kSamplerWave && step + 1 < a.steps) issue(step + 1);
+        if (q == 0) { // output and dL/dy of this lane's sample (row sum in column order)
+            float yhat = L.b3;
+            for (int o = 0; o < kH; ++o) yhat += h2w[lane * 65 + o];
+            const float y = B.ys[lane];
+            const float diff = (y == y) ? (yhat - y) : 0.f;
+            B.g3[lane] = 2.0f * diff / 128.0f;
+            l2 = diff * diff;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) l2 += __shfl_xor(l2, off, 64);
+        }
+        __syncthreads();
+        // backward through the output layer on this wave's tile: G2 = g3 * w3 * silu'(a2)
+        {
+            const int o = acc_col(tn);
+            const float w3o = L.w3[o];
+            float dw3 = 0.f, db2 = 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int s = acc_row(tm, r);
+                const float a2 = acc[r];
+                const float gs = B.g3[s];
+                dw3 += gs * silu(a2);
+                const float g2 = gs * w3o * silu_grad(a2);
+                db2 += g2;
+                const unsigned short gb = f2bf(g2);
+                B.g2[s * kLd + o] = gb;
+                B.g2t[o * kLd + s] = gb;
+            }
+            dw3 += __shfl_xor(dw3, 32, 64);
+            db2 += __shfl_xor(db2, 32, 64);
+            if (lane < 32) {
+                B.part[0][tm][o] = dw3;
+                B.part[1][tm][o] = db2;
+            }
+        }
+        __syncthreads();
+        // dW2[o][i] = sum_s G2[s][o] H1[s][i] on this wave's tile
+        {
+            f16v dw2;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dw2[r] = 0.f;
+            tile_gemm64(B.g2t, B.h1t, tm, tn, dw2);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) gw[oW2 + acc_row(tm, r) * kH + acc_col(tn)] = dw2[r];
+        }
+        // G1[s][i] = sum_o G2[s][o] W2[o][i], then through the first layer's SiLU
+        {
+            f16v g1;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) g1[r] = 0.f;
+            tile_gemm64(B.g2, L.w2t, tm, tn, g1);
+            const int i = acc_col(tn);
+            const float w1a = L.w1[2 * i], w1b = L.w1[2 * i + 1], b1i = L.b1[i];
+            float dw1a = 0.f, dw1b = 0.f, db1 = 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int s = acc_row(tm, r);
+                const float x = B.xs[s], t = B.ts[s];
+                const float a1 = __builtin_fmaf(w1a, x, __builtin_fmaf(w1b, t, b1i));
+                const float g = g1[r] * silu_grad(a1);
+                dw1a += g * x;
+                dw1b += g * t;
+                db1 += g;
+            }
+            dw1a += __shfl_xor(dw1a, 32, 64);
+            dw1b += __shfl_xor(dw1b, 32, 64);
+            db1 += __shfl_xor(db1, 32, 64);
+            if (lane < 32) {
+                B.part[2][tm][i] = dw1a;
+                B.part[3][tm][i] = dw1b;
+                B.part[4][tm][i] = db1;
+            }
+        }
+        if (q == 0) {
+            float gb3 = B.g3[lane];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) gb3 += __shfl_xor(gb3, off, 64);
+            if (lane == 0) gw[oB3] = gb3;
+        }
+        __syncthreads();
+        if (!a.grad_out && q == kSamplerWave && step + 1 < a.steps) choose();
+        if (q == 1) { // column sums of the two row tiles
+            const int c = lane;
+            gw[oW3 + c] = B.part[0][0][c] + B.part[0][1][c];
+            gw[oB2 + c] = B.part[1][0][c] + B.part[1][1][c];
+            gw[oW1 + 2 * c] = B.part[2][0][c] + B.part[2][1][c];
+            gw[oW1 + 2 * c + 1] = B.part[3][0][c] + B.part[3][1][c];
+            gw[oB1 + c] = B.part[4][0][c] + B.part[4][1][c];
+        }
         __syncthreads();
         if (a.grad_out) {
             for (int i = tid; i < kParams; i += kTrainLanes) a.grad_out[i] = L.grad[0][i] + L.grad[1][i];
             if (tid == 0 && a.loss_out) a.loss_out[0] = l2;
-            if (tid == 64 && a.loss_out) a.loss_out[1] = l2;
+            if (tid == 256 && a.loss_out) a.loss_out[1] = l2;
             return;
         }
-        if (tid == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // wave 0 batch half
+        if (tid == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // first sample block
         // Adam (FP32)
         bc1 *= b1c;
         bc2 *= b2c;
