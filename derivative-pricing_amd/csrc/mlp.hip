@@ -8,11 +8,11 @@
 // streaming pricing kernel. QR coefficients remain the fallback outside the trained range
 // and the judge of the learned exercise boundary.
 //
-// Layout: one wave owns 64 samples. The 64x64 layer and both backward products are
-// 64x64x64 GEMMs C[m][n] = sum_k A[m][k] * Bt[n][k] on BF16 tiles in LDS (k contiguous,
-// rows padded to 68 elements so a wave's 8-byte reads spread over all banks), computed as
-// 2x2 tiles of v_mfma_f32_32x32x8_bf16. MFMA lane layout (32x32x8): lane l supplies
-// A[l%32][4*(l/32)+j] and B[4*(l/32)+j][l%32], j < 4, and holds
+// Layout: the 64x64 layer and both backward products are 64x64x64 GEMMs
+// C[m][n] = sum_k A[m][k] * Bt[n][k] on BF16 tiles in LDS (k contiguous, rows padded to 72
+// elements: 16-byte aligned, and the 16-byte reads of 16 consecutive lanes land on distinct
+// banks), computed as 32x32 tiles of gfx950's v_mfma_f32_32x32x16_bf16. MFMA lane layout
+// (32x32x16): lane l supplies A[l%32][8*(l/32)+j] and B[8*(l/32)+j][l%32], j < 8, and holds
 // D[8*(r/4) + 4*(l/32) + r%4][l%32] in accumulator register r < 16.
 #include <math.h>
 
@@ -24,14 +24,14 @@
 namespace mcdx {
 
 constexpr int kH = 64;          // hidden width
-constexpr int kLd = 68;         // padded LDS row (bf16 elements)
+constexpr int kLd = 72;         // padded LDS row (bf16 elements)
 constexpr int kParams = 2 * kH + kH + kH * kH + kH + kH + 1; // 4417
 static_assert(kParams == kMlpParams, "parameter count");
 // parameter offsets in the flat FP32 vector
 constexpr int oW1 = 0, oB1 = 2 * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
               oB3 = oW3 + kH;
 
-typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 // Order this wave's LDS traffic (the wave executes in lockstep; this stops the compiler from
@@ -42,15 +42,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ unsigned short f2bf(float f) { // round to nearest even
-    const uint32_t u = __builtin_bit_cast(uint32_t, f);
-    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(r >> 16);
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
+}
+// Two values in one conversion: lo in bits 0-15, hi in bits 16-31.
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    const bf2v v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
 }
 
-__device__ __forceinline__ float silu(float a) { return a * fast_rcp(1.0f + fast_exp2(-a * 1.44269504f)); }
+__device__ __forceinline__ float sigmoid(float a) { return fast_rcp(1.0f + fast_exp2(-a * 1.44269504f)); }
+__device__ __forceinline__ float silu(float a) { return a * sigmoid(a); }
 __device__ __forceinline__ float silu_grad(float a) {
-    const float s = fast_rcp(1.0f + fast_exp2(-a * 1.44269504f));
+    const float s = sigmoid(a);
     return s * (1.0f + a * (1.0f - s));
 }
 
@@ -60,18 +67,18 @@ __device__ __forceinline__ void wave_gemm64(const unsigned short* a, const unsig
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, kg = lane >> 5;
 #pragma unroll
-    for (int kk = 0; kk < kH; kk += 8) {
-        s4v af[2], bf[2];
+    for (int kk = 0; kk < kH; kk += 16) {
+        bf8v af[2], bf[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            af[t] = *(const s4v*)(a + (t * 32 + row) * kLd + kk + 4 * kg);
-            bf[t] = *(const s4v*)(bt + (t * 32 + row) * kLd + kk + 4 * kg);
+            af[t] = *(const bf8v*)(a + (t * 32 + row) * kLd + kk + 8 * kg);
+            bf[t] = *(const bf8v*)(bt + (t * 32 + row) * kLd + kk + 8 * kg);
         }
 #pragma unroll
         for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
             for (int tn = 0; tn < 2; ++tn)
-                acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af[tm], bf[tn], acc[tm][tn], 0, 0, 0);
+                acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[tm], bf[tn], acc[tm][tn], 0, 0, 0);
     }
 }
 
@@ -83,14 +90,14 @@ __device__ __forceinline__ int acc_row(int tm, int r) {
 __device__ __forceinline__ int acc_col(int tn) { return tn * 32 + ((threadIdx.x & 63) & 31); }
 
 struct MlpLds {
-    unsigned short w2[kH * kLd];  // W2[o][i]
-    unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
+    alignas(16) unsigned short w2[kH * kLd];  // W2[o][i]
+    alignas(16) unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
     float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
     struct Wave {
-        unsigned short h1[kH * kLd];  // H1[s][i]
-        unsigned short h1t[kH * kLd]; // [i][s]
-        unsigned short g2[kH * kLd];  // G2[s][o]; also FP32 H2 staging before backward
-        unsigned short g2t[kH * kLd]; // [o][s]
+        alignas(16) unsigned short h1[kH * kLd];  // H1[s][i]
+        alignas(16) unsigned short h1t[kH * kLd]; // [i][s]
+        alignas(16) unsigned short g2[kH * kLd];  // G2[s][o]; also FP32 H2 staging before backward
+        alignas(16) unsigned short g2t[kH * kLd]; // [o][s]
         float xs[kH], ts[kH], g3[kH];
     } wave[2];
 };
@@ -175,20 +182,55 @@ constexpr int kSampleBlocks = 2;
 constexpr int kWavesPerBlock = 4;
 constexpr int kTrainLanes = 64 * kSampleBlocks * kWavesPerBlock; // 512
 constexpr int kPer = (kParams + kTrainLanes - 1) / kTrainLanes;  // parameters per lane (9)
-constexpr int kSamplerWave = 3; // the wave of each block that draws its samples
-constexpr int kAttempts = 8;    // rejection-sampling attempts per sample
+constexpr int kAttempts = 8; // rejection-sampling attempts per sample (two Philox calls)
+
+// Single-wave roles, placed so that the two blocks' busy waves of a phase sit on different
+// SIMDs (wave w of the workgroup runs on SIMD w % 4): the output wave sums each sample's
+// output-layer row, the sampler wave draws the next step's batch, the column wave combines
+// the row tiles' column sums.
+__device__ __forceinline__ int output_wave(int bi) { return bi ? 2 : 0; }
+__device__ __forceinline__ int sampler_wave(int bi) { return bi ? 1 : 3; }
+__device__ __forceinline__ int column_wave(int bi) { return bi ? 3 : 1; }
+
+// All four words of Philox4x32-10({c0,c1,c2,c3}, {k0,k1}) (the same rounds as philox_w0).
+__device__ __forceinline__ void philox4(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c[0];
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c[2];
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c[1], k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c[3], k1);
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+}
+
+// a mod n for 32-bit a and n >= 1 without a division: M = floor((2^64 - 1) / n) + 1 (Lemire,
+// Kaser and Kurz, "Faster remainder by direct computation", 2019).
+__device__ __forceinline__ uint32_t fastmod_u32(uint32_t a, uint64_t m, uint32_t n) {
+    return (uint32_t)__umul64hi(m * a, (uint64_t)n);
+}
 
 struct TrainLds {
-    unsigned short w2[kH * kLd];  // W2[o][i]
-    unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
-    float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
+    alignas(16) unsigned short w2[kH * kLd];  // W2[o][i]
+    alignas(16) unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
+    alignas(16) float w1[2 * kH];
+    alignas(16) float b1[kH];
+    float b2[kH], w3[kH], b3;
     float grad[kSampleBlocks][kParams]; // per sample block, summed in block order by Adam
     struct Block {
-        unsigned short h1[kH * kLd];  // H1[s][i]
-        unsigned short h1t[kH * kLd]; // [i][s]
-        unsigned short g2[kH * kLd];  // G2[s][o]; FP32 h2 * w3 staging before the backward
-        unsigned short g2t[kH * kLd]; // [o][s]
-        float xs[kH], ts[kH], ys[kH], g3[kH];
+        alignas(16) unsigned short h1[kH * kLd];  // H1[s][i]
+        alignas(16) unsigned short h1t[kH * kLd]; // [i][s]
+        alignas(16) unsigned short g2[kH * kLd];  // G2[s][o]; FP32 h2 * w3 staging before the backward
+        alignas(16) unsigned short g2t[kH * kLd]; // [o][s]
+        alignas(16) float xs[kH];
+        alignas(16) float ts[kH];
+        alignas(16) float ys[kH];
+        alignas(16) float g3[kH];
         float part[5][2][kH]; // dW3, db2, dW1[:,0], dW1[:,1], db1 column sums per row tile
     } blk[kSampleBlocks];
 };
@@ -200,10 +242,10 @@ __device__ __forceinline__ void tile_gemm64(const unsigned short* a, const unsig
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, kg = lane >> 5;
 #pragma unroll
-    for (int kk = 0; kk < kH; kk += 8) {
-        const s4v af = *(const s4v*)(a + (tm * 32 + row) * kLd + kk + 4 * kg);
-        const s4v bf = *(const s4v*)(bt + (tn * 32 + row) * kLd + kk + 4 * kg);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af, bf, acc, 0, 0, 0);
+    for (int kk = 0; kk < kH; kk += 16) {
+        const bf8v af = *(const bf8v*)(a + (tm * 32 + row) * kLd + kk + 8 * kg);
+        const bf8v bf = *(const bf8v*)(bt + (tn * 32 + row) * kLd + kk + 8 * kg);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, bf, acc, 0, 0, 0);
     }
 }
 
@@ -246,24 +288,42 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         wa[k] = 0.f;
     }
     // Sampling: rejection sampling over the valid (non-NaN target) samples, up to kAttempts
-    // attempts, the draw index being the attempt, so every sample's choice is a fixed function
-    // of (seed, step, sample); after kAttempts invalid draws the sample is masked. The
-    // sampler wave of each block prefetches: it issues all attempts' target loads for the next
-    // step while the output wave works, picks the first valid attempt and issues its feature
-    // loads while another wave combines column sums, and stores the sample at the next step's
-    // start, so no HBM round trip sits on the step's critical path.
+    // attempts, so every sample's choice is a fixed function of (seed, step, sample); after
+    // kAttempts invalid draws the sample is masked. Attempt j of sample c at step st is word
+    // j % 4 of Philox(key seed, counter {c, 0, j / 4, 7}) with c = st * 128 + sample, reduced
+    // mod n (beyond 2^32 samples: words j of two calls make 64 bits). The sampler wave of each
+    // block prefetches: it issues all attempts' target loads for the next step while the
+    // output wave works, picks the first valid attempt and issues its feature loads while the
+    // column wave works, and stores the sample at the next step's start, so no HBM round trip
+    // sits on the step's critical path.
     const int sample = bi * 64 + lane;
+    const int q_out = output_wave(bi), q_smp = sampler_wave(bi), q_col = column_wave(bi);
     uint64_t pidx[kAttempts];
     float pty[kAttempts];
     float nx = 0.f, nt = 0.f, ny = 0.f;
+    const bool wide = a.n > 0xFFFFFFFFull; // uniform
+    const uint64_t fm = wide ? 0 : 0xFFFFFFFFFFFFFFFFull / a.n + 1;
     auto issue = [&](int st) {
         const uint64_t ctr = (uint64_t)st * 128 + sample;
 #pragma unroll
-        for (uint32_t tr = 0; tr < kAttempts; ++tr) {
-            uint64_t r = philox_bits(a.seed, ctr, tr, 7);
-            if (a.n > 0xFFFFFFFFull) r |= (uint64_t)philox_bits(a.seed, ctr, tr + 8, 7) << 32;
-            pidx[tr] = r % a.n;
-            pty[tr] = a.ys[pidx[tr]];
+        for (uint32_t h = 0; h < kAttempts / 4; ++h) {
+            uint32_t w[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), h, 7u};
+            philox4(w, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+            uint32_t v[4] = {0u, 0u, 0u, 0u};
+            if (wide) {
+                v[0] = (uint32_t)ctr;
+                v[1] = (uint32_t)(ctr >> 32);
+                v[2] = h + kAttempts / 4;
+                v[3] = 7u;
+                philox4(v, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int tr = 4 * h + j;
+                pidx[tr] = wide ? (((uint64_t)v[j] << 32) | w[j]) % a.n
+                                : fastmod_u32(w[j], fm, (uint32_t)a.n);
+                pty[tr] = a.ys[pidx[tr]];
+            }
         }
     };
     auto choose = [&]() {
@@ -280,43 +340,59 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         nt = valid ? a.ts[idx] : 0.f;
         ny = y;
     };
-    if (!a.grad_out && q == kSamplerWave) {
+    // The weights and the batch of step 0; later steps get theirs from the previous step's
+    // update phase.
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = tid + kTrainLanes * k;
+        if (i < kParams) scatter_param(L, i, wr[k]);
+    }
+    // sample s of block bi is the batch's sample bi * 64 + s
+    if (a.grad_out) {
+        if (q == 0) {
+            const uint64_t idx = (uint64_t)sample < a.n ? (uint64_t)sample : 0;
+            const float y = a.ys[idx];
+            const bool valid = y == y;
+            B.xs[lane] = valid ? a.xs[idx] : 0.f;
+            B.ts[lane] = valid ? a.ts[idx] : 0.f;
+            B.ys[lane] = y;
+        }
+    } else if (q == q_smp) {
         issue(0);
         choose();
+        B.xs[lane] = nx;
+        B.ts[lane] = nt;
+        B.ys[lane] = ny;
     }
+    __syncthreads();
     for (int step = 0; step < a.steps; ++step) {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int i = tid + kTrainLanes * k;
-            if (i < kParams) scatter_param(L, i, wr[k]);
-        }
-        // sample s of block bi is the batch's sample bi * 64 + s
-        if (a.grad_out) {
-            if (q == 0) {
-                const uint64_t idx = (uint64_t)sample < a.n ? (uint64_t)sample : 0;
-                const float y = a.ys[idx];
-                const bool valid = y == y;
-                B.xs[lane] = valid ? a.xs[idx] : 0.f;
-                B.ts[lane] = valid ? a.ts[idx] : 0.f;
-                B.ys[lane] = y;
-            }
-        } else if (q == kSamplerWave) { // chosen and loaded during the previous step
-            B.xs[lane] = nx;
-            B.ts[lane] = nt;
-            B.ys[lane] = ny;
-        }
-        __syncthreads();
         // first layer: hidden units 16q .. 16q+15 of this lane's sample
         {
             const float x = B.xs[lane], t = B.ts[lane];
-#pragma unroll 4
-            for (int u = 0; u < 16; ++u) {
-                const int i = q * 16 + u;
-                const float a1 = __builtin_fmaf(L.w1[2 * i], x, __builtin_fmaf(L.w1[2 * i + 1], t, L.b1[i]));
-                const unsigned short h = f2bf(silu(a1));
-                B.h1[lane * kLd + i] = h;
-                B.h1t[i * kLd + lane] = h;
+            // the wave's 16 units: weights by 16-byte broadcast reads, the lane's 16 values
+            // of H1 as two 16-byte stores (its row segment is contiguous)
+            const float4* w1v = (const float4*)(L.w1 + 32 * q);
+            const float4* b1v = (const float4*)(L.b1 + 16 * q);
+            uint32_t packed[8];
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) {
+                const float4 wa = w1v[2 * u4], wb = w1v[2 * u4 + 1], bb = b1v[u4];
+                const float w[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+                const float b[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+                for (int j = 0; j < 4; j += 2) {
+                    const int u = 4 * u4 + j, i = q * 16 + u;
+                    const float a0 = __builtin_fmaf(w[2 * j], x, __builtin_fmaf(w[2 * j + 1], t, b[j]));
+                    const float a1 = __builtin_fmaf(w[2 * j + 2], x, __builtin_fmaf(w[2 * j + 3], t, b[j + 1]));
+                    const uint32_t h = f2bf2(silu(a0), silu(a1));
+                    B.h1t[i * kLd + lane] = (unsigned short)h;
+                    B.h1t[(i + 1) * kLd + lane] = (unsigned short)(h >> 16);
+                    packed[u >> 1] = h;
+                }
             }
+            uint4* row = (uint4*)(B.h1 + lane * kLd + 16 * q);
+            row[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+            row[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
         }
         __syncthreads();
         // second layer on this wave's tile; stage h2 * w3 per (s, o) in FP32
@@ -336,8 +412,8 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         }
         __syncthreads();
         float l2 = 0.f;
-        if (!a.grad_out && q == kSamplerWave && step + 1 < a.steps) issue(step + 1);
-        if (q == 0) { // output and dL/dy of this lane's sample (row sum in column order)
+        if (!a.grad_out && q == q_smp && step + 1 < a.steps) issue(step + 1);
+        if (q == q_out) { // output and dL/dy of this lane's sample (row sum in column order)
             float yhat = L.b3;
             for (int o = 0; o < kH; ++o) yhat += h2w[lane * 65 + o];
             const float y = B.ys[lane];
@@ -353,17 +429,29 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
             const int o = acc_col(tn);
             const float w3o = L.w3[o];
             float dw3 = 0.f, db2 = 0.f;
+            // rows come in runs of four consecutive samples (accumulator registers 4rg..4rg+3):
+            // their dL/dy by one 16-byte read, their G2^T entries by one 8-byte store
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int s = acc_row(tm, r);
-                const float a2 = acc[r];
-                const float gs = B.g3[s];
-                dw3 += gs * silu(a2);
-                const float g2 = gs * w3o * silu_grad(a2);
-                db2 += g2;
-                const unsigned short gb = f2bf(g2);
-                B.g2[s * kLd + o] = gb;
-                B.g2t[o * kLd + s] = gb;
+            for (int rg = 0; rg < 4; ++rg) {
+                const int s0 = acc_row(tm, 4 * rg);
+                const float4 g3v = *(const float4*)(B.g3 + s0);
+                const float gsv[4] = {g3v.x, g3v.y, g3v.z, g3v.w};
+                float g2v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float a2 = acc[4 * rg + j];
+                    const float gs = gsv[j];
+                    const float sg = sigmoid(a2); // silu and its derivative share it
+                    dw3 += gs * (a2 * sg);
+                    g2v[j] = gs * w3o * (sg * (1.0f + a2 * (1.0f - sg)));
+                    db2 += g2v[j];
+                }
+                const uint32_t p01 = f2bf2(g2v[0], g2v[1]), p23 = f2bf2(g2v[2], g2v[3]);
+                B.g2[s0 * kLd + o] = (unsigned short)p01;
+                B.g2[(s0 + 1) * kLd + o] = (unsigned short)(p01 >> 16);
+                B.g2[(s0 + 2) * kLd + o] = (unsigned short)p23;
+                B.g2[(s0 + 3) * kLd + o] = (unsigned short)(p23 >> 16);
+                *(uint2*)(B.g2t + o * kLd + s0) = make_uint2(p01, p23);
             }
             dw3 += __shfl_xor(dw3, 32, 64);
             db2 += __shfl_xor(db2, 32, 64);
@@ -392,14 +480,19 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
             const float w1a = L.w1[2 * i], w1b = L.w1[2 * i + 1], b1i = L.b1[i];
             float dw1a = 0.f, dw1b = 0.f, db1 = 0.f;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int s = acc_row(tm, r);
-                const float x = B.xs[s], t = B.ts[s];
-                const float a1 = __builtin_fmaf(w1a, x, __builtin_fmaf(w1b, t, b1i));
-                const float g = g1[r] * silu_grad(a1);
-                dw1a += g * x;
-                dw1b += g * t;
-                db1 += g;
+            for (int rg = 0; rg < 4; ++rg) {
+                const int s0 = acc_row(tm, 4 * rg);
+                const float4 xv = *(const float4*)(B.xs + s0), tv = *(const float4*)(B.ts + s0);
+                const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, ta[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float x = xa[j], t = ta[j];
+                    const float a1 = __builtin_fmaf(w1a, x, __builtin_fmaf(w1b, t, b1i));
+                    const float g = g1[4 * rg + j] * silu_grad(a1);
+                    dw1a += g * x;
+                    dw1b += g * t;
+                    db1 += g;
+                }
             }
             dw1a += __shfl_xor(dw1a, 32, 64);
             dw1b += __shfl_xor(dw1b, 32, 64);
@@ -410,15 +503,15 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
                 B.part[4][tm][i] = db1;
             }
         }
-        if (q == 0) {
+        if (q == q_out) {
             float gb3 = B.g3[lane];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) gb3 += __shfl_xor(gb3, off, 64);
             if (lane == 0) gw[oB3] = gb3;
         }
         __syncthreads();
-        if (!a.grad_out && q == kSamplerWave && step + 1 < a.steps) choose();
-        if (q == 1) { // column sums of the two row tiles
+        if (!a.grad_out && q == q_smp && step + 1 < a.steps) choose();
+        if (q == q_col) { // column sums of the two row tiles
             const int c = lane;
             gw[oW3 + c] = B.part[0][0][c] + B.part[0][1][c];
             gw[oB2 + c] = B.part[1][0][c] + B.part[1][1][c];
@@ -429,11 +522,10 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
         __syncthreads();
         if (a.grad_out) {
             for (int i = tid; i < kParams; i += kTrainLanes) a.grad_out[i] = L.grad[0][i] + L.grad[1][i];
-            if (tid == 0 && a.loss_out) a.loss_out[0] = l2;
-            if (tid == 256 && a.loss_out) a.loss_out[1] = l2;
+            if (q == q_out && lane == 0 && a.loss_out) a.loss_out[bi] = l2; // each block's sum
             return;
         }
-        if (tid == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // first sample block
+        if (bi == 0 && q == q_out && lane == 0 && a.loss_out) a.loss_out[step] = l2 / 64.0f; // first block
         // Adam (FP32)
         bc1 *= b1c;
         bc2 *= b2c;
@@ -457,7 +549,21 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
             for (int k = 0; k < kPer; ++k) wa[k] += (wr[k] - wa[k]) * inv;
         }
         lr *= a.lr_decay;
-        __syncthreads(); // every lane's reads of the LDS weights and gradients are done
+        // the next step's weights and batch: every read of the LDS weights and of the batch
+        // ended before the previous barrier
+        if (step + 1 < a.steps) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int i = tid + kTrainLanes * k;
+                if (i < kParams) scatter_param(L, i, wr[k]);
+            }
+            if (q == q_smp) { // chosen and loaded during this step
+                B.xs[lane] = nx;
+                B.ts[lane] = nt;
+                B.ys[lane] = ny;
+            }
+        }
+        __syncthreads(); // the next step's weights and batch are in place; gradients read
     }
     const bool averaged = a.avg_from < a.steps - 1;
 #pragma unroll

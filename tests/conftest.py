@@ -5,6 +5,16 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "derivative-pricing_amd", "python"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+# PyTorch-ROCm ships its own HIP runtime, which its libraries name by file name rather than by
+# the soname libmcd_hip.so links against. Loaded first, torch's runtime also serves the
+# engine (one runtime per process); loaded after the engine's, it starts a second runtime
+# that finds no device. Tests that use torch streams therefore need torch imported before
+# any test module loads the engine.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a visible HIP device (MI355X)")

@@ -115,8 +115,9 @@ def test_learned_american_put_c5():
     # Out of sample (fresh paths, streaming, no path store) and at other spots (features are
     # S/K): an exercise policy run forward on fresh paths is a low-biased estimator of the
     # option value (no policy beats the optimal one), so each price is judged against the QR
-    # policy on the same paths (within 1 SE) and may not exceed the Bermudan value by more than
-    # 3 SE; its low bias against that value is printed.
+    # policy on the same paths -- it may not fall more than 1 SE below it (a better policy may
+    # lie above it: the paired gain test measures that) -- and may not exceed the Bermudan
+    # value by more than 3 SE; its low bias against that value is printed.
     for spot in (90.0, 100.0, 110.0):
         args = (spot,) + PUT[1:]
         s = mcdx.lsm_learned_price(*args, 1 << 19, 4242, FP32, w, 1024, d["table_lo"],
@@ -124,7 +125,7 @@ def test_learned_american_put_c5():
         b = crr_bermudan_put(spot)
         print(spot, s["learned"], s["qr_forward"], "bermudan", b,
               "low bias (SE)", (s["learned"][0] - b) / s["learned"][1], s["fallbacks"])
-        assert abs(s["learned"][0] - s["qr_forward"][0]) <= s["learned"][1]
+        assert s["learned"][0] >= s["qr_forward"][0] - s["learned"][1]
         assert s["learned"][0] < b + 3 * s["learned"][1]
         assert s["qr_forward"][0] < b + 3 * s["qr_forward"][1]
 
