@@ -137,7 +137,7 @@ def test_network_policy_beats_qr_policy_out_of_sample(n_train):
     the network's exercise policy earns more than the QR policy fitted on those paths when
     both run on the same eight fresh sets of 2^20 paths (a forward policy value is low-biased,
     so more is better). The paired differences share their paths, so their spread is far
-    below either price's SE. Measured: +0.0060 +- 0.0004 (2^16), +0.0015 +- 0.0002 (2^20)."""
+    below either price's SE. Measured: +0.0055 +- 0.0005 (2^16), +0.0039 +- 0.0004 (2^20)."""
     d, w, co, has = mcdx.lsm_learned(*PUT, n_train, 42, FP32)
     diffs = []
     for k in range(8):
