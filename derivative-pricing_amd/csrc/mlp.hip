@@ -175,9 +175,12 @@ struct TrainArgs {
 
 // Training layout: one workgroup of 8 waves. A step's batch is two blocks of 64 samples (lane
 // = sample); each block's four waves split its work by accumulator tile: wave q of a block
-// owns tile (tm, tn) = (q / 2, q % 2) of every 64x64 product (one 32x32x8 MFMA chain instead
+// owns tile (tm, tn) = (q / 2, q % 2) of every 64x64 product (one 32x32x16 MFMA chain instead
 // of four) and the elementwise work on that tile, and hidden units 16q .. 16q+15 of the first
 // layer. Column sums over a tile's rows are combined across the two row tiles through LDS.
+// A step is seven barrier-separated phases: first layer; second layer and output staging;
+// output and dL/dy; output-layer backward; dW2 and the first-layer backward; column sums;
+// Adam with the next step's weights and batch written to LDS.
 constexpr int kSampleBlocks = 2;
 constexpr int kWavesPerBlock = 4;
 constexpr int kTrainLanes = 64 * kSampleBlocks * kWavesPerBlock; // 512
@@ -651,7 +654,7 @@ uint32_t mcdx_mlp_param_count(void) { return kParams; }
 // Gradient of the batch-mean squared error (weights FP32; BF16 MFMA products, FP32
 // accumulation) for n <= 128 samples (the batch is padded by repeating sample 0 when n <
 // 128; callers pass exactly 128 to compare with an exact reference). loss2[0..1] receive the
-// two waves' summed squared errors.
+// two 64-sample blocks' summed squared errors.
 int mcdx_mlp_gradients(const float* weights, const float* xs, const float* ts, const float* ys,
                        uint32_t n, float* grad, float* loss2) {
     if (!weights || !xs || !ts || !ys || !grad || n == 0 || n > 128) return MCDX_EINVAL;
