@@ -11,14 +11,24 @@
 //   laguerre_basis           :21-25, householder_least_squares src/core/linalg.cpp:7-71
 //
 // Layout: the store is SoA [date][path] (dates 1..M; the t=0 column is the constant S0),
-// FP32 in FP32 mode (2^20 paths x 50 dates = 210 MB), FP64 in FP64 mode; the cash flows are
-// one FP64 vector. Each date runs one fused kernel: solve the later date's regression from
-// its block factors, apply its exercise decision, discount, and fold the ITM rows
-// [basis(S/K) | cf] of each lane into an FP64 QR state (R upper-triangular 4x4 plus Q^T y;
-// blocked Householder steps per lane), reduce the block's stacked lane factors to one factor
-// by Householder reflections with block-wide sums (tall-skinny QR) and write it. The
-// least-squares solution is the reference's; only the rounding differs from its sequential
-// Householder sweep.
+// FP32 in FP32 mode (2^20 paths x 50 dates = 210 MB), FP64 in FP64 mode. Each path's cash
+// flow is kept discounted to t = 0, intrinsic(S at its exercise date e) x disc^e with the powers
+// of the one-date discount factor formed by repeated multiplication (the reference's
+// cf *= disc per date, lsm.cpp:84-86, up to its rounding), so a date's kernel writes a path's
+// value only when it exercises; a date's regression target is that value over disc^date, a
+// factor common to the date's rows that scales the coefficients once. A date's regression rows [basis(S/K)] are those of the paths in the money
+// at that date, fixed by the store alone; only the target (the discounted cash flows) depends
+// on the later dates' decisions. So the R factor of every date's rows is formed before the
+// backward induction, for all dates at once (blocked Householder steps per lane, then
+// Householder reflections of the stacked lane and block factors: tall-skinny QR in FP64).
+// Each date of the backward induction then runs one kernel: solve the later date's
+// regression, apply its exercise decision, discount, and accumulate the normal equations of
+// the preconditioned rows w = R^-T basis (G' = sum w w^T, which is the identity up to
+// rounding, and q = sum w cf), reduced per block. The solve is c = R^-1 G'^-1 q with G' by
+// Cholesky: the second step of Cholesky QR on rows already orthogonalised by R, as accurate as
+// the reference's Householder sweep for its condition numbers (about 2e6 at the first dates);
+// only the rounding differs from the reference's sequential Householder sweep.
+
 #include <math.h>
 
 #include <vector>
@@ -30,79 +40,27 @@
 namespace mcdx {
 
 constexpr int kLsmBasis = 4;
-constexpr int kLsmBlock = 256;
+constexpr int kLsmPathLanes = 256; // one path per lane: store and cash-flow init kernels
 
-// R (upper triangle, row-major packed 10 values) | z = Q^T y (4) | count of rows folded.
-struct QrState {
+// A date's preconditioner: P upper triangular (row-major packed 10 values) with
+// P^T P close to the Gram matrix of the date's in-the-money rows, and the reciprocals of its
+// diagonal (0 where the diagonal is 0: that column is dropped, as linalg.cpp:65-68 drops a
+// zero-diagonal column in the back-substitution).
+struct DateFactor {
     double r[10];
-    double z[4];
-    uint32_t rows;
+    double inv_d[4];
 };
 
 __host__ __device__ __forceinline__ int ridx(int i, int j) { // i <= j
     return i * kLsmBasis - (i * (i - 1)) / 2 + (j - i);
 }
 
-// Fold NR rows into (R, z) by Householder reflections of the stacked (4 + NR) x 5 matrix
-// [R z; rows ys] (linalg.cpp:7-57 applied to the lane's block): per column, the reflection
-// v = x - alpha e1 with alpha = -sign(x1) |x| and tau = 1 / (alpha (alpha - x1)) maps the
-// column below the diagonal to zero and is applied to the columns on its right. About a third
-// of the FP64 operations of NR Givens row rotations. A column that is zero from the diagonal down
-// is left as it is (tau = 0), as is every all-zero row (out-of-the-money paths).
-template <int NR>
-__device__ __forceinline__ void householder_fold_rows(QrState& st, const double (&rows)[NR][4],
-                                                      const double (&ys)[NR]) {
-    constexpr int M = kLsmBasis + NR;
-    double A[M][kLsmBasis + 1];
-#pragma unroll
-    for (int i = 0; i < kLsmBasis; ++i) {
-#pragma unroll
-        for (int j = 0; j < kLsmBasis; ++j) A[i][j] = j >= i ? st.r[ridx(i, j)] : 0.0;
-        A[i][kLsmBasis] = st.z[i];
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-#pragma unroll
-        for (int j = 0; j < kLsmBasis; ++j) A[kLsmBasis + r][j] = rows[r][j];
-        A[kLsmBasis + r][kLsmBasis] = ys[r];
-    }
-#pragma unroll
-    for (int j = 0; j < kLsmBasis; ++j) {
-        // Rows j+1 .. kLsmBasis-1 are R's rows, zero in column j (below R's diagonal), so
-        // the reflection runs over row j and the NR streamed rows only.
-        double norm2 = A[j][j] * A[j][j];
-#pragma unroll
-        for (int i = kLsmBasis; i < M; ++i) norm2 = __builtin_fma(A[i][j], A[i][j], norm2);
-        const double x1 = A[j][j];
-        const bool live = norm2 > 0.0;
-        // |x| = norm2 / sqrt(norm2) and 1 / (alpha (alpha - x1)) by the hardware reciprocal
-        // square root and reciprocal refined by Newton steps (within an ulp or two of the IEEE
-        // square root and division, at a fraction of their dependent latency).
-        const double nrm = live ? norm2 * fast_rsqrt_f64(norm2) : 0.0;
-        const double alpha = x1 > 0.0 ? -nrm : nrm;
-        const double tau = live ? fast_rcp_f64(alpha * (alpha - x1)) : 0.0;
-        const double v0 = x1 - alpha; // v = (v0, A[kLsmBasis.., j])
-#pragma unroll
-        for (int k = j + 1; k <= kLsmBasis; ++k) {
-            double d = v0 * A[j][k];
-#pragma unroll
-            for (int i = kLsmBasis; i < M; ++i) d = __builtin_fma(A[i][j], A[i][k], d);
-            const double f = tau * d;
-            A[j][k] = __builtin_fma(-f, v0, A[j][k]);
-#pragma unroll
-            for (int i = kLsmBasis; i < M; ++i) A[i][k] = __builtin_fma(-f, A[i][j], A[i][k]);
-        }
-        A[j][j] = live ? alpha : x1;
-#pragma unroll
-        for (int i = kLsmBasis; i < M; ++i) A[i][j] = live ? 0.0 : A[i][j];
-    }
-#pragma unroll
-    for (int i = 0; i < kLsmBasis; ++i) {
-#pragma unroll
-        for (int j = i; j < kLsmBasis; ++j) st.r[ridx(i, j)] = A[i][j];
-        st.z[i] = A[i][kLsmBasis];
-    }
-}
+// Sums per date: the Gram matrix of the rows (upper triangle, packed like P) and the row
+// count (preconditioner pass); G' = W^T W (packed), q = W^T y and the row count (backward
+// induction).
+constexpr int kGram = 11;
+constexpr int kNe = 15;
+constexpr int kLsmNeStride = 256; // block sums per value in the ne buffer (at most 256 blocks)
 
 // The reference's basis; x^3 / 6 is formed as x^3 * (1/6) (within an ulp of the division,
 // without its dependent FP64 division sequence). Every kernel that fits or replays a policy
@@ -125,6 +83,7 @@ __host__ __device__ __forceinline__ double intrinsic_v(double s, double k, int t
 
 struct LsmArgs {
     double spot, strike, disc, dt;
+    double disc_m;         // disc^M by M multiplications (disc_pow)
     double inv_strike;     // 1/K: the regression basis is taken at S * (1/K) (one multiply
                            // per use instead of a division; every decision uses the same)
     double drift, vsd;     // FP64: per-step GBM constants (gbm.cpp:8-10 with dt)
@@ -133,35 +92,43 @@ struct LsmArgs {
     int32_t m;             // exercise dates (monitoring points)
     uint32_t blocks;       // backward-induction layout: blocks and paths per lane
     uint32_t per_lane;
+    uint64_t sample_n;     // the first sample_n paths precondition every date
     uint8_t type;
     void* store;           // [date 1..M][path], float or double
-    double* cf;            // [path]
+    double* val;           // [path]: the path's cash flow discounted to t = 0
+    double* dpow;          // [M + 1]: disc^k by repeated multiplication
     double* coeffs;        // [(M+1) x 4] device
     uint8_t* has;          // [M+1] device
-    QrState* qr_partials;  // [2][blocks]: the block factors of even and odd dates
+    double* gram;          // [date 1..M-1][kGram][blocks]: block Gram sums of each date's rows
+    DateFactor* factors;   // [M+1]: each date's preconditioner (dates 1..M-1)
+    double* ne;            // [2][kNe][kLsmMaxBlocks]: block sums of the normal equations of
+                           // even and odd dates (zero past the blocks)
     Welford* partials;     // [tiles]
     // Optional capture of the regression samples for the network, [date 1..M-1][path]:
     // features (4 (S/K - 1), 2 date/M - 1) and target 10 cf/K (NaN when out of the money).
     float* cap_f1;
     float* cap_f2;
     float* cap_y;
-    // A batch of independent runs of the same contract on different stores (grid y = run):
-    // the distance between consecutive runs' store (bytes), cash flows, block factors and
-    // tile partials; coeffs and has advance by (M+1) x 4 and M+1. Zero for a single run.
-    uint64_t run_store_bytes, run_cf, run_qr, run_tiles;
+    // A batch of independent runs of the same contract on different stores (grid y = run,
+    // grid z for the Gram kernel): the distance between consecutive runs' store (bytes),
+    // exercise states (paths), Gram sums, normal-equation sums and tile partials; coeffs, has
+    // and factors advance by (M+1) x 4, M+1 and M+1. Zero for a single run.
+    uint64_t run_store_bytes, run_cf, run_gram, run_ne, run_tiles;
 };
 
-// The arguments of run blockIdx.y of a batch (itself for a single run). The capture
-// buffers are not batched: only single runs capture.
-__device__ __forceinline__ void select_run(LsmArgs& a) {
-    const uint64_t y = blockIdx.y;
+// The arguments of run `y` of a batch (itself for a single run). The capture buffers are not
+// batched: only single runs capture.
+__device__ __forceinline__ void select_run(LsmArgs& a, uint64_t y) {
     a.store = (char*)a.store + y * a.run_store_bytes;
-    a.cf += y * a.run_cf;
+    a.val += y * a.run_cf;
     a.coeffs += y * (uint64_t)(a.m + 1) * 4;
     a.has += y * (uint64_t)(a.m + 1);
-    a.qr_partials += y * a.run_qr;
+    a.gram += y * a.run_gram;
+    a.factors += (a.run_gram ? y : 0) * (uint64_t)(a.m + 1);
+    a.ne += y * a.run_ne;
     a.partials += y * a.run_tiles;
 }
+__device__ __forceinline__ void select_run(LsmArgs& a) { select_run(a, blockIdx.y); }
 
 constexpr double kFeatScale = 4.0;  // f1 = kFeatScale * (S/K - 1)
 constexpr double kTargetScale = 10.0; // y = kTargetScale * cf / K
@@ -172,6 +139,7 @@ __device__ __forceinline__ double store_get(const LsmArgs& a, int date, uint64_t
     if constexpr (FP64) return ((const double*)a.store)[idx];
     else return (double)((const float*)a.store)[idx];
 }
+
 
 // Forward simulation into the store (lsm.cpp:35-58), one path per lane; coalesced
 // [date][path] writes. FP32 mode steps in log2 space like the path kernels.
@@ -214,8 +182,17 @@ __global__ __launch_bounds__(256) void lsm_paths_kernel(LsmArgs a) {
     }
 }
 
-// cf = intrinsic at maturity (lsm.cpp:76-81); block 0 of each run also clears the run's
-// policy (no coefficients at any date yet).
+// disc^k by k multiplications from 1: the discount powers backward induction applies (the
+// table dpow holds the same values).
+__host__ __device__ __forceinline__ double disc_pow(double disc, int k) {
+    double d = 1.0;
+    for (int i = 0; i < k; ++i) d *= disc;
+    return d;
+}
+
+// Every path's value at maturity, intrinsic x disc^M (lsm.cpp:76-81, discounted as :84-86 and
+// :125-127 will); block 0 of each run also clears the run's policy (no coefficients at any date
+// yet), and of run 0 forms the discount powers.
 template <bool FP64>
 __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
     select_run(a);
@@ -225,31 +202,57 @@ __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) a.coeffs[4 * i + j] = 0.0;
         }
+        for (int i = (int)threadIdx.x; i < 2 * kNe * kLsmNeStride; i += (int)blockDim.x) a.ne[i] = 0.0;
+        if (threadIdx.x == 0 && blockIdx.y == 0) {
+            double d = 1.0;
+            for (int k = 0; k <= a.m; ++k) {
+                a.dpow[k] = d;
+                d *= a.disc;
+            }
+        }
     }
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_paths) return;
-    a.cf[p] = intrinsic_v(store_get<FP64>(a, a.m, p), a.strike, a.type);
+    a.val[p] = intrinsic_v(store_get<FP64>(a, a.m, p), a.strike, a.type) * a.disc_m;
 }
 
 // ---- Backward induction -------------------------------------------------------------------
-// Path layout: B = min(256, ceil(n / 4096)) blocks of 256 lanes; lane t of block b owns
-// paths ((j * B + b) * 256 + t) for j < P = ceil(n / (256 B)), coalesced per j. Each date
-// runs one kernel (reduce the later date's B block factors to one and back-substitute,
-// decision at the later date, discount, fold the ITM rows into the lane's QR state, reduce
-// the block's lane states to the block factor). The layout depends on n only and every sum
-// runs in a fixed order, so the coefficients are bitwise reproducible.
-constexpr uint32_t kLsmMaxBlocks = 256;
-static_assert(kLsmMaxBlocks <= (uint32_t)kLsmBlock, "merge_solve_all loads one block factor per lane");
+// Path layout: B = min(256, ceil(n / 4096)) blocks of 512 lanes (two waves per SIMD); lane t
+// of block b owns paths ((j * B + b) * 512 + t) for j < P = ceil(n / (512 B)), coalesced per j
+// (P = 8 at 2^20 paths). Before the dates' kernels, the Gram kernels precondition every date at once (grid:
+// blocks x dates) from the rows of the first min(n, 2^16) paths; then each date runs one
+// kernel: the later date's
+// coefficients from the B blocks' normal-equation sums, the decision at the later date,
+// discount, and the normal equations of this date's preconditioned rows, summed over the
+// block. The layout depends on n only and every sum runs in a fixed order, so the
+// coefficients are bitwise reproducible.
+constexpr int kLsmLanes = 512;
+constexpr int kLsmWaves = kLsmLanes / 64;
+constexpr uint32_t kLsmMaxBlocks = kLsmNeStride;
+static_assert(kLsmMaxBlocks <= 256 && kLsmMaxBlocks <= (uint32_t)kLsmLanes,
+              "the merges load at most four blocks' sums per lane of wave 0, one per lane otherwise");
 constexpr uint32_t kLsmChunk = 8; // paths per lane whose loads are issued together
+// The sample that preconditions each date: the first kLsmSample paths (independent paths, so
+// a random sample of the date's rows; at 2^16 paths its Gram matrix is within about 1% of
+// the full one's in every direction, so the preconditioned rows stay within about 1% of
+// orthonormal). A date whose sample holds fewer than kLsmMinSample in-the-money rows is
+// preconditioned from every path instead.
+constexpr uint64_t kLsmSample = 1ull << 16;
+constexpr double kLsmMinSample = 1024.0;
+// Diagonal shift of the preconditioner's Cholesky, relative to the Gram trace: it keeps the
+// factorisation defined for rank-deficient or extremely ill-conditioned rows and is far
+// below the smallest eigenvalue ratio of the Laguerre rows (about 2e-13 at the first dates
+// of a 50-date put).
+constexpr double kLsmShift = 0x1p-46;
 
 __host__ __device__ __forceinline__ uint64_t lane_path(uint32_t j, uint32_t block,
                                                        uint32_t blocks, uint32_t t) {
-    return ((uint64_t)j * blocks + block) * kLsmBlock + t;
+    return ((uint64_t)j * blocks + block) * kLsmLanes + t;
 }
 
-// Exercise decision at a later date with coefficients beta (lsm.cpp:110-122).
-__device__ __forceinline__ double decide(const LsmArgs& a, double s1, double cf,
-                                         const double* beta) {
+// Exercise decision at a later date with coefficients beta (lsm.cpp:110-122): whether the
+// path exercises.
+__device__ __forceinline__ bool decide(const LsmArgs& a, double s1, const double* beta) {
     const double ex = intrinsic_v(s1, a.strike, a.type);
     if (ex > 0.0) {
         double b[4];
@@ -257,28 +260,87 @@ __device__ __forceinline__ double decide(const LsmArgs& a, double s1, double cf,
         double cont = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) cont += beta[j] * b[j];
-        if (ex > cont) return ex;
+        return ex > cont;
     }
-    return cf;
+    return false;
 }
 
-// One path's regression row at `date` (lsm.cpp:88-100): the Laguerre basis of S/K when the
-// path is in range and in the money, else zeros; returns whether it is. Captures the
-// network's training sample for every path in range.
-__device__ __forceinline__ bool fold_row(const LsmArgs& a, int date, uint64_t p, bool in,
-                                         double s0, double cf, double* row) {
+// A date's continuation value without its weight, sum_j beta_j L_j(x), as the cubic
+// c0 + c1 x + c2 x^2 + c3 x^3, and the bound sum_j |beta_j| (1 + x)^j on sum_j |beta_j L_j(x)|
+// and on the cubic's terms sum_k |c_k| x^k (for x >= 0, |L_j(x)| <= (1 + x)^j term by term).
+struct ContPoly {
+    double c[4], b[4];
+};
+__device__ __forceinline__ ContPoly cont_poly(const double* beta) {
+    constexpr double kSixth = 1.0 / 6.0;
+    ContPoly p;
+    p.c[0] = ((beta[0] + beta[1]) + beta[2]) + beta[3];
+    p.c[1] = -((beta[1] + 2.0 * beta[2]) + 3.0 * beta[3]);
+    p.c[2] = 0.5 * beta[2] + 1.5 * beta[3];
+    p.c[3] = -beta[3] * kSixth;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p.b[j] = fabs(beta[j]);
+    return p;
+}
+
+// decide() with the same result, cheaper: the continuation value is estimated as the weight
+// e^{-x/2} from the hardware FP32 exponential (relative error below 1.2e-7 + 9e-8 x: one ulp of
+// v_exp_f32 and the FP32 argument's rounding) times the cubic in FP64. The estimate differs
+// from decide()'s FP64 value by at most that relative error of the estimate plus the roundings
+// of both evaluations and of the cubic's coefficients, each a few 1e-16 of the bound
+// e sum_j |beta_j| (1 + x)^j (the sum cancels: its terms are 40-1000 times its value). Where
+// the intrinsic value is farther from the estimate than 4e-6 (1 + x) |estimate| + 1e-12
+// bound, the estimate's side of the boundary is decide()'s; the remaining paths (within a few
+// 1e-6 of the boundary, or with x outside [0, 64]) are decided by decide() itself.
+__device__ __forceinline__ bool decide_fast(const LsmArgs& a, double s1, const double* beta,
+                                            const ContPoly& cp) {
+    const double ex = intrinsic_v(s1, a.strike, a.type);
+    if (!(ex > 0.0)) return false;
+    const double x = s1 * a.inv_strike;
+    const double e = (double)fast_exp2((float)x * -0.721347520444481703680f); // e^{-x/2}
+    const double est = e * __builtin_fma(__builtin_fma(__builtin_fma(cp.c[3], x, cp.c[2]), x, cp.c[1]), x, cp.c[0]);
+    const double x1 = 1.0 + x;
+    const double bound = e * __builtin_fma(__builtin_fma(__builtin_fma(cp.b[3], x1, cp.b[2]), x1, cp.b[1]), x1, cp.b[0]);
+    const bool sure = x >= 0.0 && x <= 64.0 && fabs(ex - est) > 4e-6 * x1 * fabs(est) + 1e-12 * bound;
+    if (sure) return ex > est;
+    return decide(a, s1, beta);
+}
+
+// A row for the preconditioner only: the basis with its weight e^{-x/2} from the hardware FP32
+// exponential, which scales each row by 1 + O(1e-7); the preconditioned rows of the exact
+// basis are then orthonormal to within that.
+__device__ __forceinline__ bool design_row_approx(const LsmArgs& a, bool in, double s0, double* row) {
+    constexpr double kSixth = 1.0 / 6.0;
+    const bool itm = in && intrinsic_v(s0, a.strike, a.type) > 0.0;
+    const double x = s0 * a.inv_strike;
+    const double e = itm ? (double)fast_exp2((float)x * -0.721347520444481703680f) : 0.0;
+    const double x2 = x * x;
+    row[0] = e;
+    row[1] = e * (1.0 - x);
+    row[2] = e * (1.0 - 2.0 * x + 0.5 * x2);
+    row[3] = e * (1.0 - 3.0 * x + 1.5 * x2 - x2 * x * kSixth);
+    return itm;
+}
+
+// One path's regression row (lsm.cpp:88-100): the Laguerre basis of S/K when the path is in
+// range and in the money, else zeros; returns whether it is.
+__device__ __forceinline__ bool design_row(const LsmArgs& a, bool in, double s0, double* row) {
     const bool itm = in && intrinsic_v(s0, a.strike, a.type) > 0.0;
     double b[4];
     laguerre4(s0 * a.inv_strike, b);
 #pragma unroll
     for (int k = 0; k < 4; ++k) row[k] = itm ? b[k] : 0.0;
-    if (a.cap_y && in) {
-        const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
-        a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
-        a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
-        a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
-    }
     return itm;
+}
+
+// The network's training sample of a path in range at `date`: features (4 (S/K - 1),
+// 2 date/M - 1) and target 10 cf/K (NaN when out of the money).
+__device__ __forceinline__ void capture_row(const LsmArgs& a, int date, uint64_t p, bool itm,
+                                            double s0, double cf) {
+    const uint64_t c = (uint64_t)(date - 1) * a.n_paths + p;
+    a.cap_f1[c] = (float)(kFeatScale * (s0 / a.strike - 1.0));
+    a.cap_f2[c] = (float)(2.0 * (double)date / (double)a.m - 1.0);
+    a.cap_y[c] = itm ? (float)(kTargetScale * cf / a.strike) : __builtin_nanf("");
 }
 
 // DPP lane permutation of a double (both halves moved by the same control).
@@ -290,252 +352,425 @@ __device__ __forceinline__ double dpp_f64(double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
+// Sums of NV doubles over the 64 lanes of a wave, by transposition through the wave's own LDS
+// region wl ([NV][65], padded against bank conflicts): every lane writes its values, lane
+// 4 i + q (i < NV) adds value i of lanes 16 q .. 16 q + 15 in lane order, and two quad
+// exchanges add the quarters as (q0 + q1) + (q2 + q3) (both lanes of a pair add the same two
+// operands). Lane 4 i returns the wave's total of value i. The region is the wave's alone, so
+// a wave-level barrier (no block barrier) orders the writes before the reads; a later call
+// rewrites it only after this wave has read it.
+constexpr int kLsmWaveStride = 65;
+template <int NV>
+__device__ __forceinline__ double wave_sums(const double (&v)[NV], double* wl) {
+    const int l = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) wl[i * kLsmWaveStride + l] = v[i];
+    // the other lanes' writes visible to this wave's reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int i = l >> 2, q = l & 3;
+    double s = 0.0;
+    if (i < NV) {
+        const double* r = wl + i * kLsmWaveStride + q * 16;
+        s = r[0];
+#pragma unroll
+        for (int k = 1; k < 16; ++k) s += r[k];
+    }
+    s += dpp_f64<0xB1>(s); // quad_perm [1, 0, 3, 2]
+    s += dpp_f64<0x4E>(s); // quad_perm [2, 3, 0, 1]
+    return s;
+}
+
+// Block-wide sums of NV doubles: wave sums, the wave totals through LDS (one barrier), then
+// lane i < NV of every wave adds value i's kLsmWaves totals by a fixed pairwise tree and
+// returns it. The wave regions are rewritten only by their own wave; the totals array is
+// rewritten by a later call only after a barrier that follows every read of it.
+constexpr int kLsmWaveRegion = kNe * kLsmWaveStride;
+constexpr int kLsmLdsTotals = kLsmWaves * kLsmWaveRegion;      // [wave][kNe] totals
+constexpr int kLsmLdsRes = kLsmLdsTotals + kLsmWaves * kNe;     // the solve's [kNe + 5]
+constexpr int kLsmLdsDoubles = kLsmLdsRes + kNe + 5;
+template <int NV>
+__device__ __forceinline__ double block_sums(const double (&v)[NV], double* lds) {
+    static_assert(NV <= kNe && (kLsmWaves & (kLsmWaves - 1)) == 0, "exchange layout");
+    const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const double ws = wave_sums<NV>(v, lds + w * kLsmWaveRegion);
+    double* tot = lds + kLsmLdsTotals;
+    if ((l & 3) == 0 && (l >> 2) < NV) tot[w * NV + (l >> 2)] = ws;
+    __syncthreads();
+    double t[kLsmWaves];
+#pragma unroll
+    for (int k = 0; k < kLsmWaves; ++k) t[k] = l < NV ? tot[k * NV + l] : 0.0;
+#pragma unroll
+    for (int h = kLsmWaves / 2; h >= 1; h /= 2) {
+#pragma unroll
+        for (int k = 0; k < h; ++k) t[k] = t[2 * k] + t[2 * k + 1];
+    }
+    return t[0];
+}
+
+// The preconditioned row w = P^-T row by forward substitution; a dropped column (zero
+// diagonal) gives w_i = 0.
+__device__ __forceinline__ void precondition_row(const DateFactor& f, const double* row, double* w) {
+    w[0] = row[0] * f.inv_d[0];
+    w[1] = __builtin_fma(-f.r[ridx(0, 1)], w[0], row[1]) * f.inv_d[1];
+    w[2] = __builtin_fma(-f.r[ridx(1, 2)], w[1], __builtin_fma(-f.r[ridx(0, 2)], w[0], row[2])) *
+           f.inv_d[2];
+    w[3] = __builtin_fma(-f.r[ridx(2, 3)], w[2],
+                         __builtin_fma(-f.r[ridx(1, 3)], w[1],
+                                       __builtin_fma(-f.r[ridx(0, 3)], w[0], row[3]))) *
+           f.inv_d[3];
+}
+
+// ---- Preconditioners of every date (before the backward induction) ----------------------
+// Gram sums of the rows of one date over paths p < lim, as (j gb + b) kLsmLanes + t for
+// j < per (block b of gb, lane t), added to g; with PRE, of the rows preconditioned by pf.
+template <bool FP64, bool PRE>
+__device__ __forceinline__ void gram_rows(const LsmArgs& a, int date, uint32_t gb, uint32_t b,
+                                          uint32_t per, uint64_t lim, double (&g)[kGram],
+                                          const DateFactor& pf) {
+    for (uint32_t j0 = 0; j0 < per; j0 += kLsmChunk) {
+        double s0[kLsmChunk];
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u) {
+            const uint64_t p = ((uint64_t)(j0 + u) * gb + b) * kLsmLanes + threadIdx.x;
+            s0[u] = store_get<FP64>(a, date, (j0 + u < per && p < lim) ? p : 0);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u) {
+            const uint64_t p = ((uint64_t)(j0 + u) * gb + b) * kLsmLanes + threadIdx.x;
+            double row[4];
+            const bool itm = design_row_approx(a, j0 + u < per && p < lim, s0[u], row);
+            if constexpr (PRE) {
+                double w[4];
+                precondition_row(pf, row, w);
+#pragma unroll
+                for (int i = 0; i < kLsmBasis; ++i) row[i] = w[i];
+            }
+#pragma unroll
+            for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+                for (int k = i; k < kLsmBasis; ++k) g[ridx(i, k)] = __builtin_fma(row[i], row[k], g[ridx(i, k)]);
+            }
+            g[10] += itm ? 1.0 : 0.0;
+        }
+    }
+}
+
+// Grid (gb, M - 1, runs): block b of date y + 1 sums the Gram matrix and the count of the
+// rows of the sample (the first min(n, 2^16) paths). gb <= kLsmGramBlocks.
+constexpr uint32_t kLsmGramBlocks = 32;
+template <bool FP64>
+__global__ __launch_bounds__(kLsmLanes) void lsm_gram_kernel(LsmArgs a, uint32_t gb, uint32_t per) {
+    select_run(a, blockIdx.z);
+    const int date = (int)blockIdx.y + 1;
+    __shared__ double lds[kLsmLdsDoubles];
+    double g[kGram];
+#pragma unroll
+    for (int k = 0; k < kGram; ++k) g[k] = 0.0;
+    gram_rows<FP64, false>(a, date, gb, blockIdx.x, per, a.sample_n, g, DateFactor{});
+    const double t = block_sums<kGram>(g, lds);
+    if (threadIdx.x < (unsigned)kGram)
+        a.gram[((size_t)(date - 1) * kGram + threadIdx.x) * kLsmGramBlocks + blockIdx.x] = t;
+}
+
+// P upper triangular with P^T P = G + shift I (Cholesky; the shift, relative to the trace,
+// keeps it defined for rank-deficient or extremely ill-conditioned G), with its reciprocal
+// diagonal.
+__device__ __forceinline__ DateFactor shifted_cholesky(const double (&g)[kGram]) {
+    DateFactor f;
+    const double shift = kLsmShift * (g[ridx(0, 0)] + g[ridx(1, 1)] + g[ridx(2, 2)] + g[ridx(3, 3)]);
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+        double d = g[ridx(i, i)] + shift;
+#pragma unroll
+        for (int k = 0; k < i; ++k) d -= f.r[ridx(k, i)] * f.r[ridx(k, i)];
+        const bool live = d > 0.0;
+        const double pi = live ? sqrt(d) : 0.0;
+        f.r[ridx(i, i)] = pi;
+        f.inv_d[i] = live ? 1.0 / pi : 0.0;
+#pragma unroll
+        for (int j = i + 1; j < kLsmBasis; ++j) {
+            double t2 = g[ridx(i, j)];
+#pragma unroll
+            for (int k = 0; k < i; ++k) t2 -= f.r[ridx(k, i)] * f.r[ridx(k, j)];
+            f.r[ridx(i, j)] = t2 * f.inv_d[i];
+        }
+    }
+    return f;
+}
+
+// Grid (M - 1, runs): one block per date adds its gb blocks' Gram sums (lane b holds block
+// b's) and factors them, G + shift I = P1^T P1. The sample's rows (every path's, for a date
+// whose sample held fewer than kLsmMinSample rows while the sample is not every path: one
+// block, slow, but only for such thin dates) are then preconditioned by P1 and their Gram
+// matrix G2 = U^T U factored again: P = U P1 (a second Cholesky-QR step, which leaves the
+// preconditioned rows orthonormal to within rounding even where the first step's Gram matrix
+// was too ill-conditioned to factor accurately).
+template <bool FP64>
+__global__ __launch_bounds__(kLsmLanes) void lsm_gram_merge_kernel(LsmArgs a, uint32_t gb) {
+    select_run(a);
+    const int date = (int)blockIdx.x + 1;
+    __shared__ double lds[kLsmLdsDoubles];
+    double g[kGram];
+#pragma unroll
+    for (int k = 0; k < kGram; ++k)
+        g[k] = threadIdx.x < gb ? a.gram[((size_t)(date - 1) * kGram + k) * kLsmGramBlocks + threadIdx.x]
+                                : 0.0;
+    double* res = lds + kLsmLdsRes;
+    double t = block_sums<kGram>(g, lds);
+    if (threadIdx.x == 10) res[0] = t;
+    __syncthreads();
+    const bool thin = res[0] < kLsmMinSample && a.sample_n < a.n_paths; // uniform
+    // the rows of the second step: the sample's, as the Gram kernel's gb blocks cover them, or
+    // every path's
+    const uint64_t lim = thin ? a.n_paths : a.sample_n;
+    const uint32_t per = (uint32_t)((lim + kLsmLanes - 1) / kLsmLanes);
+    if (thin) {
+        __syncthreads(); // every lane has read res[0] before the next exchange
+#pragma unroll
+        for (int k = 0; k < kGram; ++k) g[k] = 0.0;
+        gram_rows<FP64, false>(a, date, 1, 0, per, lim, g, DateFactor{});
+        t = block_sums<kGram>(g, lds);
+    }
+    if (threadIdx.x < (unsigned)kGram) res[threadIdx.x] = t;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kGram; ++k) g[k] = res[k];
+    const DateFactor p1 = shifted_cholesky(g); // every lane, the same
+    __syncthreads(); // every lane has read res before the next exchange
+#pragma unroll
+    for (int k = 0; k < kGram; ++k) g[k] = 0.0;
+    gram_rows<FP64, true>(a, date, 1, 0, per, lim, g, p1);
+    t = block_sums<kGram>(g, lds);
+    if (threadIdx.x < (unsigned)kGram) res[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int k = 0; k < kGram; ++k) g[k] = res[k];
+    const DateFactor u = shifted_cholesky(g);
+    DateFactor f; // P = U P1, upper triangular
+#pragma unroll
+    for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+        for (int j = i; j < kLsmBasis; ++j) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = i; k <= j; ++k) sum += u.r[ridx(i, k)] * p1.r[ridx(k, j)];
+            f.r[ridx(i, j)] = sum;
+        }
+        const double d = f.r[ridx(i, i)];
+        f.inv_d[i] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+    a.factors[date] = f;
+}
+
+// ---- One date of the backward induction --------------------------------------------------
+__device__ __forceinline__ double uniform_f64(double v) {
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Sum over the wave's 64 lanes, bit-identical in every lane. Inside each row of 16 lanes:
-// lane ^ 1, lane ^ 2 (quad permutations), then the half-row and row mirrors, each pairing a
-// lane with one holding the other group's total, so both lanes of a pair add the same two
-// operands; then the four row totals as (r0 + r1) + (r2 + r3) from scalar reads.
-__device__ __forceinline__ double wave_sum_f64(double v) {
-    v += dpp_f64<0xB1>(v);  // quad_perm [1, 0, 3, 2]
-    v += dpp_f64<0x4E>(v);  // quad_perm [2, 3, 0, 1]
-    v += dpp_f64<0x141>(v); // row_half_mirror
-    v += dpp_f64<0x140>(v); // row_mirror
-    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+// A date's preconditioner (the same for every lane) held in scalar registers.
+__device__ __forceinline__ DateFactor uniform_factor(const DateFactor& g) {
+    DateFactor f;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) f.r[i] = uniform_f64(g.r[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.inv_d[i] = uniform_f64(g.inv_d[i]);
+    return f;
 }
 
-// Block-wide sums of NV doubles, bit-identical in every lane: wave sums, then the four wave
-// totals as (w0 + w1) + (w2 + w3). The block's lane 0 also publishes NP values through the
-// same exchange. Consecutive exchanges alternate between two LDS buffers (buf = 0, 1), so one
-// barrier per exchange suffices: a buffer is rewritten only after every lane has passed the
-// other buffer's barrier, i.e. finished reading this one.
-constexpr int kLsmXchg = 4 * 6 + 5; // doubles per exchange buffer (NV <= 6, NP <= 5)
-template <int NV, int NP>
-__device__ __forceinline__ void block_sums(double (&v)[NV], double (&pub)[NP], double* lds,
-                                           int buf) {
-    static_assert(4 * NV + NP <= kLsmXchg, "exchange buffer too small");
-    double* x = lds + buf * kLsmXchg;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = wave_sum_f64(v[i]);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) x[w * NV + i] = v[i];
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NP; ++i) x[4 * NV + i] = pub[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = (x[i] + x[NV + i]) + (x[2 * NV + i] + x[3 * NV + i]);
-#pragma unroll
-    for (int i = 0; i < NP; ++i) pub[i] = x[4 * NV + i];
-}
-constexpr int kLsmLdsDoubles = 2 * kLsmXchg + 8; // two exchange buffers, the solve's results
-
-// One column step of the block's stacked Householder reduction (block_factor). Every lane's
-// rows keep the upper-triangular zero pattern of a factor (row r is zero before column r:
-// the reflections of earlier columns add multiples of those zeros), so column J has non-zero
-// entries only in rows 0 .. J of each lane; lane 0's rows 0 .. J - 1 are finished and its row
-// J is the pivot, so the other lanes' rows 0 .. J form the column below the pivot.
-template <int J>
-__device__ __forceinline__ void stacked_reflect(double (&A)[kLsmBasis][kLsmBasis + 1],
-                                                bool lead, double& rows, double* lds) {
-    constexpr int NK = kLsmBasis - J;             // columns J+1 .. kLsmBasis (the last is z)
-    constexpr int NV = 1 + NK + (J == 0 ? 1 : 0); // sum of squares, dot products, row count
-    double v[NV], pub[1 + NK];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = 0.0;
-    if (!lead) {
-#pragma unroll
-        for (int r = 0; r <= J; ++r) {
-            const double x = A[r][J];
-            v[0] = __builtin_fma(x, x, v[0]);
-#pragma unroll
-            for (int k = 0; k < NK; ++k) v[1 + k] = __builtin_fma(x, A[r][J + 1 + k], v[1 + k]);
-        }
-    }
-    if constexpr (J == 0) v[NV - 1] = rows;
-#pragma unroll
-    for (int k = 0; k <= NK; ++k) pub[k] = lead ? A[J][J + k] : 0.0;
-    block_sums(v, pub, lds, J & 1);
-    if constexpr (J == 0) rows = v[NV - 1];
-    // The reflection v = x - alpha e_J of the stacked column (alpha = -sign(x1) |x|,
-    // tau = 1 / (alpha (alpha - x1))), applied to the columns on its right; a column that is
-    // zero from the pivot down is left as it is.
-    const double x1 = pub[0];
-    const double norm2 = __builtin_fma(x1, x1, v[0]);
-    const bool live = norm2 > 0.0;
-    const double nrm = live ? norm2 * fast_rsqrt_f64(norm2) : 0.0; // as householder_fold_rows
-    const double alpha = x1 > 0.0 ? -nrm : nrm;
-    const double tau = live ? fast_rcp_f64(alpha * (alpha - x1)) : 0.0;
-    const double v0 = x1 - alpha;
-#pragma unroll
-    for (int k = 0; k < NK; ++k) {
-        const double f = tau * __builtin_fma(v0, pub[1 + k], v[1 + k]);
-        if (lead) {
-            A[J][J + 1 + k] = __builtin_fma(-f, v0, A[J][J + 1 + k]);
-        } else {
-#pragma unroll
-            for (int r = 0; r <= J; ++r) A[r][J + 1 + k] = __builtin_fma(-f, A[r][J], A[r][J + 1 + k]);
-        }
-    }
-    if (lead) {
-        A[J][J] = live ? alpha : x1;
-    } else {
-#pragma unroll
-        for (int r = 0; r <= J; ++r) A[r][J] = live ? 0.0 : A[r][J];
-    }
-}
-
-// The block's combined factor: the lanes' [R | z] rows stacked in lane order (1024 x 5)
-// reduced to one upper-triangular [R | z] by Householder reflections, as linalg.cpp:7-57
-// reduces the full design matrix. Column j pivots on lane 0's row j; each column costs one
-// block-wide sum of the dot products it needs, in a fixed order, so the result is bitwise
-// reproducible. The factor is returned in lane 0 (other lanes: partial garbage).
-__device__ __forceinline__ QrState block_factor(const QrState& in, double* lds) {
-    double A[kLsmBasis][kLsmBasis + 1];
+// A date's coefficients from its normal-equation sums (G' packed upper, q) and preconditioner
+// P: the least-squares problem in the preconditioned rows W = A P^-1 is well conditioned, so
+// its normal equations are solved directly, c' = G'^-1 q by Cholesky G' = U^T U, and the
+// coefficients are beta = P^-1 c' by back-substitution (linalg.cpp:58-69; a zero diagonal
+// gives coefficient 0). A non-positive pivot of G' marks a dropped column: its component
+// is 0. Square roots and reciprocals are the hardware estimates refined by Newton steps.
+__device__ __forceinline__ void ne_solve(const double (&g)[kNe], const DateFactor& f, double* beta) {
+    double u[10], iu[4];
 #pragma unroll
     for (int i = 0; i < kLsmBasis; ++i) {
+        double d = g[ridx(i, i)];
 #pragma unroll
-        for (int j = 0; j < kLsmBasis; ++j) A[i][j] = j >= i ? in.r[ridx(i, j)] : 0.0;
-        A[i][kLsmBasis] = in.z[i];
-    }
-    const bool lead = threadIdx.x == 0;
-    double rows = (double)in.rows; // exact: at most 2^32
-    stacked_reflect<0>(A, lead, rows, lds);
-    stacked_reflect<1>(A, lead, rows, lds);
-    stacked_reflect<2>(A, lead, rows, lds);
-    stacked_reflect<3>(A, lead, rows, lds);
-    QrState o;
+        for (int k = 0; k < i; ++k) d = __builtin_fma(-u[ridx(k, i)], u[ridx(k, i)], d);
+        const bool live = d > 0.0;
+        iu[i] = live ? fast_rsqrt_f64(d) : 0.0;
 #pragma unroll
-    for (int i = 0; i < kLsmBasis; ++i) {
+        for (int j = i + 1; j < kLsmBasis; ++j) {
+            double t = g[ridx(i, j)];
 #pragma unroll
-        for (int j = i; j < kLsmBasis; ++j) o.r[ridx(i, j)] = A[i][j];
-        o.z[i] = A[i][kLsmBasis];
-    }
-    o.rows = (uint32_t)rows;
-    return o;
-}
-
-// Combine nb <= 256 block factors (lane b holds factor b, the other lanes zeros) and back-substitute
-// (linalg.cpp:58-69) when at least four ITM rows were folded (lsm.cpp:102-104). Every lane of
-// the 256-lane block receives beta and has.
-__device__ __forceinline__ void merge_solve_all(const QrState& acc, double* lds, double* beta,
-                                                int* has) {
-    const QrState t = block_factor(acc, lds);
-    double* res = lds + 2 * kLsmXchg; // not touched by block_sums
-    if (threadIdx.x == 0) {
-        const int h = t.rows >= (uint32_t)kLsmBasis ? 1 : 0;
-        double b[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = kLsmBasis - 1; i >= 0; --i) {
-            double sum = t.z[i];
-            for (int j = i + 1; j < kLsmBasis; ++j) sum -= t.r[ridx(i, j)] * b[j];
-            const double diag = t.r[ridx(i, i)];
-            b[i] = diag != 0.0 ? sum / diag : 0.0;
+            for (int k = 0; k < i; ++k) t = __builtin_fma(-u[ridx(k, i)], u[ridx(k, j)], t);
+            u[ridx(i, j)] = t * iu[i];
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) res[i] = b[i];
-        res[4] = (double)h;
     }
-    __syncthreads();
+    double t[4], c[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) beta[i] = res[i];
-    *has = res[4] != 0.0 ? 1 : 0;
+    for (int i = 0; i < kLsmBasis; ++i) { // U^T t = q
+        double s = g[10 + i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s = __builtin_fma(-u[ridx(k, i)], t[k], s);
+        t[i] = s * iu[i];
+    }
+#pragma unroll
+    for (int i = kLsmBasis - 1; i >= 0; --i) { // U c' = t
+        double s = t[i];
+#pragma unroll
+        for (int j = i + 1; j < kLsmBasis; ++j) s = __builtin_fma(-u[ridx(i, j)], c[j], s);
+        c[i] = s * iu[i];
+    }
+#pragma unroll
+    for (int i = kLsmBasis - 1; i >= 0; --i) { // P beta = c'
+        double s = c[i];
+#pragma unroll
+        for (int j = i + 1; j < kLsmBasis; ++j) s = __builtin_fma(-f.r[ridx(i, j)], beta[j], s);
+        beta[i] = s * f.inv_d[i];
+    }
 }
 
 // One date of the backward induction. With fit, the later date's regression is solved here:
-// every block reduces the later date's block factors (written by the previous launch, in the
-// other half of the double-buffered partials) in the same fixed order, so every block holds
+// wave 0 of every block adds the later date's block sums (written by the previous launch, in
+// the other half of the double-buffered sums) in the same fixed order, so every block holds
 // the same coefficients, and block 0 records them; no separate solve launch per date. Without
-// fit, the policy in coeffs/has is applied.
-template <bool FP64>
-__global__ __launch_bounds__(kLsmBlock) void lsm_date_kernel(LsmArgs a, int date, int regress,
+// fit, the policy in coeffs/has is applied. CAP: the regression captures the network's
+// training samples (a.cap_*).
+template <bool FP64, bool CAP>
+__global__ __launch_bounds__(kLsmLanes) void lsm_date_kernel(LsmArgs a, int date, int regress,
                                                              int fit) {
     select_run(a);
     __shared__ double lds[kLsmLdsDoubles];
     const int later = date + 1;
-    // Chunks of kLsmChunk paths per lane, every load of a chunk issued before its first use;
-    // the first chunk's loads go out before the later date's regression is solved, so their
-    // latency overlaps it (issuing each next chunk's loads a chunk early measured slower).
-    // Loads are unconditional at in-bounds indices (path 0 past the end, store column 1 for
-    // the final date 0), so no load waits inside a branch; unused values are ignored.
+    const bool solve = fit && later <= a.m - 1; // uniform
+    // Every load is issued up front, those the solve needs first: wave 0's block sums (lane l:
+    // blocks l, l + 64, l + 128, l + 192; the buffer is zero past the blocks), the
+    // preconditioner and discount powers, then the lane's paths (path 0 past the end, store
+    // column 1 for the final date 0), so no load waits inside a branch and unused values are
+    // ignored.
+    double gv[kNe][4];
+    if (solve && threadIdx.x < 64) {
+#pragma unroll
+        for (int k = 0; k < kNe; ++k) {
+            const double* src = a.ne + ((size_t)(later & 1) * kNe + k) * kLsmNeStride;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) gv[k][q] = src[threadIdx.x + 64 * q];
+        }
+    }
+    const DateFactor f = uniform_factor(a.factors[regress ? date : 1]);
+    const double d_later = a.dpow[later <= a.m ? later : a.m];
+    const double d_date = a.dpow[date];
     const int col = date >= 1 ? date : 1;
-    auto load_chunk = [&](uint32_t j0, double (&cf)[kLsmChunk], double (&s1)[kLsmChunk],
-                          double (&s0)[kLsmChunk]) {
+    double v[kLsmChunk], s1[kLsmChunk], s0[kLsmChunk];
+    auto load_chunk = [&](uint32_t j0) {
 #pragma unroll
         for (uint32_t u = 0; u < kLsmChunk; ++u) {
             const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
             const uint64_t q = (j0 + u < a.per_lane && p < a.n_paths) ? p : 0;
-            cf[u] = a.cf[q];
+            v[u] = a.val[q];
             s1[u] = store_get<FP64>(a, later, q);
             s0[u] = store_get<FP64>(a, col, q);
         }
     };
-    const bool solve = fit && later <= a.m - 1; // uniform
-    QrState acc = {};
-    if (solve && threadIdx.x < a.blocks) acc = a.qr_partials[(size_t)(later & 1) * a.blocks + threadIdx.x];
-    double cf[kLsmChunk], s1[kLsmChunk], s0[kLsmChunk];
-    load_chunk(0, cf, s1, s0);
+    load_chunk(0);
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
     bool use = false;
     if (later <= a.m - 1) { // uniform
-        if (fit) {
-            int has = 0;
-            merge_solve_all(acc, lds, beta, &has);
-            use = has != 0;
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                a.has[later] = (uint8_t)has;
-                if (has)
-                    for (int j = 0; j < 4; ++j) a.coeffs[4 * later + j] = beta[j];
+        if (solve) {
+            // Wave 0 adds the blocks' sums as (b0 + b1) + (b2 + b3) per lane, then over its
+            // lanes, solves, and hands the coefficients to the other waves through LDS; one
+            // block barrier.
+            double* res = lds + kLsmLdsRes; // totals [kNe], then beta [4] and the flag
+            if (threadIdx.x < 64) {
+                double g[kNe];
+#pragma unroll
+                for (int k = 0; k < kNe; ++k) g[k] = (gv[k][0] + gv[k][1]) + (gv[k][2] + gv[k][3]);
+                const double t = wave_sums<kNe>(g, lds);
+                if ((threadIdx.x & 3) == 0 && (threadIdx.x >> 2) < (unsigned)kNe) res[threadIdx.x >> 2] = t;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                double tot[kNe];
+#pragma unroll
+                for (int k = 0; k < kNe; ++k) tot[k] = res[k];
+                const bool u = tot[14] >= (double)kLsmBasis; // lsm.cpp:102-104: >= 4 ITM rows
+                if (u) {
+                    // The sums regress the values discounted to t = 0; the later date's target
+                    // is those over disc^later.
+                    ne_solve(tot, a.factors[later], beta);
+                    const double inv = 1.0 / d_later;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) beta[j] *= inv;
+                }
+                if (threadIdx.x == 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) res[kNe + j] = beta[j];
+                    res[kNe + 4] = u ? 1.0 : 0.0;
+                    if (blockIdx.x == 0) {
+                        a.has[later] = u ? 1 : 0;
+                        if (u)
+                            for (int j = 0; j < 4; ++j) a.coeffs[4 * later + j] = beta[j];
+                    }
+                }
             }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) beta[j] = res[kNe + j];
+            use = res[kNe + 4] != 0.0;
         } else {
             use = a.has[later] != 0;
             if (use)
                 for (int j = 0; j < 4; ++j) beta[j] = a.coeffs[4 * later + j];
         }
     }
-    QrState st = {};
-    // Rows are folded in ascending path order, as one path at a time would.
+    const ContPoly cp = cont_poly(beta);
+    double acc[kNe];
+#pragma unroll
+    for (int k = 0; k < kNe; ++k) acc[k] = 0.0;
+    // Rows are summed in ascending path order, as one path at a time would.
     for (uint32_t j0 = 0; j0 < a.per_lane; j0 += kLsmChunk) {
-        if (j0 > 0) load_chunk(j0, cf, s1, s0);
+        if (j0 > 0) load_chunk(j0);
 #pragma unroll
         for (uint32_t u = 0; u < kLsmChunk; ++u) {
             const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
             if (j0 + u >= a.per_lane || p >= a.n_paths) break;
-            double c = cf[u];
-            if (use) c = decide(a, s1[u], c, beta);
-            c *= a.disc; // lsm.cpp:84-86 (and :125-127 after the last date)
-            a.cf[p] = c;
-            cf[u] = c;
+            if (use && decide_fast(a, s1[u], beta, cp)) { // exercise at the later date
+                v[u] = intrinsic_v(s1[u], a.strike, a.type) * d_later;
+                a.val[p] = v[u];
+            }
         }
         if (regress) { // uniform across the grid
-            // The chunk's rows (out-of-the-money and past-the-end paths as zero rows) folded
-            // into the lane's factor in one blocked Householder step.
-            double rows[kLsmChunk][4], ys[kLsmChunk];
+            // Out-of-the-money and past-the-end paths are zero rows and add nothing.
 #pragma unroll
             for (uint32_t u = 0; u < kLsmChunk; ++u) {
                 const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
                 const bool in = j0 + u < a.per_lane && p < a.n_paths;
-                const bool itm = fold_row(a, date, p, in, s0[u], cf[u], rows[u]);
-                ys[u] = itm ? cf[u] : 0.0;
-                st.rows += itm ? 1u : 0u;
+                double row[4], w[4];
+                const bool itm = design_row(a, in, s0[u], row);
+                if constexpr (CAP) {
+                    if (in) capture_row(a, date, p, itm, s0[u], v[u] / d_date);
+                }
+                const double y = itm ? v[u] : 0.0; // discounted to t = 0
+                precondition_row(f, row, w);
+#pragma unroll
+                for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+                    for (int j = i; j < kLsmBasis; ++j)
+                        acc[ridx(i, j)] = __builtin_fma(w[i], w[j], acc[ridx(i, j)]);
+                    acc[10 + i] = __builtin_fma(w[i], y, acc[10 + i]);
+                }
+                acc[14] += itm ? 1.0 : 0.0;
             }
-            householder_fold_rows<kLsmChunk>(st, rows, ys);
         }
     }
     if (!regress) return; // uniform across the grid
-    const QrState t = block_factor(st, lds);
-    if (threadIdx.x == 0) a.qr_partials[(size_t)(date & 1) * a.blocks + blockIdx.x] = t;
+    const double t = block_sums<kNe>(acc, lds);
+    if (threadIdx.x < (unsigned)kNe)
+        a.ne[((size_t)(date & 1) * kNe + threadIdx.x) * kLsmMaxBlocks + blockIdx.x] = t;
 }
 
 // Welford over the final cash flows in the canonical tile order.
+template <bool FP64>
 __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
     select_run(a);
     __shared__ Welford lds4[4];
@@ -545,7 +780,7 @@ __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
     for (int k = 0; k < kPathsPerLane; ++k) {
         const uint64_t p = tile * kTilePaths + (uint64_t)(k * kTileLanes) + threadIdx.x;
         if (p < a.n_paths) {
-            const double x = a.cf[p];
+            const double x = a.val[p]; // discounted to t = 0 (lsm.cpp:125-127)
             w.n += 1;
             const double d = x - w.mean;
             w.mean += d * (1.0 / (double)(k + 1));
@@ -722,9 +957,7 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
             }
         }
     }
-    double cn = ps.n_val, cq = ps.q_val;
-    for (int i = 0; i < ps.n_date; ++i) cn *= a.disc;
-    for (int i = 0; i < ps.q_date; ++i) cq *= a.disc;
+    const double cn = ps.n_val * disc_pow(a.disc, ps.n_date), cq = ps.q_val * disc_pow(a.disc, ps.q_date);
     sh_n[t] = cn;
     sh_q[t] = cq;
     uint32_t c_dec = ps.dec, c_dis = ps.dis, c_fb = ps.fb;
@@ -835,12 +1068,9 @@ __device__ __forceinline__ void frozen_visit(const LsmArgs& a, const uint8_t* ha
     }
 }
 
-// Cash flow of a stopped scenario discounted one multiplication per date, in backward
-// induction's order.
+// Cash flow of a stopped scenario discounted to t = 0 as backward induction discounts it.
 __device__ __forceinline__ double discounted(const LsmArgs& a, const Stop& st) {
-    double c = st.value;
-    for (int i = 0; i < st.date; ++i) c *= a.disc;
-    return c;
+    return st.value * disc_pow(a.disc, st.date);
 }
 
 // One path per thread; a block of 1024 threads is one canonical tile, partials rebuilt in
@@ -953,9 +1183,10 @@ struct LsmRun {
     uint32_t runs; // independent runs in the batch (grid y)
 };
 
-// Lays out the scratch: [store][cf][coeffs][has][qr partials][tile partials] followed by
-// `extra_bytes` for the caller (returned in *extra when non-null). `with_store` false leaves
-// out the path store and the cash-flow vector (streaming pricing). `runs` > 1 lays out that
+// Lays out the scratch: [store][values][discount powers][Gram sums][date preconditioners]
+// [normal-equation sums][tile partials][coeffs][has] followed by `extra_bytes` for the caller
+// (returned in *extra when non-null). `with_store` false leaves out the path store, the values
+// and the regression's buffers (streaming pricing). `runs` > 1 lays out that
 // many independent runs, each region holding the runs one after another (the tile partials:
 // runs x tiles, then one merged result per run).
 int lsm_setup(double spot, double strike, double rate, double carry, double vol, double time,
@@ -972,6 +1203,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.inv_strike = 1.0 / strike;
     a.dt = dt;
     a.disc = exp(-rate * dt);
+    a.disc_m = disc_pow(a.disc, m);
     a.drift = (rate - carry - 0.5 * vol * vol) * dt;
     a.vsd = vol * sqrt(dt);
     const double log2e = 1.44269504088896340735992468100189214;
@@ -981,17 +1213,20 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.n_paths = n;
     a.m = m;
     a.type = type;
-    const uint64_t blocks = (n + kLsmBlock - 1) / kLsmBlock;
+    const uint64_t blocks = (n + kLsmPathLanes - 1) / kLsmPathLanes;
     const uint64_t tiles = tiles_for(n);
     uint64_t bb = (n + 4095) / 4096;
     if (bb > kLsmMaxBlocks) bb = kLsmMaxBlocks;
     a.blocks = (uint32_t)bb;
-    a.per_lane = (uint32_t)((n + bb * kLsmBlock - 1) / (bb * kLsmBlock));
+    a.per_lane = (uint32_t)((n + bb * kLsmLanes - 1) / (bb * kLsmLanes));
+    a.sample_n = n < kLsmSample ? n : kLsmSample;
     if (runs < 1 || (runs > 1 && !with_store)) return MCDX_EINVAL;
     const size_t store_bytes = with_store ? (size_t)m * n * (fp64 ? 8 : 4) : 0;
     a.run_store_bytes = runs > 1 ? store_bytes : 0;
     a.run_cf = runs > 1 ? n : 0;
-    a.run_qr = runs > 1 ? 2 * (uint64_t)a.blocks : 0;
+    const uint64_t gram_count = (uint64_t)(m > 1 ? m - 1 : 1) * kGram * kLsmGramBlocks;
+    a.run_gram = runs > 1 ? gram_count : 0;
+    a.run_ne = runs > 1 ? 2 * (uint64_t)kNe * kLsmMaxBlocks : 0;
     a.run_tiles = runs > 1 ? tiles : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -1001,8 +1236,11 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     };
     // The tile partials (whose tail holds the merged results), the coefficients and the flags
     // are adjacent, so one copy brings a single run's results and policy back.
-    const size_t o_store = take(store_bytes * runs), o_cf = take(with_store ? n * 8 * runs : 0),
-                 o_qr = take(with_store ? 2 * (size_t)a.blocks * sizeof(QrState) * runs : 0),
+    const size_t o_store = take(store_bytes * runs), o_val = take(with_store ? n * 8 * runs : 0),
+                 o_dpow = take(((size_t)m + 1) * 8),
+                 o_gram = take(with_store ? gram_count * 8 * runs : 0),
+                 o_dfac = take(with_store ? ((size_t)m + 1) * sizeof(DateFactor) * runs : 0),
+                 o_ne = take(with_store ? 2 * (size_t)kNe * kLsmMaxBlocks * 8 * runs : 0),
                  o_part = take((tiles + 1) * runs * sizeof(Welford)),
                  o_coeffs = take((size_t)(m + 1) * 4 * 8 * runs),
                  o_has = take(((size_t)m + 1) * runs), o_extra = take(extra_bytes);
@@ -1011,10 +1249,13 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     char* base = (char*)ctx->dev;
     if (extra) *extra = base + o_extra;
     a.store = base + o_store;
-    a.cf = (double*)(base + o_cf);
+    a.val = (double*)(base + o_val);
+    a.dpow = (double*)(base + o_dpow);
     a.coeffs = (double*)(base + o_coeffs);
     a.has = (uint8_t*)(base + o_has);
-    a.qr_partials = (QrState*)(base + o_qr);
+    a.gram = (double*)(base + o_gram);
+    a.factors = (DateFactor*)(base + o_dfac);
+    a.ne = (double*)(base + o_ne);
     a.partials = (Welford*)(base + o_part);
     run->a = a;
     run->blocks = blocks;
@@ -1053,13 +1294,29 @@ struct PathSource {
 template <bool FP64>
 hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
     const dim3 g(r.a.blocks, r.runs);
-    for (int date = r.a.m - 1; date >= 1; --date)
-        hipLaunchKernelGGL(lsm_date_kernel<FP64>, g, dim3(kLsmBlock), 0, s, r.a, date,
-                           fit ? 1 : 0, fit ? 1 : 0);
+    if (fit && r.a.m > 1) { // every date's preconditioner, before the dates' kernels
+        // gb <= 32 blocks per date, at least 16 sample paths per lane
+        uint64_t b = (r.a.sample_n + 16 * kLsmLanes - 1) / (16 * kLsmLanes);
+        if (b > kLsmGramBlocks) b = kLsmGramBlocks;
+        const uint32_t gb = (uint32_t)b;
+        const uint32_t per = (uint32_t)((r.a.sample_n + b * kLsmLanes - 1) / (b * kLsmLanes));
+        hipLaunchKernelGGL(lsm_gram_kernel<FP64>, dim3(gb, r.a.m - 1, r.runs), dim3(kLsmLanes), 0,
+                           s, r.a, gb, per);
+        hipLaunchKernelGGL(lsm_gram_merge_kernel<FP64>, dim3(r.a.m - 1, r.runs), dim3(kLsmLanes), 0,
+                           s, r.a, gb);
+    }
+    const bool cap = fit && r.a.cap_y;
+    for (int date = r.a.m - 1; date >= 1; --date) {
+        if (cap)
+            hipLaunchKernelGGL((lsm_date_kernel<FP64, true>), g, dim3(kLsmLanes), 0, s, r.a, date, 1, 1);
+        else
+            hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a,
+                               date, fit ? 1 : 0, fit ? 1 : 0);
+    }
     // date 0: the decisions at date 1 (fitting date 1 first when fitting), no regression
-    hipLaunchKernelGGL(lsm_date_kernel<FP64>, g, dim3(kLsmBlock), 0, s, r.a, 0, 0,
+    hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a, 0, 0,
                        fit ? 1 : 0);
-    hipLaunchKernelGGL(lsm_welford_kernel, dim3((unsigned)r.tiles, r.runs), dim3(kTileLanes), 0,
+    hipLaunchKernelGGL(lsm_welford_kernel<FP64>, dim3((unsigned)r.tiles, r.runs), dim3(kTileLanes), 0,
                        s, r.a);
     hipError_t e = launch_check();
     if (e != hipSuccess) return e;

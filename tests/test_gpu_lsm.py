@@ -1,8 +1,9 @@
 """GPU Longstaff-Schwartz against the oracle (mcdx C ABI).
 
-The device regression is a tall-skinny QR in FP64 (blocked Householder steps per lane, then
-Householder reflections of the stacked lane and block factors with fixed-order block-wide
-sums): the same least-squares solution as the reference's sequential Householder sweep
+The device regression solves each date's least squares in FP64 by normal equations of rows
+preconditioned with a Cholesky-QR factor of the date's in-the-money rows (formed from the store
+before the backward induction, two Cholesky steps), so the preconditioned problem is well
+conditioned: the same least-squares solution as the reference's sequential Householder sweep
 (src/core/linalg.cpp:7-71) with different rounding, so coefficients are compared at 1e-7
 relative and prices at 0.02 SE (a decision within rounding of the exercise boundary may flip). FP32-generation prices are compared to
 FP64 at 0.5 SE; acceptance against CRR follows the reference's tests/lsm_test.cpp:26-123 at its
@@ -35,14 +36,25 @@ def test_fp64_matches_reference_fixture():
     assert abs(p - float.fromhex(g["price"])) <= 0.02 * float.fromhex(g["se"])
 
 
+# The last two are deep out of the money with more paths than the 2^16-path preconditioning
+# sample: at the middle dates the sample holds a few hundred in-the-money rows, so those dates
+# are preconditioned from every path, and the first dates have fewer than four rows. Their
+# earliest regressed date is an interpolation: for the put, 4 rows with condition number 9.8e9
+# and coefficients near 1e10, where any FP64 least-squares solution is only good to about
+# condition x 1e-16 ~ 1e-6 (solved in 60-digit arithmetic: the oracle's sequential Householder
+# is 3.1e-7 from the exact coefficients, the device 4.5e-7). Coefficients beyond 1e6 in
+# magnitude, which only such near-interpolations produce, are compared at 1e-6.
 @pytest.mark.parametrize("typ,s0,vol,m,n", [(1, 100.0, 0.2, 50, 40_000), (1, 90.0, 0.3, 12, 30_011),
-                                           (0, 110.0, 0.25, 20, 20_000)])
+                                           (0, 110.0, 0.25, 20, 20_000),
+                                           (1, 140.0, 0.2, 20, 100_000), (0, 70.0, 0.25, 20, 100_000)])
 def test_fp64_matches_oracle(typ, s0, vol, m, n):
     p, se, co, has, _ = mcdx.lsm(s0, 100.0, 0.06, 0.01, vol, 1.0, typ, m, n, 42, FP64)
     op, ose, oco, ohas, _ = O.lsm(s0, 100.0, 0.06, 0.01, vol, 1.0, typ, m, n, 42)
     assert has.tolist() == ohas.tolist()
     mask = has.astype(bool)
-    assert np.allclose(co[mask], oco[mask], rtol=1e-7, atol=1e-9)
+    big = np.abs(oco[mask]).max(axis=1) > 1e6
+    assert np.allclose(co[mask][~big], oco[mask][~big], rtol=1e-7, atol=1e-9)
+    assert np.allclose(co[mask][big], oco[mask][big], rtol=1e-6, atol=1e-9)
     assert abs(p - op) <= 0.02 * ose
 
 
