@@ -12,22 +12,23 @@
 //
 // Layout: the store is SoA [date][path] (dates 1..M; the t=0 column is the constant S0),
 // FP32 in FP32 mode (2^20 paths x 50 dates = 210 MB), FP64 in FP64 mode. Each path's cash
-// flow is kept discounted to t = 0, intrinsic(S at its exercise date e) x disc^e with the powers
-// of the one-date discount factor formed by repeated multiplication (the reference's
+// flow is kept discounted to t = 0, intrinsic(S at its exercise date e) x disc^e with the
+// powers of the one-date discount factor formed by repeated multiplication (the reference's
 // cf *= disc per date, lsm.cpp:84-86, up to its rounding), so a date's kernel writes a path's
 // value only when it exercises; a date's regression target is that value over disc^date, a
-// factor common to the date's rows that scales the coefficients once. A date's regression rows [basis(S/K)] are those of the paths in the money
-// at that date, fixed by the store alone; only the target (the discounted cash flows) depends
-// on the later dates' decisions. So the R factor of every date's rows is formed before the
-// backward induction, for all dates at once (blocked Householder steps per lane, then
-// Householder reflections of the stacked lane and block factors: tall-skinny QR in FP64).
-// Each date of the backward induction then runs one kernel: solve the later date's
-// regression, apply its exercise decision, discount, and accumulate the normal equations of
-// the preconditioned rows w = R^-T basis (G' = sum w w^T, which is the identity up to
-// rounding, and q = sum w cf), reduced per block. The solve is c = R^-1 G'^-1 q with G' by
-// Cholesky: the second step of Cholesky QR on rows already orthogonalised by R, as accurate as
-// the reference's Householder sweep for its condition numbers (about 2e6 at the first dates);
-// only the rounding differs from the reference's sequential Householder sweep.
+// factor common to the date's rows that scales the coefficients once.
+//
+// Regression: a date's rows [basis(S/K)] are those of the paths in the money at that date,
+// fixed by the store alone; only the target depends on the later dates' decisions. So every
+// date's preconditioner P (upper triangular, P^T P close to the rows' Gram matrix) is formed
+// before the backward induction, for all dates at once, by a Cholesky-QR step over a path
+// sample. Each date of the backward induction then runs one kernel: solve the later
+// date's regression, apply its exercise decision, and accumulate the normal equations of the
+// preconditioned rows w = P^-T basis (G' = sum w w^T, close to the identity, and
+// q = sum w cf), reduced per block. The coefficients are P^-1 G'^-1 q with G' by Cholesky: the
+// least-squares solution of the reference's Householder sweep (linalg.cpp:7-71) to within
+// rounding of order the rows' condition number (about 2e6 at the first dates of a 50-date
+// put) times 1e-16, since the preconditioned problem is well conditioned.
 
 #include <math.h>
 
@@ -292,18 +293,20 @@ __device__ __forceinline__ ContPoly cont_poly(const double* beta) {
 // the intrinsic value is farther from the estimate than 4e-6 (1 + x) |estimate| + 1e-12
 // bound, the estimate's side of the boundary is decide()'s; the remaining paths (within a few
 // 1e-6 of the boundary, or with x outside [0, 64]) are decided by decide() itself.
-__device__ __forceinline__ bool decide_fast(const LsmArgs& a, double s1, const double* beta,
-                                            const ContPoly& cp) {
+// Without branches: whether the path exercises by the estimate (*out) and whether the
+// estimate cannot decide it (returned: the caller then calls decide()).
+__device__ __forceinline__ bool decide_screen(const LsmArgs& a, double s1, const ContPoly& cp,
+                                              bool* out) {
     const double ex = intrinsic_v(s1, a.strike, a.type);
-    if (!(ex > 0.0)) return false;
     const double x = s1 * a.inv_strike;
     const double e = (double)fast_exp2((float)x * -0.721347520444481703680f); // e^{-x/2}
     const double est = e * __builtin_fma(__builtin_fma(__builtin_fma(cp.c[3], x, cp.c[2]), x, cp.c[1]), x, cp.c[0]);
     const double x1 = 1.0 + x;
     const double bound = e * __builtin_fma(__builtin_fma(__builtin_fma(cp.b[3], x1, cp.b[2]), x1, cp.b[1]), x1, cp.b[0]);
     const bool sure = x >= 0.0 && x <= 64.0 && fabs(ex - est) > 4e-6 * x1 * fabs(est) + 1e-12 * bound;
-    if (sure) return ex > est;
-    return decide(a, s1, beta);
+    const bool itm = ex > 0.0;
+    *out = itm && ex > est;
+    return itm && !sure;
 }
 
 // A row for the preconditioner only: the basis with its weight e^{-x/2} from the hardware FP32
@@ -424,11 +427,10 @@ __device__ __forceinline__ void precondition_row(const DateFactor& f, const doub
 
 // ---- Preconditioners of every date (before the backward induction) ----------------------
 // Gram sums of the rows of one date over paths p < lim, as (j gb + b) kLsmLanes + t for
-// j < per (block b of gb, lane t), added to g; with PRE, of the rows preconditioned by pf.
-template <bool FP64, bool PRE>
+// j < per (block b of gb, lane t), added to g.
+template <bool FP64>
 __device__ __forceinline__ void gram_rows(const LsmArgs& a, int date, uint32_t gb, uint32_t b,
-                                          uint32_t per, uint64_t lim, double (&g)[kGram],
-                                          const DateFactor& pf) {
+                                          uint32_t per, uint64_t lim, double (&g)[kGram]) {
     for (uint32_t j0 = 0; j0 < per; j0 += kLsmChunk) {
         double s0[kLsmChunk];
 #pragma unroll
@@ -441,12 +443,6 @@ __device__ __forceinline__ void gram_rows(const LsmArgs& a, int date, uint32_t g
             const uint64_t p = ((uint64_t)(j0 + u) * gb + b) * kLsmLanes + threadIdx.x;
             double row[4];
             const bool itm = design_row_approx(a, j0 + u < per && p < lim, s0[u], row);
-            if constexpr (PRE) {
-                double w[4];
-                precondition_row(pf, row, w);
-#pragma unroll
-                for (int i = 0; i < kLsmBasis; ++i) row[i] = w[i];
-            }
 #pragma unroll
             for (int i = 0; i < kLsmBasis; ++i) {
 #pragma unroll
@@ -468,7 +464,7 @@ __global__ __launch_bounds__(kLsmLanes) void lsm_gram_kernel(LsmArgs a, uint32_t
     double g[kGram];
 #pragma unroll
     for (int k = 0; k < kGram; ++k) g[k] = 0.0;
-    gram_rows<FP64, false>(a, date, gb, blockIdx.x, per, a.sample_n, g, DateFactor{});
+    gram_rows<FP64>(a, date, gb, blockIdx.x, per, a.sample_n, g);
     const double t = block_sums<kGram>(g, lds);
     if (threadIdx.x < (unsigned)kGram)
         a.gram[((size_t)(date - 1) * kGram + threadIdx.x) * kLsmGramBlocks + blockIdx.x] = t;
@@ -501,12 +497,12 @@ __device__ __forceinline__ DateFactor shifted_cholesky(const double (&g)[kGram])
 }
 
 // Grid (M - 1, runs): one block per date adds its gb blocks' Gram sums (lane b holds block
-// b's) and factors them, G + shift I = P1^T P1. The sample's rows (every path's, for a date
-// whose sample held fewer than kLsmMinSample rows while the sample is not every path: one
-// block, slow, but only for such thin dates) are then preconditioned by P1 and their Gram
-// matrix G2 = U^T U factored again: P = U P1 (a second Cholesky-QR step, which leaves the
-// preconditioned rows orthonormal to within rounding even where the first step's Gram matrix
-// was too ill-conditioned to factor accurately).
+// b's) and factors them, G + shift I = P^T P. A date whose sample held fewer than
+// kLsmMinSample rows (while the sample is not every path) is summed again over every path
+// (one block: slow, but only for such thin dates). One Cholesky step suffices: for rows of
+// condition number k, the preconditioned rows' condition number is about 1 + k^2 1e-16 (1.0004
+// at k = 2e6), and the date kernel's normal equations in them lose only its square times
+// 1e-16; beyond k ~ 1e8 the shift keeps P defined.
 template <bool FP64>
 __global__ __launch_bounds__(kLsmLanes) void lsm_gram_merge_kernel(LsmArgs a, uint32_t gb) {
     select_run(a);
@@ -519,50 +515,21 @@ __global__ __launch_bounds__(kLsmLanes) void lsm_gram_merge_kernel(LsmArgs a, ui
                                 : 0.0;
     double* res = lds + kLsmLdsRes;
     double t = block_sums<kGram>(g, lds);
-    if (threadIdx.x == 10) res[0] = t;
+    if (threadIdx.x == 10) res[kNe + 4] = t; // the row count, in a slot nothing else writes
     __syncthreads();
-    const bool thin = res[0] < kLsmMinSample && a.sample_n < a.n_paths; // uniform
-    // the rows of the second step: the sample's, as the Gram kernel's gb blocks cover them, or
-    // every path's
-    const uint64_t lim = thin ? a.n_paths : a.sample_n;
-    const uint32_t per = (uint32_t)((lim + kLsmLanes - 1) / kLsmLanes);
+    const bool thin = res[kNe + 4] < kLsmMinSample && a.sample_n < a.n_paths; // uniform
     if (thin) {
-        __syncthreads(); // every lane has read res[0] before the next exchange
 #pragma unroll
         for (int k = 0; k < kGram; ++k) g[k] = 0.0;
-        gram_rows<FP64, false>(a, date, 1, 0, per, lim, g, DateFactor{});
+        gram_rows<FP64>(a, date, 1, 0, (uint32_t)((a.n_paths + kLsmLanes - 1) / kLsmLanes), a.n_paths, g);
         t = block_sums<kGram>(g, lds);
     }
-    if (threadIdx.x < (unsigned)kGram) res[threadIdx.x] = t;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kGram; ++k) g[k] = res[k];
-    const DateFactor p1 = shifted_cholesky(g); // every lane, the same
-    __syncthreads(); // every lane has read res before the next exchange
-#pragma unroll
-    for (int k = 0; k < kGram; ++k) g[k] = 0.0;
-    gram_rows<FP64, true>(a, date, 1, 0, per, lim, g, p1);
-    t = block_sums<kGram>(g, lds);
     if (threadIdx.x < (unsigned)kGram) res[threadIdx.x] = t;
     __syncthreads();
     if (threadIdx.x != 0) return;
 #pragma unroll
     for (int k = 0; k < kGram; ++k) g[k] = res[k];
-    const DateFactor u = shifted_cholesky(g);
-    DateFactor f; // P = U P1, upper triangular
-#pragma unroll
-    for (int i = 0; i < kLsmBasis; ++i) {
-#pragma unroll
-        for (int j = i; j < kLsmBasis; ++j) {
-            double sum = 0.0;
-#pragma unroll
-            for (int k = i; k <= j; ++k) sum += u.r[ridx(i, k)] * p1.r[ridx(k, j)];
-            f.r[ridx(i, j)] = sum;
-        }
-        const double d = f.r[ridx(i, i)];
-        f.inv_d[i] = d != 0.0 ? 1.0 / d : 0.0;
-    }
-    a.factors[date] = f;
+    a.factors[date] = shifted_cholesky(g);
 }
 
 // ---- One date of the backward induction --------------------------------------------------
@@ -730,13 +697,24 @@ __global__ __launch_bounds__(kLsmLanes) void lsm_date_kernel(LsmArgs a, int date
     // Rows are summed in ascending path order, as one path at a time would.
     for (uint32_t j0 = 0; j0 < a.per_lane; j0 += kLsmChunk) {
         if (j0 > 0) load_chunk(j0);
+        if (use) { // uniform: the decisions at the later date, screened without branches
+            bool ex[kLsmChunk], unsure = false;
 #pragma unroll
-        for (uint32_t u = 0; u < kLsmChunk; ++u) {
-            const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
-            if (j0 + u >= a.per_lane || p >= a.n_paths) break;
-            if (use && decide_fast(a, s1[u], beta, cp)) { // exercise at the later date
-                v[u] = intrinsic_v(s1[u], a.strike, a.type) * d_later;
-                a.val[p] = v[u];
+            for (uint32_t u = 0; u < kLsmChunk; ++u) unsure |= decide_screen(a, s1[u], cp, &ex[u]);
+            if (__builtin_amdgcn_ballot_w64(unsure) != 0) { // rare: some path near the boundary
+#pragma unroll
+                for (uint32_t u = 0; u < kLsmChunk; ++u) {
+                    bool o;
+                    if (decide_screen(a, s1[u], cp, &o)) ex[u] = decide(a, s1[u], beta);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kLsmChunk; ++u) {
+                const uint64_t p = lane_path(j0 + u, blockIdx.x, a.blocks, threadIdx.x);
+                if (ex[u] && j0 + u < a.per_lane && p < a.n_paths) { // exercise at the later date
+                    v[u] = intrinsic_v(s1[u], a.strike, a.type) * d_later;
+                    a.val[p] = v[u];
+                }
             }
         }
         if (regress) { // uniform across the grid
