@@ -83,6 +83,26 @@ ISSUE_QUADS = {"asian": (58.961, 14.827), "barrier_uo": (56.643, 14.812),
 SIMDS = 1024               # MI355X: 256 CUs x 4 SIMDs
 PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 
+# The same lane-issue roofline for the secondary path kernels, from one launch of each at its
+# configuration (rocprofv3 --pmc SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VALU2 ... -- python3
+# bench.py --only kernels): (VALU quad-cycles, paired quad-cycles) per wave-unit, i.e. the
+# counters over (units / 64): C1 in FP64 per wave-path (2^24 paths: the reference's FP64
+# inverse normal with its erfc, exp and IEEE divisions, 531 VALU instructions per path), the
+# bridge-corrected barrier per wave-path-step, Sobol + bridge QMC (Asian) per wave-point-date
+# and Heston QE (9 strikes) per wave-path-step.
+ISSUE_QUADS_CONFIG = {"c1_fp64": (564.74, 58.45), "c2_barrier_uo_bb": (91.99, 18.85),
+                      "c2_qmc_sobol_bb_asian": (48.56, 13.88), "c3_heston_qe": (175.65, 34.57)}
+
+
+def config_issue(key, units, ms, timing):
+    """Lane-issue roofline of one secondary configuration: the issue slots its kernel needs
+    for `units` (counter-derived, ISSUE_QUADS_CONFIG) over the slots the SIMDs offer at the peak
+    clock in the measured time `ms` (which may include more than the kernel, `timing`)."""
+    quads, paired = ISSUE_QUADS_CONFIG[key]
+    need_ms = (quads - paired) * 4 * (units / 64) / SIMDS / PEAK_CLOCK_HZ * 1e3
+    return {"bound": "valu issue", "issue_limited_ms": need_ms, "ms": ms, "frac": need_ms / ms,
+            "quad_cycles_per_wave_unit": quads, "paired_per_wave_unit": paired, "timing": timing}
+
 
 def contracts(mcdx, n):
     common = dict(spot=100.0, strike=100.0, rate=0.05, carry_yield=0.0, vol=0.2, time=1.0,
@@ -261,7 +281,9 @@ def extras(torch, mcdx, L, stream, handle, ref):
                  "roofline": {"bound": "valu", "ops_per_path": OPS_C1_FP64_PER_PATH,
                               "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s (FP64)",
                               "frac": tf / PEAK_FP64_TFLOPS,
-                              "note": "pricing call (tile kernel + merge) timed by HIP events"},
+                              "note": "pricing call (tile kernel + merge) timed by HIP events",
+                              "issue": config_issue("c1_fp64", n, ms64,
+                                                    "pricing call (tile kernel + merge)")},
                  "fp32_generation": {"paths_per_s": n / (ms32 / 1e3), "ms": ms32, "price": p,
                                      "se": se, "dev_in_se": (p - bsm) / se}}
         if ref and n == 1_000_000:
@@ -280,7 +302,9 @@ def extras(torch, mcdx, L, stream, handle, ref):
     entry = {"paths_per_s": N_PATHS / (ms / 1e3), "ms": ms, "price_fp32": p, "se": se,
              "roofline": {"bound": "valu", "ops_per_path_step": OPS_BB, "achieved": tf,
                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP32_TFLOPS,
-                          "timing": "pricing call (tile kernel + merge), HIP events"}}
+                          "timing": "pricing call (tile kernel + merge), HIP events",
+                          "issue": config_issue("c2_barrier_uo_bb", N_PATHS * STEPS, ms,
+                                                "pricing call (tile kernel + merge)")}}
     if ref:
         rp = fh(ref["c2_barrier_uo_call_bb"]["price"])
         entry.update(reference_fp64=rp, bias_vs_reference_in_se=(p - rp) / se)
@@ -323,7 +347,10 @@ def extras(torch, mcdx, L, stream, handle, ref):
                                                  "achieved": tq, "peak": PEAK_FP32_TFLOPS,
                                                  "unit": "TFLOP/s", "frac": tq / PEAK_FP32_TFLOPS,
                                                  "timing": "synchronous call, wall clock "
-                                                           "(kernel time in profiles/)"},
+                                                           "(kernel time in profiles/)",
+                                                 "issue": config_issue(
+                                                     "c2_qmc_sobol_bb_asian", (1 << 20) * STEPS,
+                                                     wall * 1e3, "synchronous call, wall clock")},
                                     "other_payoffs_2p20x252": qpaths,
                                     "price_fp32": q32, "price_fp64": q64,
                                     "randomized_16x2p16": {"price": rq, "se": rq_se,
@@ -350,7 +377,9 @@ def extras(torch, mcdx, L, stream, handle, ref):
         "paths_per_s_wall": (1 << 22) / wall, "path_steps_per_s_wall": (1 << 22) * STEPS / wall,
         "roofline": {"bound": "valu", "ops_per_path_step": OPS_HESTON, "achieved": th,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": th / PEAK_FP32_TFLOPS,
-                     "timing": "synchronous call, wall clock (kernel time in profiles/)"},
+                     "timing": "synchronous call, wall clock (kernel time in profiles/)",
+                     "issue": config_issue("c3_heston_qe", (1 << 22) * STEPS, wall * 1e3,
+                                           "synchronous call, wall clock")},
         "ms_wall": wall * 1e3,
         "prices": [x[0] for x in r], "se": [x[1] for x in r], "semi_analytic": sa,
         "dev_in_se": [(x[0] - a) / x[1] for x, a in zip(r, sa)],
