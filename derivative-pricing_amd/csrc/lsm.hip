@@ -22,16 +22,21 @@
 // fixed by the store alone; only the target depends on the later dates' decisions. So every
 // date's preconditioner P (upper triangular, P^T P close to the rows' Gram matrix) is formed
 // before the backward induction, for all dates at once, by a Cholesky-QR step over a path
-// sample. Each date of the backward induction then runs one kernel: solve the later
-// date's regression, apply its exercise decision, and accumulate the normal equations of the
-// preconditioned rows w = P^-T basis (G' = sum w w^T, close to the identity, and
-// q = sum w cf), reduced per block. The coefficients are P^-1 G'^-1 q with G' by Cholesky: the
+// sample. Each date of the backward induction then solves the later date's regression,
+// applies its exercise decision, and accumulates the normal equations of the preconditioned
+// rows w = P^-T basis (G' = sum w w^T, close to the identity, and q = sum w cf), reduced per
+// block: one launch runs every date when the run's blocks fit on the device at once (they
+// exchange each date's sums through device-scope memory), else one launch per date, with the
+// same bits. The coefficients are P^-1 G'^-1 q with G' by Cholesky: the
 // least-squares solution of the reference's Householder sweep (linalg.cpp:7-71) to within
 // rounding of order the rows' condition number (about 2e6 at the first dates of a 50-date
 // put) times 1e-16, since the preconditioned problem is well conditioned.
 
 #include <math.h>
 
+#include <atomic>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "mlp.hpp"
@@ -66,14 +71,17 @@ constexpr int kLsmNeStride = 256; // block sums per value in the ne buffer (at m
 // The reference's basis; x^3 / 6 is formed as x^3 * (1/6) (within an ulp of the division,
 // without its dependent FP64 division sequence). Every kernel that fits or replays a policy
 // evaluates this same function, so fits and replays agree bit for bit.
-__host__ __device__ __forceinline__ void laguerre4(double x, double* out) { // lsm.cpp:21-25
+// The basis at x given its weight e = e^{-x/2}.
+__host__ __device__ __forceinline__ void laguerre4_weighted(double x, double e, double* out) {
     constexpr double kSixth = 1.0 / 6.0;
-    const double e = exp(-x / 2.0);
     const double x2 = x * x;
     out[0] = e;
     out[1] = e * (1.0 - x);
     out[2] = e * (1.0 - 2.0 * x + 0.5 * x2);
     out[3] = e * (1.0 - 3.0 * x + 1.5 * x2 - x2 * x * kSixth);
+}
+__host__ __device__ __forceinline__ void laguerre4(double x, double* out) { // lsm.cpp:21-25
+    laguerre4_weighted(x, exp(-x / 2.0), out);
 }
 
 __host__ __device__ __forceinline__ double intrinsic_v(double s, double k, int type) {
@@ -115,7 +123,14 @@ struct LsmArgs {
     // exercise states (paths), Gram sums, normal-equation sums and tile partials; coeffs, has
     // and factors advance by (M+1) x 4, M+1 and M+1. Zero for a single run.
     uint64_t run_store_bytes, run_cf, run_gram, run_ne, run_tiles;
+    // The one-launch backward induction's date counters of each run: block b adds to counter
+    // b mod kLsmSyncGroups (kLsmSyncWords words apart, so blocks contend in small groups);
+    // zeroed by the preconditioner merge that precedes it.
+    uint32_t* sync;
 };
+constexpr uint32_t kLsmSyncGroups = 32;
+constexpr uint32_t kLsmSyncWords = 64; // one counter per 256 bytes
+constexpr uint32_t kLsmSyncStride = kLsmSyncGroups * kLsmSyncWords; // words per run
 
 // The arguments of run `y` of a batch (itself for a single run). The capture buffers are not
 // batched: only single runs capture.
@@ -128,6 +143,7 @@ __device__ __forceinline__ void select_run(LsmArgs& a, uint64_t y) {
     a.factors += (a.run_gram ? y : 0) * (uint64_t)(a.m + 1);
     a.ne += y * a.run_ne;
     a.partials += y * a.run_tiles;
+    a.sync += y * kLsmSyncStride;
 }
 __device__ __forceinline__ void select_run(LsmArgs& a) { select_run(a, blockIdx.y); }
 
@@ -192,13 +208,14 @@ __host__ __device__ __forceinline__ double disc_pow(double disc, int k) {
 }
 
 // Every path's value at maturity, intrinsic x disc^M (lsm.cpp:76-81, discounted as :84-86 and
-// :125-127 will); block 0 of each run also clears the run's policy (no coefficients at any date
-// yet), and of run 0 forms the discount powers.
+// :125-127 will); block 0 of each run also zeroes the run's normal-equation sums and, when
+// fitting, clears the run's policy (no coefficients at any date yet), and of run 0 forms the
+// discount powers.
 template <bool FP64>
-__global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
+__global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a, int fit) {
     select_run(a);
     if (blockIdx.x == 0) {
-        for (int i = (int)threadIdx.x; i <= a.m; i += (int)blockDim.x) {
+        for (int i = (int)threadIdx.x; i <= a.m && fit; i += (int)blockDim.x) {
             a.has[i] = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) a.coeffs[4 * i + j] = 0.0;
@@ -220,13 +237,12 @@ __global__ __launch_bounds__(256) void lsm_init_cf_kernel(LsmArgs a) {
 // ---- Backward induction -------------------------------------------------------------------
 // Path layout: B = min(256, ceil(n / 4096)) blocks of 512 lanes (two waves per SIMD); lane t
 // of block b owns paths ((j * B + b) * 512 + t) for j < P = ceil(n / (512 B)), coalesced per j
-// (P = 8 at 2^20 paths). Before the dates' kernels, the Gram kernels precondition every date at once (grid:
-// blocks x dates) from the rows of the first min(n, 2^16) paths; then each date runs one
-// kernel: the later date's
-// coefficients from the B blocks' normal-equation sums, the decision at the later date,
-// discount, and the normal equations of this date's preconditioned rows, summed over the
-// block. The layout depends on n only and every sum runs in a fixed order, so the
-// coefficients are bitwise reproducible.
+// (P = 8 at 2^20 paths). Before the dates, the Gram kernels precondition every date at once
+// (grid: blocks x dates) from the rows of the first min(n, 2^16) paths; then, per date (in
+// one resident launch, or one launch each): the later date's coefficients from the B blocks'
+// normal-equation sums, the decision at the later date, discount, and the normal equations of
+// this date's preconditioned rows, summed over the block. The layout depends on n only and
+// every sum runs in a fixed order, so the coefficients are bitwise reproducible.
 constexpr int kLsmLanes = 512;
 constexpr int kLsmWaves = kLsmLanes / 64;
 constexpr uint32_t kLsmMaxBlocks = kLsmNeStride;
@@ -507,6 +523,7 @@ template <bool FP64>
 __global__ __launch_bounds__(kLsmLanes) void lsm_gram_merge_kernel(LsmArgs a, uint32_t gb) {
     select_run(a);
     const int date = (int)blockIdx.x + 1;
+    if (date == 1 && threadIdx.x < kLsmSyncGroups) a.sync[threadIdx.x * kLsmSyncWords] = 0; // before use
     __shared__ double lds[kLsmLdsDoubles];
     double g[kGram];
 #pragma unroll
@@ -745,6 +762,381 @@ __global__ __launch_bounds__(kLsmLanes) void lsm_date_kernel(LsmArgs a, int date
     const double t = block_sums<kNe>(acc, lds);
     if (threadIdx.x < (unsigned)kNe)
         a.ne[((size_t)(date & 1) * kNe + threadIdx.x) * kLsmMaxBlocks + blockIdx.x] = t;
+}
+
+// ---- Every date in one launch -------------------------------------------------------------
+// The dates' kernels' work in one launch whose blocks are all resident at once (the host
+// checks the occupancy before choosing it, and chains such launches so that only one occupies
+// the device at a time). A block is one control wave and kLsmWaves path waves. Path lane t (of
+// kLsmLanes) holds the paths the dates' kernels give lane t of the block (at most kLsmChunk:
+// n <= 2^20 per run) and keeps their values and the weights e^{-x/2} of the date's rows in
+// registers; the store columns come into LDS by DMA, each once, a date ahead. The blocks
+// exchange each date's normal-equation sums through the ne buffer with device-scope (sc1)
+// accesses and no cache write-back or invalidation: the control wave stores the block's 15
+// sums at device scope, waits until the stores are complete, then adds one to its group's
+// counter (block b, counter b mod kLsmSyncGroups, so that the additions of one date contend in
+// small groups); a block needs the later date's sums once every counter holds its blocks'
+// additions for that date, and its control wave then reads all blocks' sums into LDS by
+// device-scope 16-byte DMA loads, adds them, solves and hands the coefficients to the path
+// waves (one barrier); the path waves' wave sums of the date come back through LDS (a second
+// barrier). Every sum is formed by the same operations in the same order as in the dates'
+// kernels, so the coefficients, decisions and values are bit for bit theirs. The path waves
+// form the next date's rows (weights, Gram sums and count) while the control wave waits, and
+// rebuild each row from its weight for the q sums. A wait gives up after kLsmSpinLimit polls
+// (seconds; never reached while every block is resident): the block stops waiting and its
+// values become NaN, so the run's price reads NaN and the entry point reports a device error.
+constexpr int kLsmResidentMaxDates = 512; // discount powers held in LDS
+constexpr uint32_t kLsmSpinLimit = 1u << 21;
+constexpr int kLsmResidentLanes = kLsmLanes + 64; // the control wave and the path waves
+constexpr int kLsmStageDoubles = kNe * kLsmNeStride;
+constexpr int kGramSums = 11; // G' packed (sums 0..9) and the count (sum 14)
+constexpr int kLsmPathRegion = kGramSums * kLsmWaveStride; // a path wave's wave_sums region
+
+template <bool FP64>
+using StoreT = typename std::conditional<FP64, double, float>::type;
+
+// Cache policy of a device-scope (sc1) access: coherent across the device's XCDs without a
+// write-back or invalidation of the L2.
+constexpr int kCpolSc1 = 16;
+// s_waitcnt with at most n vector-memory operations outstanding (expcnt, lgkmcnt not waited).
+constexpr int vmcnt_at_most(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// A path wave's copy of one store column in LDS, filled by 4-byte DMA loads so no register is
+// held while they are in flight: cb[u][lane] = path u of the lane (FP32), or its low and high
+// words at cb[2u][lane] and cb[2u + 1][lane] (FP64); path 0 past the end. The lane's path u is
+// at byte boff + u stride of the column.
+template <bool FP64>
+constexpr int kColWords = (FP64 ? 2 : 1) * (int)kLsmChunk * 64;
+template <bool FP64>
+constexpr int kColLoads = (FP64 ? 2 : 1) * (int)kLsmChunk; // DMA loads per column and wave
+
+template <bool FP64>
+__device__ __forceinline__ void stage_column(const LsmArgs& a, int date, uint32_t boff,
+                                             uint32_t stride, uint32_t in_mask, uint32_t* cb) {
+    const char* col = (const char*)a.store + (uint64_t)(date - 1) * a.n_paths * sizeof(StoreT<FP64>);
+#pragma unroll
+    for (uint32_t u = 0; u < kLsmChunk; ++u) {
+        const char* src = col + (((in_mask >> u) & 1u) ? boff + u * stride : 0u);
+#pragma unroll
+        for (uint32_t h = 0; h < (FP64 ? 2u : 1u); ++h)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 4 * h),
+                                             (__attribute__((address_space(3))) void*)(cb + ((FP64 ? 2 : 1) * u + h) * 64),
+                                             4, 0, 0);
+    }
+}
+
+template <bool FP64>
+__device__ __forceinline__ double staged(const uint32_t* cb, uint32_t u, uint32_t l) {
+    if constexpr (FP64) {
+        const uint64_t lo = cb[(2 * u) * 64 + l], hi = cb[(2 * u + 1) * 64 + l];
+        return __builtin_bit_cast(double, (hi << 32) | lo);
+    } else {
+        return (double)__builtin_bit_cast(float, cb[u * 64 + l]);
+    }
+}
+
+// A date's preconditioned row of path u of the lane from its weight e^{-x/2} (0 out of the
+// money and past the end) and its staged column: the dates' kernel's design_row and
+// precondition_row, operation for operation.
+template <bool FP64>
+__device__ __forceinline__ void resident_row(const LsmArgs& a, const DateFactor& f, const uint32_t* cb,
+                                             uint32_t u, double e, double* w) {
+    double row[4];
+    laguerre4_weighted(staged<FP64>(cb, u, threadIdx.x & 63) * a.inv_strike, e, row);
+    precondition_row(f, row, w);
+}
+
+// A date's rows from its staged column: the weights e^{-x/2} of the lane's paths (0 out of the
+// money and past the end; the rows follow from them and the column) and the wave's sums of what
+// the rows fix alone, G' and the count, as sums 0..9 and 14 of the wave's totals `tot` (the q
+// sums follow once the values are known). Per sum, the operations and their order are those of
+// the dates' kernel.
+template <bool FP64>
+__device__ __forceinline__ void resident_rows(const LsmArgs& a, const DateFactor& f, const uint32_t* cb,
+                                              uint32_t in_mask, double (&ew)[kLsmChunk],
+                                              uint32_t& itm_mask, double* wl, double* tot) {
+    const uint32_t l = threadIdx.x & 63;
+    double g[kGramSums];
+#pragma unroll
+    for (int k = 0; k < kGramSums; ++k) g[k] = 0.0;
+    itm_mask = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kLsmChunk; ++u) {
+        const double s0 = staged<FP64>(cb, u, l);
+        const bool itm = ((in_mask >> u) & 1u) && intrinsic_v(s0, a.strike, a.type) > 0.0;
+        ew[u] = itm ? exp(-(s0 * a.inv_strike) / 2.0) : 0.0;
+        double w[4];
+        resident_row<FP64>(a, f, cb, u, ew[u], w);
+#pragma unroll
+        for (int i = 0; i < kLsmBasis; ++i) {
+#pragma unroll
+            for (int j = i; j < kLsmBasis; ++j) g[ridx(i, j)] = __builtin_fma(w[i], w[j], g[ridx(i, j)]);
+        }
+        g[10] += itm ? 1.0 : 0.0;
+        itm_mask |= (itm ? 1u : 0u) << u;
+    }
+    const double s = wave_sums<kGramSums>(g, wl);
+    if ((l & 3) == 0 && (l >> 2) < (unsigned)kGramSums) tot[(l >> 2) < 10 ? (l >> 2) : 14] = s;
+}
+
+// A date's preconditioner staged in LDS (slot of 16 doubles) by a 4-byte DMA load per lane of
+// lanes 0..27.
+__device__ __forceinline__ void stage_factor(const DateFactor* src, double* slot) {
+    static_assert(sizeof(DateFactor) == 28 * 4, "28 words");
+    if ((threadIdx.x & 63) < 28)
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)((const uint32_t*)src + (threadIdx.x & 63)),
+            (__attribute__((address_space(3))) void*)slot, 4, 0, 0);
+}
+__device__ __forceinline__ DateFactor staged_factor(const double* slot) {
+    DateFactor f;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) f.r[i] = uniform_f64(slot[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.inv_d[i] = uniform_f64(slot[10 + i]);
+    return f;
+}
+
+template <bool FP64, bool CAP>
+__global__ __launch_bounds__(kLsmResidentLanes) void lsm_resident_kernel(LsmArgs a, int fit) {
+    select_run(a);
+    __shared__ double regions[kLsmWaveRegion + kLsmWaves * kLsmPathRegion]; // wave_sums regions
+    __shared__ uint32_t cols[kLsmWaves][2][kColWords<FP64>]; // column d at [d & 1], per path wave
+    __shared__ double tot[2][kLsmWaves][kNe]; // the path waves' totals of the even / odd dates
+    __shared__ double res[kNe + 5];           // the solve's totals, then beta [4] and the flag
+    __shared__ double stage[kLsmStageDoubles]; // [sum][block]: the later date's block sums
+    __shared__ double dpw[kLsmResidentMaxDates + 1];
+    __shared__ double fsl[4][16];             // date d's preconditioner at [d & 3]
+    __shared__ int dead_flag;
+    const int m = a.m;
+    const uint32_t blk = blockIdx.x;
+    if (threadIdx.x < 64) { // the control wave
+        const uint32_t l = threadIdx.x;
+        if (l == 0) {
+            double d = 1.0; // disc^k by repeated multiplication, as the dpow table
+            for (int k = 0; k <= m; ++k) {
+                dpw[k] = d;
+                d *= a.disc;
+            }
+            dead_flag = 0;
+        }
+        if (fit && blk == 0) { // this run's policy, cleared before the first date records one
+            for (int i = (int)l; i <= m; i += 64) {
+                a.has[i] = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a.coeffs[4 * i + j] = 0.0;
+            }
+        }
+        // blocks adding to this lane's counter
+        const uint32_t grp_blocks = l < a.blocks ? (a.blocks - l + kLsmSyncGroups - 1) / kLsmSyncGroups : 0;
+        if (fit && m >= 2) stage_factor(a.factors + (m - 1), fsl[(m - 1) & 3]);
+        __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0));
+        __syncthreads(); // the discount powers, the flag, the cleared policy, a factor
+        bool dead = false; // a wait of this block gave up
+        for (int date = m - 1; date >= 0; --date) {
+            const int later = date + 1;
+            // Date - 1's preconditioner, for the path waves' rows after this date's sums.
+            if (fit && date >= 2) stage_factor(a.factors + (date - 1), fsl[(date - 1) & 3]);
+            if (fit && later <= m - 1) {
+                // Every block's sums of the later date (each block adds one per date, dates
+                // m - 1 down), staged in LDS: row k's 256 sums are two 16-byte loads per lane.
+                // Lane g < kLsmSyncGroups reads counter g, whose blocks b = g mod kLsmSyncGroups
+                // have each added once per date.
+                const uint32_t target = (uint32_t)(m - later) * grp_blocks;
+                uint32_t spins = 0;
+                while (!dead) {
+                    const uint32_t c = l < kLsmSyncGroups
+                                           ? __hip_atomic_load(a.sync + l * kLsmSyncWords, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT)
+                                           : target;
+                    if (__builtin_amdgcn_ballot_w64(c < target) == 0) break;
+                    if (++spins > kLsmSpinLimit) dead = true;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                // Device-scope (sc1) reads of the sums, issued once the count is seen.
+                const double* src = a.ne + (size_t)(later & 1) * kNe * kLsmNeStride;
+#pragma unroll
+                for (int k = 0; k < kNe; ++k) {
+#pragma unroll
+                    for (int h = 0; h < kLsmNeStride / 128; ++h)
+                        __builtin_amdgcn_global_load_lds(
+                            (const __attribute__((address_space(1))) void*)(src + k * kLsmNeStride + h * 128 + 2 * l),
+                            (__attribute__((address_space(3))) void*)(stage + k * kLsmNeStride + h * 128),
+                            16, 0, kCpolSc1);
+                }
+                __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0)); // the staged sums (and factor)
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // As the dates' kernel: lane l adds blocks (l + l64) + (l128 + l192), zero past
+                // the blocks, then the wave adds its lanes.
+                double g[kNe];
+#pragma unroll
+                for (int k = 0; k < kNe; ++k) {
+                    double q4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        q4[q] = l + 64 * q < a.blocks ? stage[k * kLsmNeStride + l + 64 * q] : 0.0;
+                    g[k] = (q4[0] + q4[1]) + (q4[2] + q4[3]);
+                }
+                const double tw = wave_sums<kNe>(g, regions);
+                if ((l & 3) == 0 && (l >> 2) < (unsigned)kNe) res[l >> 2] = tw;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                double tt[kNe];
+#pragma unroll
+                for (int k = 0; k < kNe; ++k) tt[k] = res[k];
+                const bool u = tt[14] >= (double)kLsmBasis; // lsm.cpp:102-104
+                double bz[4] = {0.0, 0.0, 0.0, 0.0};
+                if (u) {
+                    ne_solve(tt, staged_factor(fsl[later & 3]), bz);
+                    const double inv = 1.0 / dpw[later];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bz[j] *= inv;
+                }
+                if (l == 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) res[kNe + j] = bz[j];
+                    res[kNe + 4] = u ? 1.0 : 0.0;
+                    if (dead) dead_flag = 1;
+                    if (blk == 0) {
+                        a.has[later] = u ? 1 : 0;
+                        if (u)
+                            for (int j = 0; j < 4; ++j) a.coeffs[4 * later + j] = bz[j];
+                    }
+                }
+                __syncthreads(); // the coefficients to the path waves
+            }
+            if (fit && date >= 1) {
+                __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0)); // date - 1's factor is in
+                __syncthreads(); // the path waves' totals of this date
+                // As block_sums: lane i < kNe adds sum i of the kLsmWaves waves pairwise.
+                double tk[kLsmWaves];
+#pragma unroll
+                for (int k = 0; k < kLsmWaves; ++k) tk[k] = l < (unsigned)kNe ? tot[date & 1][k][l] : 0.0;
+#pragma unroll
+                for (int h = kLsmWaves / 2; h >= 1; h /= 2) {
+#pragma unroll
+                    for (int k = 0; k < h; ++k) tk[k] = tk[2 * k] + tk[2 * k + 1];
+                }
+                // Device-scope (sc1) stores of the sums, complete (the wave's store count drained)
+                // before lane 0's one addition to the count.
+                if (l < (unsigned)kNe)
+                    __hip_atomic_store(a.ne + ((size_t)(date & 1) * kNe + l) * kLsmNeStride + blk, tk[0],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0));
+                __asm__ __volatile__("" ::: "memory");
+                if (l == 0)
+                    __hip_atomic_fetch_add(a.sync + (blk % kLsmSyncGroups) * kLsmSyncWords, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads(); // the flag to the path waves
+    } else { // a path wave
+        const uint32_t t = threadIdx.x - 64, pw = t >> 6, l = t & 63; // path lane, path wave
+        double* wl = regions + kLsmWaveRegion + pw * kLsmPathRegion;
+        // Byte offsets of the lane's paths within a column (n <= 2^20 here): path u at
+        // boff + u stride.
+        const uint32_t esz = (uint32_t)sizeof(StoreT<FP64>);
+        const uint32_t boff = (uint32_t)lane_path(0, blk, a.blocks, t) * esz;
+        const uint32_t stride = a.blocks * (uint32_t)kLsmLanes * esz;
+        uint32_t in_mask = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u) {
+            const uint64_t p = lane_path(u, blk, a.blocks, t);
+            in_mask |= (u < a.per_lane && p < a.n_paths ? 1u : 0u) << u;
+        }
+        // Column d is staged at cols[pw][d & 1] during date d + 1 (after its decisions, which
+        // read the buffer's previous column d + 2), read by the rows of date d and the
+        // decisions at d.
+        stage_column<FP64>(a, m, boff, stride, in_mask, cols[pw][m & 1]);
+        if (m >= 2) stage_column<FP64>(a, m - 1, boff, stride, in_mask, cols[pw][(m - 1) & 1]);
+        __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0));
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double v[kLsmChunk];
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u)
+            v[u] = intrinsic_v(staged<FP64>(cols[pw][m & 1], u, l), a.strike, a.type) * a.disc_m;
+        __syncthreads(); // the discount powers, the flag, the cleared policy, a factor
+        double ew[kLsmChunk]; // the date's row weights
+        uint32_t itm_mask = 0;
+        if (fit && m >= 2)
+            resident_rows<FP64>(a, staged_factor(fsl[(m - 1) & 3]), cols[pw][(m - 1) & 1], in_mask, ew,
+                                itm_mask, wl, tot[(m - 1) & 1][pw]);
+        for (int date = m - 1; date >= 0; --date) {
+            const int later = date + 1;
+            const uint32_t* cl = cols[pw][later & 1]; // the later date's column
+            double beta[4] = {0.0, 0.0, 0.0, 0.0};
+            bool use = false;
+            if (later <= m - 1) { // uniform
+                if (fit) {
+                    __syncthreads(); // the control wave's coefficients
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) beta[j] = uniform_f64(res[kNe + j]);
+                    use = res[kNe + 4] != 0.0;
+                } else {
+                    use = a.has[later] != 0;
+                    if (use)
+                        for (int j = 0; j < 4; ++j) beta[j] = a.coeffs[4 * later + j];
+                }
+            }
+            if (use) { // uniform: the decisions at the later date, as the dates' kernel
+                const double d_later = dpw[later];
+                const ContPoly cp = cont_poly(beta);
+                uint32_t ex = 0, unsure = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < kLsmChunk; ++u) {
+                    bool e;
+                    unsure |= (decide_screen(a, staged<FP64>(cl, u, l), cp, &e) ? 1u : 0u) << u;
+                    ex |= (e ? 1u : 0u) << u;
+                }
+                if (__builtin_amdgcn_ballot_w64(unsure != 0) != 0) { // rare: a path near the boundary
+#pragma unroll 1
+                    for (uint32_t u = 0; u < kLsmChunk; ++u)
+                        if ((unsure >> u) & 1u)
+                            ex = (ex & ~(1u << u)) | ((decide(a, staged<FP64>(cl, u, l), beta) ? 1u : 0u) << u);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kLsmChunk; ++u)
+                    if ((ex >> u) & 1u) v[u] = intrinsic_v(staged<FP64>(cl, u, l), a.strike, a.type) * d_later;
+            }
+            // The decisions have read the later column: its buffer takes the column two dates
+            // earlier (the rows of date - 1, the decisions of date - 2).
+            if (date >= 2) stage_column<FP64>(a, date - 1, boff, stride, in_mask, cols[pw][(date - 1) & 1]);
+            if (fit && date >= 1) { // uniform: this date's q sums
+                const DateFactor fd = staged_factor(fsl[date & 3]);
+                double q[kLsmBasis] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (uint32_t u = 0; u < kLsmChunk; ++u) {
+                    double w[4];
+                    resident_row<FP64>(a, fd, cols[pw][date & 1], u, ew[u], w);
+                    const bool itm = (itm_mask >> u) & 1u;
+                    if constexpr (CAP) {
+                        if ((in_mask >> u) & 1u) {
+                            const uint64_t p = lane_path(u, blk, a.blocks, t);
+                            capture_row(a, date, p, itm, store_get<FP64>(a, date, p), v[u] / dpw[date]);
+                        }
+                    }
+                    const double y = itm ? v[u] : 0.0; // discounted to t = 0
+#pragma unroll
+                    for (int i = 0; i < kLsmBasis; ++i) q[i] = __builtin_fma(w[i], y, q[i]);
+                }
+                const double sq = wave_sums<kLsmBasis>(q, wl);
+                if ((l & 3) == 0 && (l >> 2) < (unsigned)kLsmBasis) tot[date & 1][pw][10 + (l >> 2)] = sq;
+                __syncthreads(); // this date's totals to the control wave
+            }
+            if (fit && date >= 2) { // the next date's rows, while the control wave waits
+                __builtin_amdgcn_s_waitcnt(vmcnt_at_most(0)); // its column
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                resident_rows<FP64>(a, staged_factor(fsl[(date - 1) & 3]), cols[pw][(date - 1) & 1], in_mask,
+                                    ew, itm_mask, wl, tot[(date - 1) & 1][pw]);
+            }
+        }
+        __syncthreads(); // the control wave's flag
+        const bool lost = dead_flag != 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kLsmChunk; ++u)
+            if ((in_mask >> u) & 1u) a.val[lane_path(u, blk, a.blocks, t)] = lost ? __builtin_nan("") : v[u];
+    }
 }
 
 // Welford over the final cash flows in the canonical tile order.
@@ -1221,7 +1613,9 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
                  o_ne = take(with_store ? 2 * (size_t)kNe * kLsmMaxBlocks * 8 * runs : 0),
                  o_part = take((tiles + 1) * runs * sizeof(Welford)),
                  o_coeffs = take((size_t)(m + 1) * 4 * 8 * runs),
-                 o_has = take(((size_t)m + 1) * runs), o_extra = take(extra_bytes);
+                 o_has = take(((size_t)m + 1) * runs),
+                 o_sync = take(with_store ? (size_t)kLsmSyncStride * 4 * runs : 0),
+                 o_extra = take(extra_bytes);
     int st = ensure_dev(ctx, off);
     if (st) return st;
     char* base = (char*)ctx->dev;
@@ -1235,6 +1629,7 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     a.factors = (DateFactor*)(base + o_dfac);
     a.ne = (double*)(base + o_ne);
     a.partials = (Welford*)(base + o_part);
+    a.sync = (uint32_t*)(base + o_sync);
     run->a = a;
     run->blocks = blocks;
     run->tiles = tiles;
@@ -1242,21 +1637,44 @@ int lsm_setup(double spot, double strike, double rate, double carry, double vol,
     return MCDX_OK;
 }
 
-// Cash flows at maturity and a cleared policy, once the store holds the paths.
-template <bool FP64>
-hipError_t enqueue_init(const LsmRun& r, hipStream_t s) {
-    const unsigned g = (unsigned)r.blocks;
-    hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3(g, r.runs), dim3(256), 0, s, r.a);
-    return launch_check();
-}
-
 template <bool FP64>
 hipError_t enqueue_paths(const LsmRun& r, hipStream_t s) {
     if (r.runs != 1) return hipErrorInvalidValue; // Philox paths: one run at a time
     const unsigned g = (unsigned)r.blocks;
     hipLaunchKernelGGL(lsm_paths_kernel<FP64>, dim3(g), dim3(256), 0, s, r.a);
-    hipError_t e = launch_check();
-    return e == hipSuccess ? enqueue_init<FP64>(r, s) : e;
+    return launch_check();
+}
+
+// Backward-induction schedule (mcdx_lsm_schedule): 0 automatic, 1 one launch per date.
+std::atomic<int> g_lsm_schedule{MCDX_LSM_AUTO};
+
+// Completion of the last resident backward induction launched on each device.
+constexpr int kLsmMaxDevices = 64;
+std::mutex g_resident_mu;
+hipEvent_t g_resident_done[kLsmMaxDevices] = {};
+
+// Whether one resident launch can run every date of this batch: at most kLsmChunk paths per
+// lane, the discount powers within LDS, and every block of the grid resident at once on the
+// current device (occupancy per CU x CUs, cached per device).
+template <bool FP64, bool CAP>
+bool resident_fits(const LsmRun& r) {
+    if (g_lsm_schedule.load(std::memory_order_relaxed) != MCDX_LSM_AUTO) return false;
+    if (r.a.per_lane > kLsmChunk || r.a.m > kLsmResidentMaxDates) return false;
+    static std::atomic<int> capacity[kLsmMaxDevices]; // resident blocks + 1; 0 = not yet known
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kLsmMaxDevices) return false;
+    int c = capacity[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
+        int occ = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, lsm_resident_kernel<FP64, CAP>, kLsmResidentLanes, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            occ = 0;
+            cus = 0;
+        }
+        c = occ * cus + 1;
+        capacity[dev].store(c, std::memory_order_relaxed);
+    }
+    return (uint64_t)r.a.blocks * r.runs <= (uint64_t)(c - 1);
 }
 
 // Where the store's paths come from: Philox (the reference's, lsm.cpp:35-58) or Sobol +
@@ -1272,6 +1690,8 @@ struct PathSource {
 template <bool FP64>
 hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
     const dim3 g(r.a.blocks, r.runs);
+    const bool cap = fit && r.a.cap_y;
+    const bool resident = cap ? resident_fits<FP64, true>(r) : resident_fits<FP64, false>(r);
     if (fit && r.a.m > 1) { // every date's preconditioner, before the dates' kernels
         // gb <= 32 blocks per date, at least 16 sample paths per lane
         uint64_t b = (r.a.sample_n + 16 * kLsmLanes - 1) / (16 * kLsmLanes);
@@ -1283,17 +1703,39 @@ hipError_t enqueue_backward(const LsmRun& r, bool fit, hipStream_t s) {
         hipLaunchKernelGGL(lsm_gram_merge_kernel<FP64>, dim3(r.a.m - 1, r.runs), dim3(kLsmLanes), 0,
                            s, r.a, gb);
     }
-    const bool cap = fit && r.a.cap_y;
-    for (int date = r.a.m - 1; date >= 1; --date) {
+    if (resident) {
+        // One resident grid at a time on the device: two that shared it could each hold CUs the
+        // other's waiting blocks need. Each launch waits (on the device) for the previous one,
+        // on whichever stream of this process it ran.
+        int dev = 0;
+        hipError_t ge = hipGetDevice(&dev);
+        if (ge != hipSuccess) return ge;
+        std::lock_guard<std::mutex> lock(g_resident_mu);
+        hipEvent_t& last = g_resident_done[dev];
+        if (last) ge = hipStreamWaitEvent(s, last, 0);
+        else ge = hipEventCreateWithFlags(&last, hipEventDisableTiming);
+        if (ge != hipSuccess) return ge;
         if (cap)
-            hipLaunchKernelGGL((lsm_date_kernel<FP64, true>), g, dim3(kLsmLanes), 0, s, r.a, date, 1, 1);
+            hipLaunchKernelGGL((lsm_resident_kernel<FP64, true>), g, dim3(kLsmResidentLanes), 0, s, r.a, 1);
         else
-            hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a,
-                               date, fit ? 1 : 0, fit ? 1 : 0);
+            hipLaunchKernelGGL((lsm_resident_kernel<FP64, false>), g, dim3(kLsmResidentLanes), 0, s, r.a,
+                               fit ? 1 : 0);
+        ge = hipEventRecord(last, s);
+        if (ge != hipSuccess) return ge;
+    } else {
+        hipLaunchKernelGGL(lsm_init_cf_kernel<FP64>, dim3((unsigned)r.blocks, r.runs), dim3(256), 0, s,
+                           r.a, fit ? 1 : 0);
+        for (int date = r.a.m - 1; date >= 1; --date) {
+            if (cap)
+                hipLaunchKernelGGL((lsm_date_kernel<FP64, true>), g, dim3(kLsmLanes), 0, s, r.a, date, 1, 1);
+            else
+                hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a,
+                                   date, fit ? 1 : 0, fit ? 1 : 0);
+        }
+        // date 0: the decisions at date 1 (fitting date 1 first when fitting), no regression
+        hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a, 0, 0,
+                           fit ? 1 : 0);
     }
-    // date 0: the decisions at date 1 (fitting date 1 first when fitting), no regression
-    hipLaunchKernelGGL((lsm_date_kernel<FP64, false>), g, dim3(kLsmLanes), 0, s, r.a, 0, 0,
-                       fit ? 1 : 0);
     hipLaunchKernelGGL(lsm_welford_kernel<FP64>, dim3((unsigned)r.tiles, r.runs), dim3(kTileLanes), 0,
                        s, r.a);
     hipError_t e = launch_check();
@@ -1336,7 +1778,7 @@ int lsm_price_src(double spot, double strike, double rate, double carry, double 
                                  src.shift_seed, src.replicate, runs, r.a.store, scratch,
                                  ctx->stream)))
             return st;
-        e = fp64 ? enqueue_init<true>(r, ctx->stream) : enqueue_init<false>(r, ctx->stream);
+        e = hipSuccess;
     } else {
         e = fp64 ? enqueue_paths<true>(r, ctx->stream) : enqueue_paths<false>(r, ctx->stream);
     }
@@ -1356,6 +1798,8 @@ int lsm_price_src(double spot, double strike, double rate, double carry, double 
     }
     // inception decision (lsm.cpp:141-148)
     const double inc = intrinsic_v(spot, strike, type);
+    for (uint32_t k = 0; k < runs; ++k) // a resident backward induction that gave up waiting
+        if (!(((const Welford*)h)[k].mean == ((const Welford*)h)[k].mean)) return MCDX_EDEVICE;
     for (uint32_t k = 0; k < runs; ++k) {
         const mcdx_result res = finish(((const Welford*)h)[k]);
         const bool exercise = inc > res.price;
@@ -1441,6 +1885,7 @@ int lsm_reprice(double spot, double strike, double rate, double carry, double vo
         e = hipMemcpyAsync(h, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return status_of(e);
+    if (!(((const Welford*)h)->mean == ((const Welford*)h)->mean)) return MCDX_EDEVICE;
     const mcdx_result res = finish(*(const Welford*)h);
     if (inception) // lsm.cpp:207-211
         *out = mcdx_result{intrinsic_v(spot, strike, type), 0.0, res.path_count};
@@ -1833,6 +2278,11 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
 using namespace mcdx;
 
 extern "C" {
+
+int mcdx_lsm_schedule(int schedule) {
+    if (schedule != MCDX_LSM_AUTO && schedule != MCDX_LSM_PER_DATE) return -MCDX_EINVAL;
+    return g_lsm_schedule.exchange(schedule);
+}
 
 int mcdx_lsm_american_qmc(double spot, double strike, double rate, double carry_yield, double vol,
                           double time, uint8_t type, int32_t monitoring_points,

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mcdx_issue_probe": (C.c_int, [C.c_int, u32, P(dbl), P(u64)]),
     "mcdx_fault_inject": (C.c_int, [C.c_int, C.c_int32]),
     "mcdx_fault_pending": (C.c_int, [C.c_int]),
+    "mcdx_lsm_schedule": (C.c_int, [C.c_int]),
     "mcdx_monte_carlo_european": (C.c_int, [dbl] * 6 + [u8, u64, u64, P(Options), P(Result)]),
     "mcdx_monte_carlo_digital": (C.c_int, [dbl] * 6 + [u8, u8, dbl, u64, u64, P(Options),
                                                        P(Result)]),
@@ -215,6 +216,18 @@ def fault_inject(site, countdown):
     """Arm (countdown >= 1) or disarm (0) an injected failure of this thread's
     countdown-th device allocation (FAULT_ALLOC) or launch check (FAULT_LAUNCH)."""
     check(lib().mcdx_fault_inject(site, countdown), "mcdx_fault_inject")
+
+
+LSM_AUTO, LSM_PER_DATE = 0, 1
+
+
+def lsm_schedule(schedule):
+    """Select the LSM backward-induction schedule (LSM_AUTO: one resident launch for every
+    date when the run fits; LSM_PER_DATE: one launch per date); returns the previous one."""
+    prev = lib().mcdx_lsm_schedule(schedule)
+    if prev < 0:
+        check(-prev, "mcdx_lsm_schedule")
+    return prev
 
 
 ISSUE_CLASSES = {"fma_f32": 0, "mad_u64_u32": 1, "mul_hi_u32": 2, "xor3": 3, "exp2_f32": 4,

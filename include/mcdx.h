@@ -267,6 +267,15 @@ int mcdx_lsm_reprice_frozen(double spot, double strike, double rate, double carr
                             const uint8_t* has_coefficients, uint8_t exercise_at_inception,
                             int precision, mcdx_result* out);
 
+/* Backward-induction schedule (no reference counterpart): MCDX_LSM_AUTO (the default) runs
+ * every date of a regression in one launch whose blocks stay resident when the run fits (at
+ * most 2^20 paths per run, at most 1024 dates, every block of the batch resident at once on
+ * the device), else one launch per date; MCDX_LSM_PER_DATE always launches per date. Both give
+ * the same bits. Process-wide; returns the previous schedule, or -MCDX_EINVAL. */
+#define MCDX_LSM_AUTO 0
+#define MCDX_LSM_PER_DATE 1
+int mcdx_lsm_schedule(int schedule);
+
 /* American Greeks by central differences of frozen-policy prices (reprice_against_frozen_policy,
  * src/pricers/lsm.cpp:151-213, as the reference's tests/lsm_test.cpp:138-170 forms delta):
  * one QR-LSM fit at the base parameters, then the base, spot +/- spot_bump and vol +/- vol_bump
