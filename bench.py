@@ -558,7 +558,7 @@ def c5_lsm(mcdx, mcd):
     FP32 store. QR-LSM (store + 49 device regressions) against the learned continuation
     value: the full pipeline (QR fit, capture, training, table, pricing) once, then the
     trained policy serving fresh paths in one streaming kernel (no path store). Wall clock
-    of the synchronous calls, median of 3 after a warm-up."""
+    of the synchronous calls after a warm-up: median of 15 (QR-LSM), 7 (Greeks) or 3."""
     put = (100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 1, 50)
     n = 1 << 20
 
@@ -571,7 +571,7 @@ def c5_lsm(mcdx, mcd):
             ts.append(time.perf_counter() - t0)
         return sorted(ts)[len(ts) // 2] * 1e3, out
 
-    qr_ms, q = med(lambda: mcdx.lsm(*put, n, SEED, mcdx.PREC_FP32))
+    qr_ms, q = med(lambda: mcdx.lsm(*put, n, SEED, mcdx.PREC_FP32), reps=15)
     pipe_ms, (d, w, co, has) = med(lambda: mcdx.lsm_learned(*put, n, SEED, mcdx.PREC_FP32), reps=1)
     serve_ms, sv = med(lambda: mcdx.lsm_learned_price(*put, n, SEED + 1, mcdx.PREC_FP32, w, 1024,
                                                       d["table_lo"], d["table_hi"], co, has,
@@ -594,7 +594,7 @@ def c5_lsm(mcdx, mcd):
     berm = crr_bermudan_put(100.0, 100.0, 0.06, 0.0, 0.2, 1.0, 10_000, 200)
     # Frozen-policy American Greeks (one fused pass of 5 bumped scenarios over the QR policy),
     # against central differences of CRR-2000 with the same bumps (1% spot, 1% vol).
-    gk_ms, gk = med(lambda: mcdx.lsm_greeks(*put, n, SEED, mcdx.PREC_FP32))
+    gk_ms, gk = med(lambda: mcdx.lsm_greeks(*put, n, SEED, mcdx.PREC_FP32), reps=7)
 
     # QR-LSM on Sobol + bridge paths: 16 digitally shifted runs of 2^16 paths (2^20 in all),
     # whose spread gives the error the in-sample Philox estimate does not state.
