@@ -285,6 +285,27 @@ constexpr int kTilePaths = kTileLanes * kPathsPerLane;
 constexpr int kMergeGroups = 1024;
 
 #if defined(__HIPCC__)
+// Fixed-order merge of n_tiles consecutive tile partials by one block of kMergeGroups threads:
+// group g folds the tiles chunk_bounds(g, n_tiles, kMergeGroups) left to right (the
+// reference's static partition, thread_pool.cpp:9-18), then a pairwise tree over the groups
+// in index order. The total is valid on thread 0.
+__device__ __forceinline__ Welford merge_tiles_fixed_order(const Welford* __restrict__ partials,
+                                                           uint64_t n_tiles, Welford* lds) {
+    const uint32_t g = threadIdx.x;
+    const uint64_t q = n_tiles / kMergeGroups, rem = n_tiles % kMergeGroups;
+    const uint64_t b = g * q + (g < rem ? g : rem);
+    const uint64_t e = b + q + (g < rem ? 1 : 0);
+    Welford acc = {0, 0.0, 0.0};
+    for (uint64_t t = b; t < e; ++t) acc = welford_merge(acc, partials[t]);
+    lds[g] = acc;
+    __syncthreads();
+    for (uint32_t s = 1; s < kMergeGroups; s <<= 1) {
+        if ((g & (2 * s - 1)) == 0) lds[g] = welford_merge(lds[g], lds[g + s]);
+        __syncthreads();
+    }
+    return lds[0];
+}
+
 // Pairwise tree over the 256 lanes of a block: within each wave by lane offset 1..32, then
 // across the four waves (w0+w1, w2+w3, then both). Returns the tile total on thread 0.
 __device__ __forceinline__ Welford block_merge_256(Welford w, Welford* lds4) {

@@ -1165,6 +1165,14 @@ __global__ __launch_bounds__(kTileLanes) void lsm_welford_kernel(LsmArgs a) {
 // The network's continuation value enters as a table over S/K per exercise date: it is the
 // network evaluated (by the MFMA forward kernel) on a uniform grid, read with linear
 // interpolation. Outside the table's range the QR coefficients decide (the fallback).
+// A policy run's merged results, adjacent so that one copy brings them back: the network's
+// and the QR policy's Welford totals and the counts of joint ITM decisions, disagreements and
+// table-range fallbacks.
+struct PolicyResults {
+    Welford nn, qr;
+    unsigned long long ctr[3];
+};
+
 struct PolicyArgs {
     const float* table;           // [date 1..M-1][grid], kTargetScale * continuation / K
     uint32_t grid;
@@ -1176,39 +1184,13 @@ struct PolicyArgs {
     uint32_t* tile_counts;        // [3][tiles] ITM decisions, disagreements, fallbacks
     int judge;                    // 0: the QR policy only decides outside the table range;
                                   // 1: both policies on every path; 2: the QR policy alone
+    // FP32 decisions (lsm_serve_kernel): table coordinate u = s ua + ub, in range for
+    // 0 <= u <= umax = grid - 1, cell index clamped to gmax = grid - 2; in the money iff
+    // phi s > kedge (phi = +1 call, -1 put; kedge = phi times the strike rounded outward to
+    // FP32); exercise margin (phi s + phik) - c cont_scale_f (phik = -phi K).
+    float ua, ub, umax, gmax;
+    float kedge, phik, cont_scale_f;
 };
-
-// counters[c] = sum over tiles of tile_counts[c][tile], c < 3 (one block).
-__global__ __launch_bounds__(256) void lsm_count_kernel(const uint32_t* tile_counts,
-                                                        uint64_t tiles,
-                                                        unsigned long long* counters) {
-    __shared__ unsigned long long part[3][4];
-    unsigned long long acc[3] = {0, 0, 0};
-    for (uint64_t i = threadIdx.x; i < tiles; i += blockDim.x)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += tile_counts[(uint64_t)c * tiles + i];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
-        if ((threadIdx.x & 63) == 0) part[c][threadIdx.x >> 6] = acc[c];
-    }
-    __syncthreads();
-    if (threadIdx.x < 3)
-        counters[threadIdx.x] = part[threadIdx.x][0] + part[threadIdx.x][1] +
-                                part[threadIdx.x][2] + part[threadIdx.x][3];
-}
-
-// Grid features for the table: (4 (x_g - 1), 2 d/M - 1) for d = 1..M-1, g < grid.
-__global__ __launch_bounds__(256) void lsm_grid_kernel(float* f1, float* f2, int m, uint32_t grid,
-                                                       double xlo, double h) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)(m - 1) * grid) return;
-    const int date = (int)(i / grid) + 1;
-    const uint32_t g = (uint32_t)(i % grid);
-    f1[i] = (float)(kFeatScale * (xlo + h * (double)g - 1.0));
-    f2[i] = (float)(2.0 * (double)date / (double)m - 1.0);
-}
 
 // Both policies' state along one path: stopped flags, exercise dates and values, and the
 // path's counts of joint ITM decisions, disagreements and table-range fallbacks.
@@ -1273,18 +1255,132 @@ __device__ __forceinline__ void policy_visit(const LsmArgs& a, const PolicyArgs&
     if (!ps.n_done && dn) { ps.n_done = true; ps.n_date = date; ps.n_val = ex; }
 }
 
-// One path per thread; a block of 1024 threads is one canonical tile (thread t computes path
-// tile * 1024 + t), and the tile's Welford partial is rebuilt from LDS in the canonical
-// order (lane l < 256 adds paths l, l+256, l+512, l+768 in turn, then the 256-lane tree), so
-// it equals the partial of the 256-lane path kernels. Each path is regenerated exactly as the
-// store kernel writes it and stopped at each policy's first exercise date; the cash flow is
-// then discounted by one multiplication per date, the order backward induction applies them
-// in, so the QR policy reproduces the backward-induction price bit for bit.
-template <bool FP64>
-__global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, PolicyArgs q) {
-    __shared__ double sh_n[kTilePaths], sh_q[kTilePaths];
+// The tile's close for a policy kernel (one path per thread, a block of 1024 threads is one
+// canonical tile: thread t computed path tile * 1024 + t): the per-tile counts of ITM
+// decisions, disagreements and fallbacks, and the Welford partials of the policies formed (NN:
+// the network's, QR: the QR policy's) rebuilt from LDS in the canonical order (lane l < 256
+// adds paths l, l+256, l+512, l+768 in turn, then the 256-lane tree in waves 0..3), so they
+// equal the partials of the 256-lane path kernels.
+template <bool NN, bool QR>
+__device__ __forceinline__ void policy_tile_close(const LsmArgs& a, const PolicyArgs& q,
+                                                  uint64_t tile, uint32_t t, double cn, double cq,
+                                                  uint32_t c_dec, uint32_t c_dis, uint32_t c_fb) {
+    __shared__ double sh_n[NN ? kTilePaths : 1], sh_q[QR ? kTilePaths : 1];
     __shared__ Welford lds_n[4], lds_q[4];
     __shared__ uint32_t sh_cnt[3][16];
+    if constexpr (NN) sh_n[t] = cn;
+    if constexpr (QR) sh_q[t] = cq;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        c_dec += __shfl_xor(c_dec, off, 64);
+        c_dis += __shfl_xor(c_dis, off, 64);
+        c_fb += __shfl_xor(c_fb, off, 64);
+    }
+    if ((t & 63) == 0) {
+        sh_cnt[0][t >> 6] = c_dec;
+        sh_cnt[1][t >> 6] = c_dis;
+        sh_cnt[2][t >> 6] = c_fb;
+    }
+    __syncthreads();
+    if (t < 3) { // per-tile counts, summed by lsm_policy_close_kernel (no same-address atomics)
+        uint32_t sum = 0;
+        for (int w = 0; w < 16; ++w) sum += sh_cnt[t][w];
+        q.tile_counts[(uint64_t)t * gridDim.x + tile] = sum;
+    }
+    if (t < (uint32_t)kTileLanes) { // waves 0..3 (wave-uniform)
+        Welford wn = {0, 0.0, 0.0}, wq = {0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < kPathsPerLane; ++k) {
+            const uint32_t i = (uint32_t)(k * kTileLanes) + t;
+            if (tile * kTilePaths + i >= a.n_paths) break;
+            const double rk = 1.0 / (double)(k + 1);
+            if constexpr (NN) {
+                const double xn = sh_n[i];
+                wn.n += 1;
+                const double d = xn - wn.mean;
+                wn.mean += d * rk;
+                wn.m2 += d * (xn - wn.mean);
+            }
+            if constexpr (QR) {
+                const double xq = sh_q[i];
+                wq.n += 1;
+                const double d = xq - wq.mean;
+                wq.mean += d * rk;
+                wq.m2 += d * (xq - wq.mean);
+            }
+        }
+        // the 256-lane tree of block_merge_256
+        const int lane = t & 63;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            if constexpr (NN) {
+                Welford on;
+                on.n = __shfl_down(wn.n, off, 64);
+                on.mean = __shfl_down(wn.mean, off, 64);
+                on.m2 = __shfl_down(wn.m2, off, 64);
+                if ((lane & (2 * off - 1)) == 0) wn = welford_merge(wn, on);
+            }
+            if constexpr (QR) {
+                Welford oq;
+                oq.n = __shfl_down(wq.n, off, 64);
+                oq.mean = __shfl_down(wq.mean, off, 64);
+                oq.m2 = __shfl_down(wq.m2, off, 64);
+                if ((lane & (2 * off - 1)) == 0) wq = welford_merge(wq, oq);
+            }
+        }
+        if (lane == 0) {
+            lds_n[t >> 6] = wn;
+            lds_q[t >> 6] = wq;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        if constexpr (NN)
+            q.part_nn[tile] = welford_merge(welford_merge(lds_n[0], lds_n[1]),
+                                            welford_merge(lds_n[2], lds_n[3]));
+        if constexpr (QR)
+            q.part_qr[tile] = welford_merge(welford_merge(lds_q[0], lds_q[1]),
+                                            welford_merge(lds_q[2], lds_q[3]));
+    }
+}
+
+// A policy run's close in one launch: the fixed-order merges of the policies formed (mode 0:
+// the network's; 1: both; 2: the QR policy's), one block each, and a last block summing the
+// per-tile counts into res->ctr.
+__global__ __launch_bounds__(kMergeGroups) void lsm_policy_close_kernel(
+    const Welford* __restrict__ pn, const Welford* __restrict__ pq, uint64_t tiles,
+    const uint32_t* __restrict__ tile_counts, int mode, PolicyResults* __restrict__ res) {
+    __shared__ Welford lds[kMergeGroups];
+    __shared__ unsigned long long part[3][kMergeGroups / 64];
+    if (blockIdx.x + 1 == gridDim.x) {
+        unsigned long long acc[3] = {0, 0, 0};
+        for (uint64_t i = threadIdx.x; i < tiles; i += blockDim.x)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += tile_counts[(uint64_t)c * tiles + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+            if ((threadIdx.x & 63) == 0) part[c][threadIdx.x >> 6] = acc[c];
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            unsigned long long sum = 0;
+            for (int w = 0; w < kMergeGroups / 64; ++w) sum += part[threadIdx.x][w];
+            res->ctr[threadIdx.x] = sum;
+        }
+        return;
+    }
+    const bool qr = mode == 2 || blockIdx.x == 1;
+    const Welford total = merge_tiles_fixed_order(qr ? pq : pn, tiles, lds);
+    if (threadIdx.x == 0) (qr ? res->qr : res->nn) = total;
+}
+
+// FP64 mode: each path regenerated exactly as the store kernel writes it (the reference's
+// arithmetic) and stopped at each policy's first exercise date; the cash flow is then
+// discounted by one multiplication per date, the order backward induction applies them in,
+// so the QR policy reproduces the backward-induction price bit for bit.
+__global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_f64_kernel(LsmArgs a, PolicyArgs q) {
     extern __shared__ uint8_t sh_has[]; // [m + 1], launched with that much dynamic LDS
     const uint64_t tile = blockIdx.x;
     const uint32_t t = threadIdx.x;
@@ -1299,100 +1395,155 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_kernel(LsmArgs a, P
     if (p < a.n_paths) {
         // t < 1024 = kTilePaths: the block's paths share p >> 32, a scalar here.
         const uint32_t plo = (uint32_t)p, phh = (uint32_t)((tile * kTilePaths) >> 32);
-        if constexpr (FP64) {
-            double sp = a.spot;
-            for (int step = 0; step < a.m && !(ps.n_done && ps.q_done); ++step) {
-                const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
-                sp = sp * exp(a.drift + a.vsd * z);
-                policy_visit(a, q, sh_has, ps, step + 1, sp);
-            }
-        } else {
-            float x = 0.f;
-            const float s0 = (float)a.spot;
-            int step = 0;
-            constexpr int B = 4;
-            for (; step + B <= a.m && !(ps.n_done && ps.q_done); step += B) {
-                float z[B];
-                normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
-#pragma unroll
-                for (int j = 0; j < B; ++j) {
-                    x = x + __builtin_fmaf(a.sig2, z[j], a.mu2);
-                    policy_visit(a, q, sh_has, ps, step + j + 1, (double)(s0 * fast_exp2(x)));
-                }
-            }
-            for (; step < a.m && !(ps.n_done && ps.q_done); ++step) {
-                const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
-                x = x + __builtin_fmaf(a.sig2, z, a.mu2);
-                policy_visit(a, q, sh_has, ps, step + 1, (double)(s0 * fast_exp2(x)));
-            }
+        double sp = a.spot;
+        for (int step = 0; step < a.m && !(ps.n_done && ps.q_done); ++step) {
+            const double z = inv_normal_f64(uniform_open01_f64(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1)));
+            sp = sp * exp(a.drift + a.vsd * z);
+            policy_visit(a, q, sh_has, ps, step + 1, sp);
         }
     }
     const double cn = ps.n_val * disc_pow(a.disc, ps.n_date), cq = ps.q_val * disc_pow(a.disc, ps.q_date);
-    sh_n[t] = cn;
-    sh_q[t] = cq;
-    uint32_t c_dec = ps.dec, c_dis = ps.dis, c_fb = ps.fb;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        c_dec += __shfl_xor(c_dec, off, 64);
-        c_dis += __shfl_xor(c_dis, off, 64);
-        c_fb += __shfl_xor(c_fb, off, 64);
-    }
-    if ((t & 63) == 0) {
-        sh_cnt[0][t >> 6] = c_dec;
-        sh_cnt[1][t >> 6] = c_dis;
-        sh_cnt[2][t >> 6] = c_fb;
-    }
+    policy_tile_close<true, true>(a, q, tile, t, cn, cq, ps.dec, ps.dis, ps.fb);
+}
+
+// FP32 mode, one instance per judge mode (0: the network, the QR policy deciding only outside
+// the table range; 1: both policies; 2: the QR policy alone). The path is the store kernel's
+// FP32 log2-space path. The network's decision is taken in FP32 without branches: the
+// in-the-money test against the strike rounded outward to FP32 (exactly the FP64 test
+// intrinsic > 0), the table coordinate u = s ua + ub by one FMA (range [0, grid - 1]), the
+// interpolated continuation c and the exercise margin phi (s - K) - c K / 10 by FMAs; every
+// date's table entries are loaded (the index clamped into the table, used only in range), so
+// the four dates of a draw block issue their loads before the first decision waits. The QR
+// policy's decisions are policy_visit's FP64 ones, behind a branch (the judge, or a
+// table-range fallback), so its price equals the FP64-decision replays bit for bit. A stopped
+// policy keeps its date and FP32 spot; its cash flow is formed once at the end from the FP64
+// intrinsic and the discount powers backward induction applies.
+template <int MODE, bool PUT>
+__global__ __launch_bounds__(4 * kTileLanes, 8) void lsm_serve_kernel(LsmArgs a, PolicyArgs q) {
+    extern __shared__ uint8_t sh_has[]; // [m + 1], launched with that much dynamic LDS
+    const uint64_t tile = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    for (int i = (int)t; i <= a.m; i += (int)blockDim.x) sh_has[i] = a.has[i];
     __syncthreads();
-    if (t < 3) { // per-tile counts, summed by lsm_count_kernel (no same-address atomics)
-        uint32_t sum = 0;
-        for (int w = 0; w < 16; ++w) sum += sh_cnt[t][w];
-        q.tile_counts[(uint64_t)t * gridDim.x + tile] = sum;
-    }
-    Welford wn = {0, 0.0, 0.0}, wq = {0, 0.0, 0.0};
-    if (t < (uint32_t)kTileLanes) {
+    const uint64_t p = tile * kTilePaths + t;
+    bool nd = MODE == 2, qd = MODE == 0; // stopped
+    int nds = 0, qds = 0;                // stop dates (0: never stopped, no cash flow)
+    float ns = 0.f, qs = 0.f;            // spot at the stop
+    uint32_t dec = 0, dis = 0, fb = 0;
+    if (p < a.n_paths) {
+        const uint32_t plo = (uint32_t)p, phh = (uint32_t)((tile * kTilePaths) >> 32);
+        const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+        const float s0 = (float)a.spot, mu2 = a.mu2, sig2 = a.sig2;
+        const float kedge = q.kedge, phik = q.phik, cscale = q.cont_scale_f;
+        const float ua = q.ua, ub = q.ub, umax = q.umax, gmax = q.gmax;
+        const uint32_t grid = q.grid;
+        const char* table = (const char*)q.table;
+        const int last = a.m;
+        // The QR policy's FP64 decision at date d (lsm.cpp:110-122, as policy_visit).
+        auto qr_exercises = [&](int date, float s) {
+            const double sd = (double)s;
+            const double ex = intrinsic_v(sd, a.strike, a.type);
+            double b[4];
+            laguerre4(sd * a.inv_strike, b);
+            const double* beta = a.coeffs + 4 * date;
+            double cont = 0.0;
+            cont += beta[0] * b[0];
+            cont += beta[1] * b[1];
+            cont += beta[2] * b[2];
+            cont += beta[3] * b[3];
+            return ex > cont;
+        };
+        // One exercise date d < M of the path at spot s.
+        auto visit = [&](int date, float s, float lo, float hi, float u, uint32_t g) {
+            // phi (s - K) > 0 in FP64: s above (call) or below (put) the outward-rounded strike
+            const bool itm = PUT ? s < -kedge : s > kedge;
+            const bool inr = u >= 0.f && u <= umax;
+            bool dn = false;
+            if constexpr (MODE != 2) {
+                const float c = __builtin_fmaf(u - (float)g, hi - lo, lo);
+                const float ex = PUT ? phik - s : s + phik; // phi s - phi K
+                dn = __builtin_fmaf(-c, cscale, ex) > 0.f;
+            }
+            if constexpr (MODE == 0) {
+                // the table decides in range; the QR policy outside it (the fallback)
+                const bool open = itm && !nd;
+                fb += (open && !inr) ? 1u : 0u;
+                if (open && inr && dn) { nd = true; nds = date; ns = s; }
+                if (open && !inr && sh_has[date] && qr_exercises(date, s)) {
+                    nd = true;
+                    nds = date;
+                    ns = s;
+                }
+            } else if constexpr (MODE == 2) {
+                if (itm && !qd && sh_has[date] && qr_exercises(date, s)) {
+                    qd = true;
+                    qds = date;
+                    qs = s;
+                }
+            } else {
+                const bool need_q = itm && (!qd || (!nd && !inr));
+                bool dq = false;
+                if (need_q && sh_has[date]) dq = qr_exercises(date, s);
+                const bool open = itm && !nd;
+                fb += (open && !inr) ? 1u : 0u;
+                dn = inr ? dn : dq;
+                const bool both = open && !qd;
+                dec += both ? 1u : 0u;
+                dis += (both && dn != dq) ? 1u : 0u;
+                if (open && dn) { nd = true; nds = date; ns = s; }
+                if (itm && !qd && dq) { qd = true; qds = date; qs = s; }
+            }
+        };
+        // The table coordinate of spot s at date d and its cell's two entries (the index
+        // clamped into the table so that every lane's loads are in bounds; a 32-bit byte
+        // offset from the table's base).
+        auto locate = [&](int date, float s, float& lo, float& hi, float& u, uint32_t& g) {
+            u = __builtin_fmaf(s, ua, ub);
+            g = (uint32_t)__builtin_fminf(__builtin_fmaxf(u, 0.f), gmax);
+            if constexpr (MODE != 2) {
+                const uint32_t off = ((uint32_t)(date - 1) * grid + g) * 4u;
+                const float* cell = (const float*)(table + off);
+                lo = cell[0];
+                hi = cell[1];
+            } else {
+                lo = hi = 0.f;
+            }
+        };
+        float x = 0.f;
+        int step = 0;
+        constexpr int B = 4;
+        for (; step + B < last && !(nd && qd); step += B) { // dates step+1 .. step+B < M
+            float z[B];
+            normal_block_f32<B>(plo, phh, (uint32_t)step, k0, k1, z);
+            float s[B], lo[B], hi[B], u[B];
+            uint32_t g[B];
 #pragma unroll
-        for (int k = 0; k < kPathsPerLane; ++k) {
-            const uint32_t i = (uint32_t)(k * kTileLanes) + t;
-            if (tile * kTilePaths + i >= a.n_paths) break;
-            const double rk = 1.0 / (double)(k + 1);
-            const double xn = sh_n[i], xq = sh_q[i];
-            wn.n += 1;
-            double d = xn - wn.mean;
-            wn.mean += d * rk;
-            wn.m2 += d * (xn - wn.mean);
-            wq.n += 1;
-            d = xq - wq.mean;
-            wq.mean += d * rk;
-            wq.m2 += d * (xq - wq.mean);
+            for (int j = 0; j < B; ++j) {
+                x = x + __builtin_fmaf(sig2, z[j], mu2);
+                s[j] = s0 * fast_exp2(x);
+                locate(step + j + 1, s[j], lo[j], hi[j], u[j], g[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) visit(step + j + 1, s[j], lo[j], hi[j], u[j], g[j]);
+        }
+        for (; step < last && !(nd && qd); ++step) {
+            const float z = inv_normal_f32(philox_w0(plo, phh, (uint32_t)step, 0u, k0, k1));
+            x = x + __builtin_fmaf(sig2, z, mu2);
+            const float s = s0 * fast_exp2(x);
+            if (step + 1 == last) { // lsm.cpp:76-81: the last date exercises whatever is left
+                if (!nd) { nd = true; nds = last; ns = s; }
+                if (!qd) { qd = true; qds = last; qs = s; }
+            } else {
+                float lo, hi, u;
+                uint32_t g;
+                locate(step + 1, s, lo, hi, u, g);
+                visit(step + 1, s, lo, hi, u, g);
+            }
         }
     }
-    // the 256-lane tree of block_merge_256, run by every wave (waves 4..15 hold zeros)
-    const int lane = t & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        Welford on, oq;
-        on.n = __shfl_down(wn.n, off, 64);
-        on.mean = __shfl_down(wn.mean, off, 64);
-        on.m2 = __shfl_down(wn.m2, off, 64);
-        oq.n = __shfl_down(wq.n, off, 64);
-        oq.mean = __shfl_down(wq.mean, off, 64);
-        oq.m2 = __shfl_down(wq.m2, off, 64);
-        if ((lane & (2 * off - 1)) == 0) {
-            wn = welford_merge(wn, on);
-            wq = welford_merge(wq, oq);
-        }
-    }
-    if (lane == 0 && t < (uint32_t)kTileLanes) {
-        lds_n[t >> 6] = wn;
-        lds_q[t >> 6] = wq;
-    }
-    __syncthreads();
-    if (t == 0) {
-        q.part_nn[tile] = welford_merge(welford_merge(lds_n[0], lds_n[1]),
-                                        welford_merge(lds_n[2], lds_n[3]));
-        q.part_qr[tile] = welford_merge(welford_merge(lds_q[0], lds_q[1]),
-                                        welford_merge(lds_q[2], lds_q[3]));
-    }
+    const double cn = nds ? intrinsic_v((double)ns, a.strike, a.type) * disc_pow(a.disc, nds) : 0.0;
+    const double cq = qds ? intrinsic_v((double)qs, a.strike, a.type) * disc_pow(a.disc, qds) : 0.0;
+    policy_tile_close<MODE != 2, MODE != 0>(a, q, tile, t, cn, cq, dec, dis, fb);
 }
 
 // ---- American Greeks by frozen-policy replay ------------------------------------------------
@@ -1444,7 +1595,7 @@ __device__ __forceinline__ double discounted(const LsmArgs& a, const Stop& st) {
 }
 
 // One path per thread; a block of 1024 threads is one canonical tile, partials rebuilt in
-// the canonical order as lsm_policy_kernel does.
+// the canonical order as policy_tile_close does.
 template <bool FP64>
 __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a, AmGreekArgs g) {
     __shared__ double sh[4][kTilePaths];
@@ -1920,32 +2071,31 @@ constexpr int kMaxLdsDates = 32768;
 size_t has_lds_bytes(int m) { return ((size_t)m + 1 + 15) & ~(size_t)15; }
 
 struct PolicyBufs {
-    float *table, *gx, *gt;
-    Welford *pn, *pq;
-    unsigned long long* ctr;
+    float* table;
+    Welford *pn, *pq; // [tiles] partials
+    PolicyResults* res;
     uint32_t* tile_counts;
 };
 
 size_t policy_bytes(int32_t m, uint32_t grid, uint64_t tiles) {
-    return 3 * align256((size_t)(m - 1) * grid * 4) + 2 * align256((tiles + 1) * sizeof(Welford)) +
-           256 + align256(3 * tiles * 4);
+    return align256((size_t)(m - 1) * grid * 4) + 2 * align256(tiles * sizeof(Welford)) +
+           align256(sizeof(PolicyResults)) + align256(3 * tiles * 4);
 }
 
 PolicyBufs carve_policy(char* p, int32_t m, uint32_t grid, uint64_t tiles) {
     PolicyBufs b;
-    const size_t tb = align256((size_t)(m - 1) * grid * 4), wb = align256((tiles + 1) * sizeof(Welford));
+    const size_t tb = align256((size_t)(m - 1) * grid * 4), wb = align256(tiles * sizeof(Welford));
     b.table = (float*)p;
-    b.gx = (float*)(p + tb);
-    b.gt = (float*)(p + 2 * tb);
-    b.pn = (Welford*)(p + 3 * tb);
-    b.pq = (Welford*)(p + 3 * tb + wb);
-    b.ctr = (unsigned long long*)(p + 3 * tb + 2 * wb);
-    b.tile_counts = (uint32_t*)(p + 3 * tb + 2 * wb + 256);
+    b.pn = (Welford*)(p + tb);
+    b.pq = (Welford*)(p + tb + wb);
+    b.res = (PolicyResults*)(p + tb + 2 * wb);
+    b.tile_counts = (uint32_t*)(p + tb + 2 * wb + align256(sizeof(PolicyResults)));
     return b;
 }
 
-// Table (MFMA evaluation of the network on the grid), then the streaming policy kernel and
-// the two fixed-order merges. Results land in pn[tiles], pq[tiles] and ctr[0..2].
+// Table (MFMA evaluation of the network on the grid), then the streaming policy kernel, the
+// counts and the fixed-order merges of the policies formed (the network's unless mode 2, the
+// QR policy's unless mode 0). Results land in *b.res.
 // mode: 0 the network (QR outside the table range), 1 both policies (the judge), 2 the QR
 // policy alone (no table).
 int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_t grid,
@@ -1954,35 +2104,51 @@ int enqueue_policy(const LsmRun& r, const PolicyBufs& b, const float* w, uint32_
     if (m + 1 > kMaxLdsDates) return MCDX_EINVAL;
     const uint64_t pts = (uint64_t)(m - 1) * grid;
     const double h = (hi - lo) / (double)(grid - 1);
-    if (mode != 2) {
-        hipLaunchKernelGGL(lsm_grid_kernel, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s,
-                           b.gx, b.gt, m, grid, lo, h);
-        int st = mlp_eval_device(w, b.gx, b.gt, pts, b.table, s);
+    if (mode != 2) { // features (4 (x_g - 1), 2 d/M - 1), d = 1..M-1, g < grid
+        const MlpEvalGrid eg{grid, m, lo, h, kFeatScale};
+        int st = mlp_eval_device(w, nullptr, nullptr, pts, b.table, s, &eg);
         if (st) return st;
     }
     PolicyArgs q{b.table, grid, lo, hi, 1.0 / h, 1.0 / r.a.strike, r.a.strike / kTargetScale,
                  b.pn, b.pq, b.tile_counts, mode};
+    // FP32 decisions: the strike rounded outward (up for a put, down for a call), so that
+    // phi s > kedge holds exactly when phi (s - K) > 0 for every FP32 spot s.
+    const double k = r.a.strike;
+    const bool put = r.a.type == 1;
+    float kf = (float)k;
+    if (put && (double)kf < k) kf = nextafterf(kf, INFINITY);
+    if (!put && (double)kf > k) kf = nextafterf(kf, -INFINITY);
+    q.kedge = put ? -kf : kf;
+    q.phik = (float)(put ? k : -k);
+    q.cont_scale_f = (float)(k / kTargetScale);
+    q.ua = (float)(1.0 / (k * h));
+    q.ub = (float)(-lo / h);
+    q.umax = (float)(grid - 1);
+    q.gmax = (float)(grid - 2);
+    const dim3 g((unsigned)r.tiles), blk(kTilePaths);
     if (fp64)
-        hipLaunchKernelGGL(lsm_policy_kernel<true>, dim3((unsigned)r.tiles), dim3(kTilePaths),
-                           has_lds_bytes(m), s, r.a, q);
-    else
-        hipLaunchKernelGGL(lsm_policy_kernel<false>, dim3((unsigned)r.tiles), dim3(kTilePaths),
-                           has_lds_bytes(m), s, r.a, q);
+        hipLaunchKernelGGL(lsm_policy_f64_kernel, g, blk, has_lds_bytes(m), s, r.a, q);
+    else {
+        using ServeKernel = void (*)(LsmArgs, PolicyArgs);
+        static constexpr ServeKernel serve[3][2] = {
+            {lsm_serve_kernel<0, false>, lsm_serve_kernel<0, true>},
+            {lsm_serve_kernel<1, false>, lsm_serve_kernel<1, true>},
+            {lsm_serve_kernel<2, false>, lsm_serve_kernel<2, true>}};
+        hipLaunchKernelGGL(serve[mode][put ? 1 : 0], g, blk, has_lds_bytes(m), s, r.a, q);
+    }
     hipError_t e = launch_check();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(lsm_count_kernel, dim3(1), dim3(256), 0, s, b.tile_counts, r.tiles, b.ctr);
+        hipLaunchKernelGGL(lsm_policy_close_kernel, dim3(mode == 1 ? 3 : 2), dim3(kMergeGroups), 0,
+                           s, b.pn, b.pq, r.tiles, b.tile_counts, mode, b.res);
         e = launch_check();
     }
-    if (e == hipSuccess) e = launch_final_merge(b.pn, nullptr, r.tiles, 1, b.pn + r.tiles, s);
-    if (e == hipSuccess) e = launch_final_merge(b.pq, nullptr, r.tiles, 1, b.pq + r.tiles, s);
     return status_of(e);
 }
 
 // Pinned read-back layout for the policy results.
 struct PolicyHost {
-    Welford qr_lsm, nn, qr;
-    unsigned long long ctr[3];
-    double loss_sum;
+    Welford qr_lsm;
+    PolicyResults r;
 };
 
 // Inception rule (lsm.cpp:141-148) applied to a forward-policy price.
@@ -2161,9 +2327,7 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
     char* h = (char*)ctx->pinned;
     PolicyHost* ph = (PolicyHost*)h;
     e = hipMemcpyAsync(&ph->qr_lsm, r.a.partials + r.tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&ph->qr, pb.pq + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ph->r, pb.res, sizeof(PolicyResults), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(h + h_tbl, pb.table, tbl * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(h + h_loss, losses, (size_t)o->steps * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(h + h_coef, r.a.coeffs, (size_t)(m + 1) * 32, hipMemcpyDeviceToHost, s);
@@ -2176,11 +2340,11 @@ int lsm_learned(double spot, double strike, double rate, double carry, double vo
     const float* hl = (const float*)(h + h_loss);
     mcdx_lsm_learned_result res{};
     res.qr_lsm = with_inception(ph->qr_lsm, spot, strike, type);
-    res.learned = with_inception(ph->nn, spot, strike, type);
-    res.qr_forward = with_inception(ph->qr, spot, strike, type);
-    res.itm_decisions = ph->ctr[0];
-    res.disagreements = ph->ctr[1];
-    res.fallbacks = ph->ctr[2];
+    res.learned = with_inception(ph->r.nn, spot, strike, type);
+    res.qr_forward = with_inception(ph->r.qr, spot, strike, type);
+    res.itm_decisions = ph->r.ctr[0];
+    res.disagreements = ph->r.ctr[1];
+    res.fallbacks = ph->r.ctr[2];
     const int last = o->steps < 100 ? o->steps : 100;
     double ls = 0.0;
     for (int i = o->steps - last; i < o->steps; ++i) ls += (double)hl[i];
@@ -2219,10 +2383,10 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     float* w = (float*)ex;
     const PolicyBufs pb = carve_policy(ex + align256((size_t)kMlpParams * 4), m, grid, tiles);
     const size_t tbl = (size_t)(m - 1) * grid;
-    // Host staging: results, table, then the policy laid out as on the device (coefficients
-    // and flags adjacent, lsm_setup), so one copy uploads both; then the weights. The result
-    // and table areas do not overlap the uploads' sources, so results can come back without
-    // waiting for the uploads first.
+    // Host staging: results, table, then the policy and the weights laid out as on the device
+    // (coefficients, flags and the weights that follow them, lsm_setup), so one copy uploads
+    // all three. The result and table areas do not overlap the upload's source, so results
+    // can come back without waiting for the upload first.
     const size_t pol_span = (size_t)((const char*)(r.a.has + (m + 1)) - (const char*)r.a.coeffs);
     const size_t h_tbl = align256(sizeof(PolicyHost)), h_coef = h_tbl + align256(tbl * 4),
                  h_has = h_coef + (size_t)((const char*)r.a.has - (const char*)r.a.coeffs),
@@ -2236,16 +2400,20 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     for (int i = 0; i <= m; ++i) ((uint8_t*)(h + h_has))[i] = has[i] ? 1 : 0;
     if (judge != 2)
         for (int i = 0; i < kMlpParams; ++i) ((float*)(h + h_w))[i] = weights[i];
-    e = hipMemcpyAsync(r.a.coeffs, h + h_coef, pol_span, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && judge != 2)
-        e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
+    const bool one_copy = (size_t)((const char*)w - (const char*)r.a.coeffs) == h_w - h_coef;
+    if (judge != 2 && one_copy) {
+        e = hipMemcpyAsync(r.a.coeffs, h + h_coef, h_w - h_coef + (size_t)kMlpParams * 4,
+                           hipMemcpyHostToDevice, s);
+    } else {
+        e = hipMemcpyAsync(r.a.coeffs, h + h_coef, pol_span, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && judge != 2)
+            e = hipMemcpyAsync(w, h + h_w, (size_t)kMlpParams * 4, hipMemcpyHostToDevice, s);
+    }
     if (e != hipSuccess) return status_of(e);
     if ((st = enqueue_policy(r, pb, w, grid, lo, hi, precision == MCDX_PREC_FP64, judge, s)))
         return st;
     PolicyHost* ph = (PolicyHost*)h;
-    e = hipMemcpyAsync(&ph->nn, pb.pn + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&ph->qr, pb.pq + tiles, sizeof(Welford), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ph->ctr, pb.ctr, 3 * 8, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(&ph->r, pb.res, sizeof(PolicyResults), hipMemcpyDeviceToHost, s);
     // The table comes back only for the judge's boundary comparison: serving without the
     // judge skips the read-back and the host scan of (m - 1) x grid points.
     if (e == hipSuccess && judge == 1)
@@ -2254,11 +2422,11 @@ int lsm_learned_price(double spot, double strike, double rate, double carry, dou
     if (e != hipSuccess) return status_of(e);
     mcdx_lsm_learned_result res{};
     res.qr_lsm = mcdx_result{NAN, NAN, 0};
-    res.learned = judge != 2 ? with_inception(ph->nn, spot, strike, type) : mcdx_result{NAN, NAN, 0};
-    res.qr_forward = judge ? with_inception(ph->qr, spot, strike, type) : mcdx_result{NAN, NAN, 0};
-    res.itm_decisions = ph->ctr[0];
-    res.disagreements = ph->ctr[1];
-    res.fallbacks = ph->ctr[2];
+    res.learned = judge != 2 ? with_inception(ph->r.nn, spot, strike, type) : mcdx_result{NAN, NAN, 0};
+    res.qr_forward = judge ? with_inception(ph->r.qr, spot, strike, type) : mcdx_result{NAN, NAN, 0};
+    res.itm_decisions = ph->r.ctr[0];
+    res.disagreements = ph->r.ctr[1];
+    res.fallbacks = ph->r.ctr[2];
     res.final_loss = NAN;
     if (judge == 1) {
         boundary_gap((const float*)(h + h_tbl), coeffs, has, m, grid, lo, hi, strike, type,

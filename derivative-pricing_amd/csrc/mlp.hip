@@ -61,101 +61,12 @@ __device__ __forceinline__ float silu_grad(float a) {
     return s * (1.0f + a * (1.0f - s));
 }
 
-// acc[tm][tn] += A(rows tm*32.., k) * Bt(rows tn*32.., k) over k < 64.
-__device__ __forceinline__ void wave_gemm64(const unsigned short* a, const unsigned short* bt,
-                                            f16v acc[2][2]) {
-    const int lane = threadIdx.x & 63;
-    const int row = lane & 31, kg = lane >> 5;
-#pragma unroll
-    for (int kk = 0; kk < kH; kk += 16) {
-        bf8v af[2], bf[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            af[t] = *(const bf8v*)(a + (t * 32 + row) * kLd + kk + 8 * kg);
-            bf[t] = *(const bf8v*)(bt + (t * 32 + row) * kLd + kk + 8 * kg);
-        }
-#pragma unroll
-        for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-            for (int tn = 0; tn < 2; ++tn)
-                acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[tm], bf[tn], acc[tm][tn], 0, 0, 0);
-    }
-}
-
 // Row (m) and column (n) of accumulator element r of tile (tm, tn) for this lane.
 __device__ __forceinline__ int acc_row(int tm, int r) {
     const int lane = threadIdx.x & 63;
     return tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
 __device__ __forceinline__ int acc_col(int tn) { return tn * 32 + ((threadIdx.x & 63) & 31); }
-
-struct MlpLds {
-    alignas(16) unsigned short w2[kH * kLd];  // W2[o][i]
-    alignas(16) unsigned short w2t[kH * kLd]; // W2[o][i] stored as [i][o]
-    float w1[2 * kH], b1[kH], b2[kH], w3[kH], b3;
-    struct Wave {
-        alignas(16) unsigned short h1[kH * kLd];  // H1[s][i]
-        alignas(16) unsigned short h1t[kH * kLd]; // [i][s]
-        alignas(16) unsigned short g2[kH * kLd];  // G2[s][o]; also FP32 H2 staging before backward
-        alignas(16) unsigned short g2t[kH * kLd]; // [o][s]
-        float xs[kH], ts[kH], g3[kH];
-    } wave[2];
-};
-
-__device__ void load_weights(const float* __restrict__ w, MlpLds& L) {
-    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) {
-        const int o = i / kH, k = i % kH;
-        const unsigned short b = f2bf(w[oW2 + i]);
-        L.w2[o * kLd + k] = b;
-        L.w2t[k * kLd + o] = b;
-    }
-    for (int i = threadIdx.x; i < 2 * kH; i += blockDim.x) L.w1[i] = w[oW1 + i];
-    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
-        L.b1[i] = w[oB1 + i];
-        L.b2[i] = w[oB2 + i];
-        L.w3[i] = w[oW3 + i];
-    }
-    if (threadIdx.x == 0) L.b3 = w[oB3];
-}
-
-// Forward of this wave's 64 samples (x, t per lane); returns the network output of this
-// lane's sample and leaves H1 / H1t in LDS and a2 (pre-activation of layer 2) in acc.
-__device__ float forward64(MlpLds& L, MlpLds::Wave& W, float x, float t, f16v acc[2][2]) {
-    const int s = threadIdx.x & 63;
-    W.xs[s] = x;
-    W.ts[s] = t;
-    for (int i = 0; i < kH; ++i) {
-        const float a1 = __builtin_fmaf(L.w1[2 * i], x, __builtin_fmaf(L.w1[2 * i + 1], t, L.b1[i]));
-        const unsigned short h = f2bf(silu(a1));
-        W.h1[s * kLd + i] = h;
-        W.h1t[i * kLd + s] = h;
-    }
-    wave_sync();
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[tm][tn][r] = 0.f;
-    wave_gemm64(W.h1, L.w2, acc);
-    // a2 += b2; stage h2 * w3 per (s, o) in FP32 over the g2 area, then each lane sums its row
-    float* h2w = (float*)W.g2; // [64][65] floats fits in g2 + g2t
-#pragma unroll
-    for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int o = acc_col(tn);
-                acc[tm][tn][r] += L.b2[o];
-                h2w[acc_row(tm, r) * 65 + o] = silu(acc[tm][tn][r]) * L.w3[o];
-            }
-    wave_sync();
-    float y = L.b3;
-    for (int o = 0; o < kH; ++o) y += h2w[s * 65 + o];
-    wave_sync();
-    return y;
-}
 
 struct TrainArgs {
     float* w;          // [kParams] weights (in/out)
@@ -576,21 +487,67 @@ __global__ __launch_bounds__(kTrainLanes) void mlp_train_kernel(TrainArgs a) {
     }
 }
 
-// Forward only, 64 samples per wave, any number of waves; out[i] = C(xs[i], ts[i]).
-__global__ __launch_bounds__(128) void mlp_eval_kernel(const float* __restrict__ w,
+// Forward only: out[i] = C(x_i, t_i) for i < n. A block of four waves evaluates one 64-sample
+// chunk (lane = sample), split as the trainer splits a step: wave q forms hidden units
+// 16q .. 16q+15 of the first layer (weights read through the scalar unit), then accumulator
+// tile (tm, tn) = (q / 2, q % 2) of the 64x64x64 layer-2 product on MFMA (A = the BF16 H1
+// rows in LDS, B = W2 rows read from memory and rounded to BF16 as the trainer rounds them,
+// FP32 accumulation over k in the trainer's order), SiLU and the output-layer products; wave 0
+// then sums each sample's 64 products in a fixed order. The features come from xs, ts, or,
+// with g.grid > 0, from the exercise-date grid: i = (d - 1) grid + k gives
+// (fs (xlo + h k - 1), 2 d / m - 1).
+__global__ __launch_bounds__(256) void mlp_eval_kernel(const float* __restrict__ w,
                                                        const float* __restrict__ xs,
                                                        const float* __restrict__ ts, uint64_t n,
-                                                       float* __restrict__ out) {
-    extern __shared__ unsigned char smem[];
-    MlpLds& L = *(MlpLds*)smem;
-    load_weights(w, L);
-    __syncthreads();
-    MlpLds::Wave& W = L.wave[threadIdx.x >> 6];
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                       MlpEvalGrid g, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) unsigned short h1[kH * kLd]; // H1[s][i]
+    __shared__ float h2w[kH * (kH + 1)]; // h2[s][o] * w3[o], rows padded to 65
+    const uint32_t q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;
     const uint64_t j = i < n ? i : n - 1;
-    f16v acc[2][2];
-    const float y = forward64(L, W, xs[j], ts[j], acc);
-    if (i < n) out[i] = y;
+    float x, t;
+    if (g.grid) {
+        const int date = (int)(j / g.grid) + 1;
+        const uint32_t k = (uint32_t)(j % g.grid);
+        x = (float)(g.feat_scale * (g.xlo + g.h * (double)k - 1.0));
+        t = (float)(2.0 * (double)date / (double)g.m - 1.0);
+    } else {
+        x = xs[j];
+        t = ts[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u += 2) {
+        const int i0 = 16 * (int)q + u;
+        const float a0 = __builtin_fmaf(w[oW1 + 2 * i0], x, __builtin_fmaf(w[oW1 + 2 * i0 + 1], t, w[oB1 + i0]));
+        const float a1 = __builtin_fmaf(w[oW1 + 2 * i0 + 2], x,
+                                        __builtin_fmaf(w[oW1 + 2 * i0 + 3], t, w[oB1 + i0 + 1]));
+        *(uint32_t*)(h1 + lane * kLd + i0) = (uint32_t)f2bf(silu(a0)) | ((uint32_t)f2bf(silu(a1)) << 16);
+    }
+    __syncthreads();
+    const int tm = (int)q >> 1, tn = (int)q & 1;
+    const int row = lane & 31, kg = lane >> 5;
+    f16v acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < kH; kk += 16) {
+        const bf8v af = *(const bf8v*)(h1 + (tm * 32 + row) * kLd + kk + 8 * kg);
+        const float* wr = w + oW2 + (tn * 32 + row) * kH + kk + 8 * kg; // W2[o][k .. k+7]
+        bf8v bfr;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bfr[e] = __builtin_bit_cast(__bf16, f2bf(wr[e]));
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, bfr, acc, 0, 0, 0);
+    }
+    const int o = acc_col(tn);
+    const float b2 = w[oB2 + o], w3 = w[oW3 + o];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) h2w[acc_row(tm, r) * (kH + 1) + o] = silu(acc[r] + b2) * w3;
+    __syncthreads();
+    if (q == 0) {
+        float y = w[oB3];
+        for (int c = 0; c < kH; ++c) y += h2w[lane * (kH + 1) + c];
+        if (i < n) out[i] = y;
+    }
 }
 
 // w[i] from Philox(seed, i): layer-wise scales (see mlp.hpp).
@@ -609,9 +566,6 @@ __global__ __launch_bounds__(256) void mlp_init_kernel(float* w, uint64_t seed, 
     w[i] = val;
 }
 
-namespace {
-size_t lds_bytes() { return sizeof(MlpLds); }
-} // namespace
 
 int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s) {
     hipLaunchKernelGGL(mlp_init_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, w, seed,
@@ -633,13 +587,12 @@ int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys
 }
 
 int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
-                    hipStream_t s) {
+                    hipStream_t s, const MlpEvalGrid* grid) {
     if (n == 0) return MCDX_OK;
-    hipError_t e = hipFuncSetAttribute((const void*)mlp_eval_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes());
-    if (e != hipSuccess) return status_of(e);
-    hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), lds_bytes(),
-                       s, w, xs, ts, n, out);
+    const uint64_t chunks = (n + 63) / 64; // one block per 64 samples
+    if (chunks > 0x7FFFFFFFull) return MCDX_EINVAL;
+    const MlpEvalGrid g = grid ? *grid : MlpEvalGrid{0, 0, 0.0, 0.0, 0.0};
+    hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)chunks), dim3(256), 0, s, w, xs, ts, n, g, out);
     return status_of(launch_check());
 }
 

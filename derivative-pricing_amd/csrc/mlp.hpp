@@ -21,7 +21,16 @@ int mlp_init_device(float* w, uint64_t seed, float y_mean, hipStream_t s);
 int mlp_train_device(float* w, const float* xs, const float* ts, const float* ys, uint64_t n,
                      int steps, float lr, float lr_decay, uint64_t seed, float* loss, float* grad,
                      hipStream_t s, int avg_from = -1);
+// Features of an exercise-date grid, formed in the evaluation kernel: sample
+// i = (d - 1) grid + k (d = 1 .. m - 1, k < grid) has features
+// (feat_scale (xlo + h k - 1), 2 d / m - 1).
+struct MlpEvalGrid {
+    uint32_t grid; // 0: features from xs, ts
+    int32_t m;
+    double xlo, h, feat_scale;
+};
+// out[i] = network(x_i, t_i), i < n, on the MFMA forward kernel.
 int mlp_eval_device(const float* w, const float* xs, const float* ts, uint64_t n, float* out,
-                    hipStream_t s);
+                    hipStream_t s, const MlpEvalGrid* grid = nullptr);
 
 } // namespace mcdx

@@ -162,19 +162,8 @@ __global__ __launch_bounds__(kMergeGroups) void final_merge_kernel(
         base = tile_offsets[c];
         n_tiles = tile_offsets[c + 1] - base;
     }
-    const uint32_t g = threadIdx.x;
-    const uint64_t q = n_tiles / kMergeGroups, rem = n_tiles % kMergeGroups;
-    const uint64_t b = g * q + (g < rem ? g : rem);
-    const uint64_t e = b + q + (g < rem ? 1 : 0);
-    Welford acc = {0, 0.0, 0.0};
-    for (uint64_t t = b; t < e; ++t) acc = welford_merge(acc, partials[base + t]);
-    lds[g] = acc;
-    __syncthreads();
-    for (uint32_t s = 1; s < kMergeGroups; s <<= 1) {
-        if ((g & (2 * s - 1)) == 0) lds[g] = welford_merge(lds[g], lds[g + s]);
-        __syncthreads();
-    }
-    if (g == 0) out[c] = lds[0];
+    const Welford total = merge_tiles_fixed_order(partials + base, n_tiles, lds);
+    if (threadIdx.x == 0) out[c] = total;
 }
 
 hipError_t launch_final_merge(const Welford* partials, const uint64_t* tile_offsets,
