@@ -90,8 +90,11 @@ PEAK_CLOCK_HZ = 2.4e9      # MI355X peak engine clock
 # inverse normal with its erfc, exp and IEEE divisions, 531 VALU instructions per path), the
 # bridge-corrected barrier per wave-path-step, Sobol + bridge QMC (Asian) per wave-point-date
 # and Heston QE (9 strikes) per wave-path-step.
+# C5's learned-policy serving kernel per nominal wave-path-date (2^20 x 50 / 64; a wave runs
+# until its last path stops), from rocprofv3 --pmc ... -- python3 bench.py --only c5.
 ISSUE_QUADS_CONFIG = {"c1_fp64": (564.74, 58.45), "c2_barrier_uo_bb": (91.99, 18.85),
-                      "c2_qmc_sobol_bb_asian": (48.56, 13.88), "c3_heston_qe": (175.65, 34.57)}
+                      "c2_qmc_sobol_bb_asian": (48.56, 13.88), "c3_heston_qe": (175.65, 34.57),
+                      "c5_learned_serving": (88.71, 21.00)}
 
 
 def config_issue(key, units, ms, timing):
@@ -630,6 +633,10 @@ def c5_lsm(mcdx, mcd):
                                "final_loss": d["final_loss"], "pipeline_ms_wall": pipe_ms},
         "learned_serving_fresh_paths": {"price": sp, "se": sse, "ms_wall": serve_ms,
                                         "paths_per_s": n / (serve_ms / 1e3),
+                                        "issue": config_issue(
+                                            "c5_learned_serving", n * 50, serve_ms,
+                                            "wall of the synchronous call: upload, network "
+                                            "table, serving kernel, close, read-back"),
                                         "dev_from_crr_in_se": (sp - crr) / sse,
                                         "dev_from_bermudan_in_se": (sp - berm) / sse,
                                         "note": "out-of-sample exercise policy on fresh paths: "

@@ -88,6 +88,24 @@ def test_qr_forward_replay_reproduces_backward_induction(prec):
     assert s["boundary_gap"] == d["boundary_gap"] and s["boundary_dates"] == d["boundary_dates"]
 
 
+@pytest.mark.parametrize("kind,spot,strike", [(1, 100.0, 99.7), (0, 100.0, 100.3)])
+def test_fp32_serving_in_the_money_test_at_unrepresentable_strikes(kind, spot, strike):
+    # The FP32 serving kernel tests "in the money" against the strike rounded outward to
+    # FP32; at a strike FP32 cannot represent, any spot it misjudged would change that path's
+    # QR decision. The QR policy served alone and alongside the network must still reproduce
+    # the backward-induction price (FP64 decisions) bit for bit, for a put and a call.
+    assert float(np.float32(strike)) != strike
+    c = (spot, strike, 0.06, 0.04 if kind == 0 else 0.0, 0.25, 1.0, kind, 20)
+    n = 20_011
+    d, w, co, has = mcdx.lsm_learned(*c, n, 42, FP32, steps=50)
+    p, se, _, _, _ = mcdx.lsm(*c, n, 42, FP32)
+    assert d["qr_forward"][:2] == (p, se)
+    for judge in (1, 2):
+        s = mcdx.lsm_learned_price(*c, n, 42, FP32, w if judge == 1 else None, 1024,
+                                   d["table_lo"], d["table_hi"], co, has, judge=judge)
+        assert s["qr_forward"][:2] == (p, se)
+
+
 def crr_bermudan_put(spot, dates=50, steps=10_000):
     return O.lib().orc_crr_bermudan(spot, 100, 0.06, 0, 0.2, 1, steps, 1, steps // dates)
 
