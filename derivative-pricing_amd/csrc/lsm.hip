@@ -1630,6 +1630,9 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
                 frozen_visit(a, sh_has, c4, d, s4);
             }
         } else {
+            // frozen_visit's decisions, screened: the date's continuation cubic is formed once
+            // for the five scenarios, each scenario decides by decide_screen's estimate, and
+            // only the lanes it leaves unsure take the exact FP64 decision (the same result)
             float x = 0.f, xu = 0.f, xd = 0.f;
             const float s0 = (float)a.spot;
             for (int step = 0; step < a.m && !all_done(); ++step) {
@@ -1639,11 +1642,31 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
                 xd = xd + __builtin_fmaf(g.sig2_d, z, g.mu2_d);
                 const float e = fast_exp2(x);
                 const int d = step + 1;
-                frozen_visit(a, sh_has, c0, d, (double)(s0 * e));
-                frozen_visit(a, sh_has, c1, d, (double)(g.s0_up * e));
-                frozen_visit(a, sh_has, c2, d, (double)(g.s0_dn * e));
-                frozen_visit(a, sh_has, c3, d, (double)(s0 * fast_exp2(xu)));
-                frozen_visit(a, sh_has, c4, d, (double)(s0 * fast_exp2(xd)));
+                const double sc[5] = {(double)(s0 * e), (double)(g.s0_up * e), (double)(g.s0_dn * e),
+                                      (double)(s0 * fast_exp2(xu)), (double)(s0 * fast_exp2(xd))};
+                Stop* st[5] = {&c0, &c1, &c2, &c3, &c4};
+                if (d == a.m) { // lsm.cpp:76-81
+#pragma unroll
+                    for (int i = 0; i < 5; ++i)
+                        if (!st[i]->done) *st[i] = Stop{true, d, intrinsic_v(sc[i], a.strike, a.type)};
+                } else if (sh_has[d]) { // uniform
+                    const double* beta = a.coeffs + 4 * d;
+                    const ContPoly cp = cont_poly(beta);
+                    bool ex[5], unsure[5], any = false;
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        unsure[i] = !st[i]->done && decide_screen(a, sc[i], cp, &ex[i]);
+                        any = any || unsure[i];
+                    }
+                    if (any) {
+#pragma unroll
+                        for (int i = 0; i < 5; ++i)
+                            if (unsure[i]) ex[i] = decide(a, sc[i], beta);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 5; ++i)
+                        if (!st[i]->done && ex[i]) *st[i] = Stop{true, d, intrinsic_v(sc[i], a.strike, a.type)};
+                }
             }
         }
     }
@@ -1674,20 +1697,22 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_am_greeks_kernel(LsmArgs a
         }
     }
     const int lane = t & 63;
+    if (t < (uint32_t)kTileLanes) { // the 256-lane tree in waves 0..3 (wave-uniform)
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
+        for (int off = 1; off < 64; off <<= 1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            Welford o;
-            o.n = __shfl_down(w[q].n, off, 64);
-            o.mean = __shfl_down(w[q].mean, off, 64);
-            o.m2 = __shfl_down(w[q].m2, off, 64);
-            if ((lane & (2 * off - 1)) == 0) w[q] = welford_merge(w[q], o);
+            for (int q = 0; q < 4; ++q) {
+                Welford o;
+                o.n = __shfl_down(w[q].n, off, 64);
+                o.mean = __shfl_down(w[q].mean, off, 64);
+                o.m2 = __shfl_down(w[q].m2, off, 64);
+                if ((lane & (2 * off - 1)) == 0) w[q] = welford_merge(w[q], o);
+            }
         }
-    }
-    if (lane == 0 && t < (uint32_t)kTileLanes)
+        if (lane == 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) lds[q][t >> 6] = w[q];
+            for (int q = 0; q < 4; ++q) lds[q][t >> 6] = w[q];
+    }
     __syncthreads();
     if (t == 0)
 #pragma unroll
