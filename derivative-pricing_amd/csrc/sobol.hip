@@ -504,17 +504,19 @@ __global__ __launch_bounds__(kTilePaths) void qmc_tile_kernel(QmcArgs a) {
             }
         }
     }
-    // block_merge_256's tree over lanes 0..255 (waves 4..15 hold zeros)
+    // block_merge_256's tree over lanes 0..255, in waves 0..3 (wave-uniform)
     const int lane = t & 63;
+    if (t < (uint32_t)kTileLanes) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        Welford o;
-        o.n = __shfl_down(w.n, off, 64);
-        o.mean = __shfl_down(w.mean, off, 64);
-        o.m2 = __shfl_down(w.m2, off, 64);
-        if ((lane & (2 * off - 1)) == 0) w = welford_merge(w, o);
+        for (int off = 1; off < 64; off <<= 1) {
+            Welford o;
+            o.n = __shfl_down(w.n, off, 64);
+            o.mean = __shfl_down(w.mean, off, 64);
+            o.m2 = __shfl_down(w.m2, off, 64);
+            if ((lane & (2 * off - 1)) == 0) w = welford_merge(w, o);
+        }
+        if (lane == 0) lds4[t >> 6] = w;
     }
-    if (lane == 0 && t < (uint32_t)kTileLanes) lds4[t >> 6] = w;
     __syncthreads();
     if (t == 0)
         a.partials[(uint64_t)blockIdx.y * gridDim.x + tile] =
