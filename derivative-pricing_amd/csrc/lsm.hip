@@ -1453,6 +1453,17 @@ __global__ __launch_bounds__(4 * kTileLanes, 8) void lsm_serve_kernel(LsmArgs a,
             cont += beta[3] * b[3];
             return ex > cont;
         };
+        // The QR policy's decision at a date with coefficients, screened as the date kernel
+        // screens it (decide_screen's estimate, the exact decide() for the lanes it leaves
+        // unsure: the same result as qr_exercises), for the lanes that need it.
+        auto qr_screened = [&](int date, float s, bool need) {
+            const double* beta = a.coeffs + 4 * date;
+            const ContPoly cp = cont_poly(beta);
+            bool ex = false;
+            const bool unsure = need && decide_screen(a, (double)s, cp, &ex);
+            if (unsure) ex = decide(a, (double)s, beta);
+            return need && ex;
+        };
         // One exercise date d < M of the path at spot s.
         auto visit = [&](int date, float s, float lo, float hi, float u, uint32_t g) {
             // phi (s - K) > 0 in FP64: s above (call) or below (put) the outward-rounded strike
@@ -1475,7 +1486,7 @@ __global__ __launch_bounds__(4 * kTileLanes, 8) void lsm_serve_kernel(LsmArgs a,
                     ns = s;
                 }
             } else if constexpr (MODE == 2) {
-                if (itm && !qd && sh_has[date] && qr_exercises(date, s)) {
+                if (sh_has[date] && qr_screened(date, s, itm && !qd)) { // the flag: uniform
                     qd = true;
                     qds = date;
                     qs = s;
@@ -1483,7 +1494,7 @@ __global__ __launch_bounds__(4 * kTileLanes, 8) void lsm_serve_kernel(LsmArgs a,
             } else {
                 const bool need_q = itm && (!qd || (!nd && !inr));
                 bool dq = false;
-                if (need_q && sh_has[date]) dq = qr_exercises(date, s);
+                if (sh_has[date]) dq = qr_screened(date, s, need_q); // the flag: uniform
                 const bool open = itm && !nd;
                 fb += (open && !inr) ? 1u : 0u;
                 dn = inr ? dn : dq;
