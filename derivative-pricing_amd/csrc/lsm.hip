@@ -1407,15 +1407,18 @@ __global__ __launch_bounds__(4 * kTileLanes) void lsm_policy_f64_kernel(LsmArgs 
 }
 
 // FP32 mode, one instance per judge mode (0: the network, the QR policy deciding only outside
-// the table range; 1: both policies; 2: the QR policy alone). The path is the store kernel's
+// the table range; 1: both policies; 2: the QR policy alone) and option type (PUT: type 1).
+// The path is the store kernel's
 // FP32 log2-space path. The network's decision is taken in FP32 without branches: the
 // in-the-money test against the strike rounded outward to FP32 (exactly the FP64 test
 // intrinsic > 0), the table coordinate u = s ua + ub by one FMA (range [0, grid - 1]), the
 // interpolated continuation c and the exercise margin phi (s - K) - c K / 10 by FMAs; every
 // date's table entries are loaded (the index clamped into the table, used only in range), so
 // the four dates of a draw block issue their loads before the first decision waits. The QR
-// policy's decisions are policy_visit's FP64 ones, behind a branch (the judge, or a
-// table-range fallback), so its price equals the FP64-decision replays bit for bit. A stopped
+// policy's decisions are policy_visit's FP64 ones: screened by the date kernel's estimate with
+// the exact FP64 decision for the lanes it leaves unsure (the judge and the QR-alone mode), or
+// exact behind a branch (the table-range fallback), so its price equals the FP64-decision
+// replays bit for bit. A stopped
 // policy keeps its date and FP32 spot; its cash flow is formed once at the end from the FP64
 // intrinsic and the discount powers backward induction applies.
 template <int MODE, bool PUT>
