@@ -104,6 +104,13 @@ def test_fp32_serving_in_the_money_test_at_unrepresentable_strikes(kind, spot, s
         s = mcdx.lsm_learned_price(*c, n, 42, FP32, w if judge == 1 else None, 1024,
                                    d["table_lo"], d["table_hi"], co, has, judge=judge)
         assert s["qr_forward"][:2] == (p, se)
+        if judge == 1:
+            judged = s
+    # the network served alone takes the main run's network decisions, bit for bit
+    u = mcdx.lsm_learned_price(*c, n, 42, FP32, w, 1024, d["table_lo"], d["table_hi"], co, has,
+                               judge=0)
+    assert u["learned"] == judged["learned"] == d["learned"]
+    assert u["fallbacks"] == judged["fallbacks"]
 
 
 def crr_bermudan_put(spot, dates=50, steps=10_000):
